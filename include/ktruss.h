@@ -1,0 +1,187 @@
+/*
+ * ktruss.h -- C ABI of the MI355X k-truss engine (libktruss_hip.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns KT_OK (0) or a negative kt_status; kt_last_error() gives the text.
+ * Host buffers are caller-owned unless an entry point says it allocates
+ * (free those with kt_free).  Thread safety: one plan per host thread.
+ *
+ * The reference (KTrussExplorer) has no FFI; its boundary is the C++ call
+ *     void ktruss(COOGraph* graph, COOGraph* truss, Config config)
+ * (kernel.h:12, defined kernel.cu:9-39), specialised per variant into
+ * Device::{Directed,Undirected}::{COOCSRInput,TiledCOOCSRInput}::ktruss
+ * (kernel.h:34-52).  Each entry point below names the reference code it
+ * replaces.
+ */
+#ifndef KTRUSS_H
+#define KTRUSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KT_ABI_VERSION 1
+
+typedef enum {
+    KT_OK = 0,
+    KT_ERR_ARG = -1,        /* NULL pointer, k < 2, unknown enum value, bad size      */
+    KT_ERR_SELF_LOOP = -2,  /* graph.cu:54 asserts no self-loop; here an error        */
+    KT_ERR_ASYMMETRIC = -3, /* symmetric input lacks the reverse of an entry          */
+    KT_ERR_DUPLICATE = -4,  /* an entry or pair appears twice                         */
+    KT_ERR_RANGE = -5,      /* vertex id >= n, or n >= 2^31                           */
+    KT_ERR_NOMEM = -6,      /* host or device allocation failed                      */
+    KT_ERR_CAPACITY = -7,   /* caller buffer too small; the required size is returned */
+    KT_ERR_IO = -8,         /* file missing or malformed                             */
+    KT_ERR_HIP = -9,        /* a HIP runtime call failed                              */
+    KT_ERR_NO_DEVICE = -10, /* no MI355X (gfx950) device visible                      */
+    KT_ERR_STATE = -11      /* call order violated (e.g. results before a run)        */
+} kt_status;
+
+/* config.h:75-79, 111-114, 137-140 */
+enum { KT_UNDIRECTED = 0, KT_DIRECTED_BY_INDEX = 1, KT_DIRECTED_BY_DEGREE = 2 };
+enum { KT_COOCSR = 0, KT_TILED_COOCSR = 1 };
+enum { KT_RECOUNT_ALL = 0, KT_RECOUNT_AFFECTED = 1 };
+
+/* Decomposition schedule (the reference has one k per run; these govern the
+ * k++ loop added on top).  JUMP: after an exact full count, go straight to
+ * k = min live support + 3 (phases in between delete nothing).  REUSE
+ * (directed, recount-all): a phase that ends on a deletion-free count hands
+ * its exact supports to the next phase. */
+enum { KT_SCHED_JUMP = 1u, KT_SCHED_REUSE = 2u };
+
+/* Input flags for kt_graph_create. */
+enum {
+    KT_INPUT_SYMMETRIC = 0u, /* both (u,v) and (v,u) present, as createCOOFromFile
+                                returns them (graph.cu:37-79)                       */
+    KT_INPUT_PAIRS = 1u      /* each undirected edge exactly once, either direction */
+};
+
+/* One point of the design space.  Fields mirror Config (config.h:163-181);
+ * the last three are this engine's own knobs. */
+typedef struct kt_variant {
+    int32_t directedness;          /* -d                                                  */
+    int32_t format;                /* -f                                                  */
+    int32_t recount;               /* -r                                                  */
+    uint32_t num_edge_remove_iter; /* -e: compact tombstones after the first e iterations */
+    uint32_t block_size;           /* -b: threads per count workgroup (64..1024, x64)    */
+    uint32_t tiles_per_dim;        /* -t: tiles per dimension (tiled format)             */
+    uint32_t num_parallel_tiles;   /* -y: x-tile groups counted in parallel (tiled)      */
+    uint32_t tile_size;            /* vertices per tile; 0 = ceil(n / tiles_per_dim)      */
+    uint32_t chunk;                /* worklist entries per wave task; 0 = default        */
+    uint32_t schedule;             /* KT_SCHED_* bits for kt_plan_decompose              */
+} kt_variant;
+
+/* The reference's defaults (config.h:185-199): directed-by-degree, coocsr,
+ * recount all, -e 0, -b 512, -t 8, -y 8; schedule JUMP|REUSE. */
+void kt_variant_default(kt_variant* v);
+
+/* Counters and times of the last ktruss / decompose run.  B_alg =
+ *   sum over count passes [ sum_{e in R_p} (4 (L_u+L_v) + 24) + 12 |E_p| ]
+ *   + sum over compactions 16 |E_c|
+ * with L the stored lengths of the row segments a merge would scan. */
+typedef struct kt_stats {
+    uint64_t count_passes;
+    uint64_t recount_edges;   /* sum |R_p|                        */
+    uint64_t merge_elems;     /* sum_{e in R_p} (L_u + L_v)       */
+    uint64_t live_in_passes;  /* sum |E_p|                        */
+    uint64_t compactions;
+    uint64_t compact_elems;   /* sum |E_c|                        */
+    uint64_t iterations;      /* peel iterations, all phases      */
+    uint64_t phases;          /* phases run                       */
+    uint32_t k_max;           /* largest truss number             */
+    uint32_t k_last;          /* k of the last phase              */
+    double ms_total;          /* device time of the whole run (hipEvents)        */
+    double ms_count;          /* summed device time of count-kernel launches      */
+    uint64_t count_launches;  /* count-kernel launches timed into ms_count        */
+    double balg_count_bytes;  /* B_alg of the count passes only                   */
+} kt_stats;
+
+typedef struct kt_graph kt_graph; /* validated host-side edge set            */
+typedef struct kt_plan kt_plan;   /* device-resident layout of one variant    */
+
+/* ---- runtime ------------------------------------------------------------ */
+int kt_abi_version(void);
+const char* kt_last_error(void);
+const char* kt_status_string(int status);
+/* Select the device (hipSetDevice) and check it is gfx950.  Optional: the
+ * first plan calls it with device 0. */
+int kt_init(int device);
+void kt_free(void* p);
+
+/* ---- graphs (host) -------------------------------------------------------
+ * Replaces createCOOFromFile / the COOGraph handed to ktruss (kernel.cu:9):
+ * validates what the reference asserts or assumes (no self-loop, graph.cu:54;
+ * symmetric input, kernel_h.cu:24), numbers the undirected edges and keeps the
+ * map from every input entry to its edge. */
+int kt_graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                    uint32_t flags, kt_graph** out);
+void kt_graph_destroy(kt_graph* g);
+/* n, undirected edge count, input entry count, maximum degree. */
+int kt_graph_info(const kt_graph* g, uint32_t* n, uint64_t* m_undirected, uint64_t* m_input,
+                  uint32_t* max_degree);
+
+/* createCOOFromFile (graph.cu:37-79): "dst src w" per line, 1-based.  Returns
+ * malloc'ed arrays (kt_free) of 0-based entries. */
+int kt_read_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst);
+/* writeCOOGraphToFile (graph.cu:96-104). */
+int kt_write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst);
+/* verify (verify.cu:12-40) without the abort: 0 when both COO graphs hold the
+ * same entries, 1 when they differ. */
+int kt_verify_edges(uint64_t m_a, const uint32_t* src_a, const uint32_t* dst_a,
+                    uint64_t m_b, const uint32_t* src_b, const uint32_t* dst_b);
+
+/* ---- synthetic graphs (undirected pairs u < v, sorted, deduplicated,
+ * self-loops removed; arrays malloc'ed, kt_free) ---------------------------- */
+int kt_generate_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                     uint64_t seed, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst);
+/* Chung-Lu with power-law weights of exponent gamma, mean degree avg_degree,
+ * weights capped at max_weight (0 = no cap), plus num_cliques planted cliques
+ * of sizes uniform in [clique_min, clique_max] on seeded vertex sets. */
+int kt_generate_chung_lu(uint32_t n, double gamma, double avg_degree, double max_weight,
+                         uint32_t num_cliques, uint32_t clique_min, uint32_t clique_max,
+                         uint64_t seed, uint64_t* m, uint32_t** src, uint32_t** dst);
+/* Erdos-Renyi G(n, m = n*avg_degree/2) sampled pairs. */
+int kt_generate_erdos_renyi(uint32_t n, double avg_degree, uint64_t seed, uint64_t* m,
+                            uint32_t** src, uint32_t** dst);
+
+/* ---- plans (device) ------------------------------------------------------
+ * Replaces the set-up half of Device::ktruss (kernel_d.cu:11-33) and of the
+ * variant wrappers (kernel_ddc.cu:216-224, kernel_ddt.cu:226-234,
+ * kernel_duc.cu:193-201, kernel_dut.cu:205-213): relabel (kernel.cu:13-20),
+ * orientation (graph.cu:194-237), CSR / tiled build (graph.cu:372-478,
+ * 656-780), device allocation of the peel state (info.cu:22-35).  The layout
+ * stays resident; every run starts from it. */
+int kt_plan_create(const kt_graph* g, const kt_variant* v, kt_plan** out);
+void kt_plan_destroy(kt_plan* p);
+/* Stored entries of the layout (directed: m; undirected: 2m), max row length. */
+int kt_plan_info(const kt_plan* p, uint64_t* entries, uint32_t* max_row);
+
+/* One k, the reference's ./ktruss -k k: the peel loop of
+ * Device::...::ktruss(COOCSRGraph* / TiledCOOCSRGraph*, Info, Config)
+ * (kernel_ddc.cu:116-214, kernel_ddt.cu:125-224, kernel_duc.cu:95-191,
+ * kernel_dut.cu:106-203) and the conversion back to a symmetric COO in the
+ * caller's labels (kernel_ddc.cu:246-250, kernel_d.cu:57-79, kernel.cu:31-37).
+ * out_src/out_dst receive 2*|T_k| entries; *out_m is set even when cap is too
+ * small (then KT_ERR_CAPACITY). */
+int kt_plan_ktruss(kt_plan* p, uint32_t k, uint32_t* out_src, uint32_t* out_dst, uint64_t cap,
+                   uint64_t* out_m, kt_stats* stats);
+
+/* Full decomposition (k++ loop over the same peel) on the device-resident
+ * layout; truss numbers stay on the device.  stats->ms_total is the hipEvent
+ * time from the resident oriented layout to the resident truss numbers. */
+int kt_plan_decompose(kt_plan* p, kt_stats* stats);
+/* Copy the last decomposition's truss numbers: one per input entry of the
+ * graph the plan was made from (count = m_input). */
+int kt_plan_truss_numbers(kt_plan* p, uint32_t* tau, uint64_t count);
+/* Per-iteration trace of the last run: (k, iteration, undirected edges
+ * deleted, entries recounted) quadruples as uint64; returns the number of
+ * records in *len (all of them even when cap is smaller). */
+int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

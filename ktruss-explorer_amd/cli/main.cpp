@@ -1,0 +1,214 @@
+// main.cpp -- the ktruss command-line tool on the MI355X engine.
+//
+// Flow of the reference's main (main.cu:13-53): set up (read graph), print
+// the configuration, compute, optionally write the result, verify it against
+// the -c file, clean up -- with the same printed labels.  The computation
+// goes through the C ABI (include/ktruss.h) only.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "config.h"
+#include "ktruss.h"
+#include "timer.h"
+
+using namespace cli;
+
+namespace {
+
+int report(int rc, const char* what) {
+    std::fprintf(stderr, "%s failed: %s\n", what, kt_last_error());
+    return rc;
+}
+
+std::vector<std::string> split(const char* s, char sep) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (const char* p = s; *p; ++p) {
+        if (*p == sep) {
+            out.push_back(cur);
+            cur.clear();
+        } else {
+            cur += *p;
+        }
+    }
+    out.push_back(cur);
+    return out;
+}
+
+// Synthetic input: returns unique pairs (KT_INPUT_PAIRS).
+int make_synthetic(const char* spec, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst) {
+    auto f = split(spec, ':');
+    if (f[0] == "rmat" && f.size() == 4) {
+        return kt_generate_rmat((uint32_t)std::stoul(f[1]), (uint32_t)std::stoul(f[2]), 0.57, 0.19,
+                                0.19, std::stoull(f[3]), n, m, src, dst);
+    }
+    if (f[0] == "er" && f.size() == 4) {
+        *n = (uint32_t)std::stoul(f[1]);
+        return kt_generate_erdos_renyi(*n, std::stod(f[2]), std::stoull(f[3]), m, src, dst);
+    }
+    if (f[0] == "cl" && (f.size() == 5 || f.size() == 9)) {
+        *n = (uint32_t)std::stoul(f[1]);
+        double maxw = 0;
+        uint32_t cliques = 0, cmin = 0, cmax = 0;
+        if (f.size() == 9) {
+            maxw = std::stod(f[5]);
+            cliques = (uint32_t)std::stoul(f[6]);
+            cmin = (uint32_t)std::stoul(f[7]);
+            cmax = (uint32_t)std::stoul(f[8]);
+        }
+        return kt_generate_chung_lu(*n, std::stod(f[2]), std::stod(f[3]), maxw, cliques, cmin, cmax,
+                                    std::stoull(f[4]), m, src, dst);
+    }
+    std::fprintf(stderr, "Unrecognized -s spec: %s\n", spec);
+    return KT_ERR_ARG;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Config c;
+    const int prc = parseArgs(argc, argv, c);
+    if (prc == 1) return 0;
+    if (prc) return prc;
+    if (c.processor == CPU) {
+        std::fprintf(stderr, "-p cpu: this build runs the peel on the GPU only\n");
+        return 2;
+    }
+    Timer timer(c.verbosity >= 1);
+
+    // Set up
+    timer.printAndStart("Setting up\n");
+    uint32_t n = 0;
+    uint64_t m = 0;
+    uint32_t *src = nullptr, *dst = nullptr;
+    uint32_t flags = KT_INPUT_SYMMETRIC;
+    int rc;
+    if (c.synthetic) {
+        rc = make_synthetic(c.synthetic, &n, &m, &src, &dst);
+        flags = KT_INPUT_PAIRS;
+    } else {
+        rc = kt_read_tsv(c.graphFileName, &n, &m, &src, &dst);
+    }
+    if (rc) return report(1, "reading the graph");
+    kt_graph* graph = nullptr;
+    if ((rc = kt_graph_create(n, m, src, dst, flags, &graph))) return report(1, "kt_graph_create");
+    uint64_t m_und = 0, m_in = 0;
+    kt_graph_info(graph, nullptr, &m_und, &m_in, nullptr);
+    c.numNodes = n;
+    c.numEdges = (unsigned)(2 * m_und);
+    timer.stopAndPrint("    Set up time");
+    if (c.verbosity >= 1) printConfig(c);
+
+    // Compute
+    Timer total(c.verbosity >= 1);
+    total.printAndStart("Computing\n");
+    kt_variant v;
+    kt_variant_default(&v);
+    v.directedness = c.directedness;
+    v.format = c.inputFormat;
+    v.recount = c.recount;
+    v.num_edge_remove_iter = c.numEdgeRemoveIter;
+    v.block_size = c.blockSize;
+    v.tiles_per_dim = c.numTiles;
+    v.num_parallel_tiles = c.numParallelTiles;
+    Timer t(c.verbosity >= 1);
+    t.start();
+    kt_plan* plan = nullptr;
+    if ((rc = kt_plan_create(graph, &v, &plan))) return report(1, "kt_plan_create");
+    t.stopAndPrint("    Relabel, orient, build COO/CSR and copy to GPU");
+
+    std::vector<uint32_t> out_src(2 * m_und + 1), out_dst(2 * m_und + 1), tau;
+    uint64_t out_m = 0;
+    kt_stats st{};
+    for (unsigned i = 0; i < c.numWarmupRuns + c.numTimedRuns; ++i) {
+        t.printAndStart("    Performing K-truss\n");
+        if (c.decompose) rc = kt_plan_decompose(plan, &st);
+        else rc = kt_plan_ktruss(plan, c.k, out_src.data(), out_dst.data(), out_src.size(), &out_m, &st);
+        if (rc) return report(1, c.decompose ? "kt_plan_decompose" : "kt_plan_ktruss");
+        if (c.verbosity >= 2) {
+            uint64_t len = 0;
+            kt_plan_trace(plan, nullptr, 0, &len);
+            std::vector<uint64_t> tr(4 * len);
+            kt_plan_trace(plan, tr.data(), len, &len);
+            for (uint64_t r = 0; r < len; ++r) {
+                std::printf("        k = %llu, iteration %llu\n", (unsigned long long)tr[4 * r],
+                            (unsigned long long)tr[4 * r + 1]);
+                std::printf("            # edges recounted = %llu, # edges deleted = %llu\n",
+                            (unsigned long long)tr[4 * r + 3], (unsigned long long)tr[4 * r + 2]);
+            }
+            Timer::print(st.ms_count, "            Triangle counting time (all passes)");
+        }
+        if (c.verbosity == 0 && i >= c.numWarmupRuns) {
+            printConfigAsCSV(c);
+            std::printf("%f ms\n", st.ms_total);
+        }
+        if (c.verbosity >= 1) {
+            Timer::print(st.ms_total, "        Total K-truss time", GREEN);
+            if (c.decompose) std::printf("        k_max = %u, phases = %llu, iterations = %llu\n", st.k_max,
+                                         (unsigned long long)st.phases, (unsigned long long)st.iterations);
+        }
+    }
+    if (c.decompose) {
+        tau.resize(m_in);
+        if ((rc = kt_plan_truss_numbers(plan, tau.data(), m_in))) return report(1, "kt_plan_truss_numbers");
+        // the k-truss for verification: entries with truss number >= k, both directions
+        out_m = 0;
+        for (uint64_t e = 0; e < m_in; ++e) {
+            if (tau[e] < c.k) continue;
+            out_src[out_m] = src[e];
+            out_dst[out_m] = dst[e];
+            ++out_m;
+            if (flags == KT_INPUT_PAIRS) {
+                out_src[out_m] = dst[e];
+                out_dst[out_m] = src[e];
+                ++out_m;
+            }
+        }
+    }
+    total.stopAndPrint("    Total time");
+
+    // Write result
+    if (c.outFileName) {
+        if (c.verbosity >= 1) std::printf("Writing result to file: %s\n", c.outFileName);
+        if (c.decompose) {
+            FILE* fp = std::fopen(c.outFileName, "w");
+            if (!fp) return report(1, "opening the output file");
+            for (uint64_t e = 0; e < m_in; ++e) std::fprintf(fp, "%u\t%u\t%u\n", dst[e] + 1, src[e] + 1, tau[e]);
+            std::fclose(fp);
+        } else if ((rc = kt_write_tsv(c.outFileName, out_m, out_src.data(), out_dst.data()))) {
+            return report(1, "kt_write_tsv");
+        }
+    }
+
+    // Verify result (verify.cu:12-40)
+    int status = 0;
+    if (std::strcmp(c.compareFileName, "none") != 0) {
+        timer.printAndStart("Verifying result\n");
+        uint32_t rn = 0;
+        uint64_t rm = 0;
+        uint32_t *rs = nullptr, *rd = nullptr;
+        if ((rc = kt_read_tsv(c.compareFileName, &rn, &rm, &rs, &rd))) return report(1, "reading the compare file");
+        const int diff = kt_verify_edges(out_m, out_src.data(), out_dst.data(), rm, rs, rd);
+        kt_free(rs);
+        kt_free(rd);
+        if (diff != 0) {
+            std::printf("    FAILED\n");
+            status = 1;
+        } else if (c.verbosity >= 1) {
+            std::printf("    PASSED\n");
+        }
+        timer.stopAndPrint("    Verification time");
+    }
+
+    // Free memory
+    timer.printAndStart("Cleaning up\n");
+    kt_plan_destroy(plan);
+    kt_graph_destroy(graph);
+    kt_free(src);
+    kt_free(dst);
+    timer.stopAndPrint("    Clean up time");
+    return status;
+}

@@ -1,0 +1,187 @@
+// kt_capi.cpp -- the extern "C" boundary declared in include/ktruss.h.
+// Every entry point converts C++ exceptions into status codes; nothing
+// throws across the ABI.
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kt_host.h"
+#include "kt_internal.h"
+#include "kt_plan.h"
+#include "ktruss.h"
+
+#define KT_GUARD(...)                                                     \
+    try {                                                                 \
+        __VA_ARGS__                                                       \
+    } catch (const std::bad_alloc&) {                                     \
+        return kt::fail(KT_ERR_NOMEM, "host allocation failed");          \
+    } catch (const std::exception& ex) {                                  \
+        return kt::fail(KT_ERR_STATE, ex.what());                         \
+    }
+
+extern "C" {
+
+int kt_abi_version(void) { return KT_ABI_VERSION; }
+
+const char* kt_last_error(void) { return kt::last_error(); }
+
+const char* kt_status_string(int status) {
+    switch (status) {
+        case KT_OK: return "ok";
+        case KT_ERR_ARG: return "invalid argument";
+        case KT_ERR_SELF_LOOP: return "self-loop";
+        case KT_ERR_ASYMMETRIC: return "asymmetric input";
+        case KT_ERR_DUPLICATE: return "duplicate entry";
+        case KT_ERR_RANGE: return "vertex id out of range";
+        case KT_ERR_NOMEM: return "out of memory";
+        case KT_ERR_CAPACITY: return "buffer too small";
+        case KT_ERR_IO: return "i/o error";
+        case KT_ERR_HIP: return "HIP error";
+        case KT_ERR_NO_DEVICE: return "no gfx950 device";
+        case KT_ERR_STATE: return "invalid state";
+        default: return "unknown status";
+    }
+}
+
+int kt_init(int device) { KT_GUARD(return kt::init_device(device);) }
+
+void kt_free(void* p) { std::free(p); }
+
+void kt_variant_default(kt_variant* v) {
+    if (!v) return;
+    std::memset(v, 0, sizeof(*v));
+    v->directedness = KT_DIRECTED_BY_DEGREE;  // config.h:192-198
+    v->format = KT_COOCSR;
+    v->recount = KT_RECOUNT_ALL;
+    v->num_edge_remove_iter = 0;
+    v->block_size = 512;
+    v->tiles_per_dim = 8;
+    v->num_parallel_tiles = 8;
+    v->tile_size = 0;
+    v->chunk = 0;
+    v->schedule = KT_SCHED_JUMP | KT_SCHED_REUSE;
+}
+
+int kt_graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                    uint32_t flags, kt_graph** out) {
+    KT_GUARD(return kt::graph_create(n, m, src, dst, flags, out);)
+}
+
+void kt_graph_destroy(kt_graph* g) { kt::graph_release(g); }
+
+int kt_graph_info(const kt_graph* g, uint32_t* n, uint64_t* m_undirected, uint64_t* m_input,
+                  uint32_t* max_degree) {
+    if (!g) return kt::fail(KT_ERR_ARG, "graph is NULL");
+    if (n) *n = g->n;
+    if (m_undirected) *m_undirected = g->m;
+    if (m_input) *m_input = g->m_input;
+    if (max_degree) *max_degree = g->max_degree;
+    return KT_OK;
+}
+
+int kt_read_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst) {
+    KT_GUARD(return kt::read_tsv(path, n, m, src, dst);)
+}
+
+int kt_write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst) {
+    KT_GUARD(return kt::write_tsv(path, m, src, dst);)
+}
+
+int kt_verify_edges(uint64_t m_a, const uint32_t* src_a, const uint32_t* dst_a, uint64_t m_b,
+                    const uint32_t* src_b, const uint32_t* dst_b) {
+    if ((m_a && (!src_a || !dst_a)) || (m_b && (!src_b || !dst_b)))
+        return kt::fail(KT_ERR_ARG, "NULL edge arrays");
+    KT_GUARD(return kt::verify_edges(m_a, src_a, dst_a, m_b, src_b, dst_b);)
+}
+
+int kt_generate_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                     uint64_t seed, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst) {
+    KT_GUARD(return kt::generate_rmat(scale, edge_factor, a, b, c, seed, n, m, src, dst);)
+}
+
+int kt_generate_chung_lu(uint32_t n, double gamma, double avg_degree, double max_weight,
+                         uint32_t num_cliques, uint32_t clique_min, uint32_t clique_max,
+                         uint64_t seed, uint64_t* m, uint32_t** src, uint32_t** dst) {
+    KT_GUARD(return kt::generate_chung_lu(n, gamma, avg_degree, max_weight, num_cliques,
+                                          clique_min, clique_max, seed, m, src, dst);)
+}
+
+int kt_generate_erdos_renyi(uint32_t n, double avg_degree, uint64_t seed, uint64_t* m,
+                            uint32_t** src, uint32_t** dst) {
+    KT_GUARD(return kt::generate_erdos_renyi(n, avg_degree, seed, m, src, dst);)
+}
+
+int kt_plan_create(const kt_graph* g, const kt_variant* v, kt_plan** out) {
+    if (!g || !out) return kt::fail(KT_ERR_ARG, "graph or out is NULL");
+    *out = nullptr;
+    kt_variant dv;
+    kt_variant_default(&dv);
+    KT_GUARD({
+        auto* p = new kt_plan();
+        const int rc = p->impl.create(g, v ? v : &dv);
+        if (rc) {
+            delete p;
+            return rc;
+        }
+        *out = p;
+        return KT_OK;
+    })
+}
+
+void kt_plan_destroy(kt_plan* p) { delete p; }
+
+int kt_plan_info(const kt_plan* p, uint64_t* entries, uint32_t* max_row) {
+    if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
+    if (entries) *entries = p->impl.E0;
+    if (max_row) *max_row = p->impl.max_row;
+    return KT_OK;
+}
+
+int kt_plan_ktruss(kt_plan* p, uint32_t k, uint32_t* out_src, uint32_t* out_dst, uint64_t cap,
+                   uint64_t* out_m, kt_stats* stats) {
+    if (!p || !out_m) return kt::fail(KT_ERR_ARG, "plan or out_m is NULL");
+    KT_GUARD({
+        std::vector<uint32_t> s, d;
+        const int rc = p->impl.ktruss(k, s, d);
+        if (rc) return rc;
+        if (stats) *stats = p->impl.stats;
+        *out_m = s.size();
+        if (s.size() > cap) return kt::fail(KT_ERR_CAPACITY, "output needs " + std::to_string(s.size()) + " entries");
+        if (!s.empty() && (!out_src || !out_dst)) return kt::fail(KT_ERR_ARG, "output arrays NULL");
+        if (!s.empty()) {
+            std::memcpy(out_src, s.data(), s.size() * 4);
+            std::memcpy(out_dst, d.data(), d.size() * 4);
+        }
+        return KT_OK;
+    })
+}
+
+int kt_plan_decompose(kt_plan* p, kt_stats* stats) {
+    if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
+    KT_GUARD({
+        const int rc = p->impl.decompose();
+        if (rc) return rc;
+        if (stats) *stats = p->impl.stats;
+        return KT_OK;
+    })
+}
+
+int kt_plan_truss_numbers(kt_plan* p, uint32_t* tau, uint64_t count) {
+    if (!p || (count && !tau)) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return p->impl.truss_numbers(tau, count);)
+}
+
+int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len) {
+    if (!p || !len) return kt::fail(KT_ERR_ARG, "NULL argument");
+    const auto& tr = p->impl.trace;
+    *len = tr.size();
+    if (recs) {
+        const uint64_t c = std::min<uint64_t>(cap, tr.size());
+        for (uint64_t i = 0; i < c; ++i)
+            for (int j = 0; j < 4; ++j) recs[4 * i + j] = tr[i][j];
+    }
+    return KT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,730 @@
+// kt_device.hip -- gfx950 kernels of the k-truss peel.
+//
+// Wave64 throughout.  The support count is the hot kernel; everything else
+// streams the entry arrays once (coalesced, 4 entries per lane).
+//
+// Support count (replaces count_triangles_kernel, kernel_ddc.cu:11-54,
+// kernel_ddt.cu:11-63, kernel_duc.cu:11-54, kernel_dut.cu:11-65).  A wave owns
+// a task of `chunk` consecutive worklist items.  Lanes load the metadata of 64
+// items at once; the wave then walks the items in order.  For item e = (u,v)
+// the row of u is staged once in the wave's LDS slice (reused by every
+// following item of the same row) and the row of v is streamed from HBM 64
+// entries at a time; each lane binary-searches its entry in the staged row and
+// the wave counts hits with ballot + popcount.  A row longer than the LDS slice
+// is searched in place in HBM instead; when u's candidate range is much
+// shorter than v's row, the roles flip and u's entries are searched in v's row.
+//   directed   : triangle u<v<w found on (u,v) credits (u,v) and (u,w) in LDS
+//                (flushed with one atomic per entry) and (v,w) with one global
+//                atomic -- the three credits of kernel_ddc.cu:42-50.
+//   undirected : |N(u) n N(v)| capped at k-2 with early exit, plain store
+//                (kernel_duc.cu:27-50); tiled: capped partial per x-tile class,
+//                atomically summed (kernel_dut.cu:57-61).
+// Blocks are remapped so each XCD gets a contiguous range of tasks (and so of
+// rows), keeping a row's data and its support counters in one XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include "kt_internal.h"
+
+namespace kt {
+namespace {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kNumXcd = 8;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nb) {
+    // Workgroup bid runs on XCD bid % 8; hand XCD x a contiguous block range.
+    const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    const uint32_t x = bid % kNumXcd, i = bid / kNumXcd;
+    return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t readlane(uint32_t x, uint32_t l) {
+    return __builtin_amdgcn_readlane(x, l);
+}
+
+// LDS operations of one wave complete in issue order; this keeps the
+// compiler from moving LDS accesses across the point where lanes start
+// reading what other lanes wrote.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// First index in [lo, hi) whose value (dead bit masked) is >= x.
+__device__ __forceinline__ uint32_t lower_bound_masked(const uint32_t* a, uint32_t lo,
+                                                       uint32_t hi, uint32_t x) {
+    uint32_t len = hi - lo;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if ((a[lo + half] & KT_VMASK) < x) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// Wave-uniform lower bound in an HBM row; lanes split the range 64-way per
+// step so the search takes log64(len) dependent loads.
+__device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint32_t hi,
+                                                     uint32_t x, uint32_t lane) {
+    while (hi - lo > kWave) {
+        const uint32_t step = (hi - lo + kWave - 1) / kWave;
+        const uint32_t idx = lo + lane * step;
+        const bool below = idx < hi && (a[idx] & KT_VMASK) < x;
+        const uint32_t nb = (uint32_t)__popcll(ballot(below));  // lanes 0..nb-1 are below
+        const uint32_t nlo = nb == 0 ? lo : lo + (nb - 1) * step + 1;
+        const uint32_t nhi = lo + nb * step < hi ? lo + nb * step : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    const uint32_t idx = lo + lane;
+    const bool below = idx < hi && (a[idx] & KT_VMASK) < x;
+    return lo + (uint32_t)__popcll(ballot(below));
+}
+
+struct RowState {
+    uint32_t u;      // staged row (0xFFFFFFFF none)
+    uint32_t ub;     // row start in HBM
+    uint32_t ulen;   // stored length
+    uint32_t umax;   // largest value in the row (dead bit masked)
+    bool in_lds;
+};
+
+template <bool DIR>
+__device__ void flush_row(const RowState& r, const uint32_t* s_cnt, uint32_t* sup, uint32_t lane) {
+    if (!DIR || !r.in_lds || r.u == 0xFFFFFFFFu) return;
+    wave_lds_sync();
+    for (uint32_t j = lane; j < r.ulen; j += kWave) {
+        const uint32_t c = s_cnt[j];
+        if (c) atomicAdd(sup + r.ub + j, c);
+    }
+}
+
+template <bool DIR>
+__device__ void stage_row(RowState& r, uint32_t u, const uint32_t* ptr, const uint32_t* dst,
+                          uint32_t* s_val, uint32_t* s_cnt, uint32_t cap_lds, uint32_t lane) {
+    r.u = u;
+    r.ub = ptr[u];
+    r.ulen = ptr[u + 1] - r.ub;
+    r.in_lds = r.ulen <= cap_lds;
+    if (r.in_lds) {
+        wave_lds_sync();  // previous readers of the slice are done
+        for (uint32_t j = lane; j < r.ulen; j += kWave) {
+            s_val[j] = dst[r.ub + j];
+            if (DIR) s_cnt[j] = 0;
+        }
+        wave_lds_sync();
+        r.umax = r.ulen ? (s_val[r.ulen - 1] & KT_VMASK) : 0;
+    } else {
+        r.umax = r.ulen ? (dst[r.ub + r.ulen - 1] & KT_VMASK) : 0;
+    }
+}
+
+// Next x-tile of class y (mod Y) at or after tile t.
+__device__ __forceinline__ uint32_t next_class_tile(uint32_t t, uint32_t y, uint32_t Y) {
+    const uint32_t m = t % Y;
+    return m == y ? t : t + (y + Y - m) % Y;
+}
+
+// Directed entry e = (u, v): credits of every triangle u < v < w.
+// u's candidates are row positions [lo, ulen) (the entries after v).
+template <bool TILED>
+__device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, uint32_t lo,
+                                        uint32_t vb, uint32_t vlen, uint32_t* s_val,
+                                        uint32_t* s_cnt, uint32_t y, uint32_t lane) {
+    const uint32_t* dst = a.dst;
+    uint32_t* sup = a.sup;
+    const uint32_t ulen = r.ulen;
+    if (lo >= ulen || vlen == 0) return 0;
+    const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
+    const uint32_t umin = urow[lo] & KT_VMASK;
+    uint32_t c = 0;
+
+    if (!TILED && (ulen - lo) * a.query_factor < vlen) {
+        // Query mode: u's candidates searched in v's row (HBM).
+        const uint32_t vmax = dst[vb + vlen - 1] & KT_VMASK;
+        for (uint32_t p0 = lo; p0 < ulen; p0 += kWave) {
+            const uint32_t p = p0 + lane;
+            const uint32_t x = p < ulen ? urow[p] : KT_DEAD;
+            bool hit = false;
+            uint32_t j = 0;
+            if (!(x & KT_DEAD) && x <= vmax) {
+                j = lower_bound_masked(dst + vb, 0, vlen, x);
+                hit = j < vlen && dst[vb + j] == x;
+            }
+            if (hit) {
+                if (r.in_lds) atomicAdd(s_cnt + p, 1u);
+                else atomicAdd(sup + r.ub + p, 1u);
+                atomicAdd(sup + vb + j, 1u);
+            }
+            c += (uint32_t)__popcll(ballot(hit));
+            if (ballot(p < ulen && (x & KT_VMASK) > vmax)) break;
+        }
+        return c;
+    }
+
+    // Stream mode: v's row streamed, each entry searched among u's candidates.
+    uint32_t j0 = 0;
+    uint32_t jend = vlen;
+    uint32_t t = 0;
+    const uint32_t S = a.tile_size, Y = a.ytiles;
+    if (TILED) {
+        // first segment of class y at or after the tile of umin
+        t = next_class_tile(umin / S, y, Y);
+        j0 = wave_lower_bound(dst + vb, 0, vlen, t * S, lane);
+        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
+    } else if (vlen > 2 * kWave) {
+        j0 = wave_lower_bound(dst + vb, 0, vlen, umin, lane);
+    }
+    while (j0 < vlen) {
+        bool done = false;
+        for (; j0 < jend; j0 += kWave) {
+            const uint32_t j = j0 + lane;
+            const bool inr = j < jend;
+            const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
+            const uint32_t wm = w & KT_VMASK;
+            bool hit = false;
+            uint32_t p = 0;
+            if (inr && !(w & KT_DEAD) && wm <= r.umax) {
+                p = lower_bound_masked(urow, lo, ulen, wm);
+                hit = p < ulen && urow[p] == wm;
+            }
+            if (hit) {
+                if (r.in_lds) atomicAdd(s_cnt + p, 1u);
+                else atomicAdd(sup + r.ub + p, 1u);
+                atomicAdd(sup + vb + j, 1u);
+            }
+            c += (uint32_t)__popcll(ballot(hit));
+            if (ballot(inr && wm > r.umax)) { done = true; break; }
+        }
+        if (!TILED || done) break;
+        // next segment of class y
+        if (jend >= vlen) break;
+        const uint32_t tn = (dst[vb + jend] & KT_VMASK) / S;
+        t = next_class_tile(tn, y, Y);
+        if (t * S > r.umax) break;
+        j0 = t == tn ? jend : wave_lower_bound(dst + vb, jend, vlen, t * S, lane);
+        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
+    }
+    return c;
+}
+
+// Undirected entry e = (u, v): |N(u) n N(v)| restricted to x-tile class y
+// (tiled), stopping at cap.
+template <bool TILED>
+__device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r, uint32_t vb,
+                                          uint32_t vlen, const uint32_t* s_val, uint32_t y,
+                                          uint32_t lane) {
+    const uint32_t* dst = a.dst;
+    const uint32_t ulen = r.ulen, cap = a.cap;
+    if (ulen == 0 || vlen == 0 || cap == 0) return 0;
+    const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
+    uint32_t c = 0;
+
+    if (!TILED && ulen * a.query_factor < vlen) {
+        const uint32_t vmax = dst[vb + vlen - 1] & KT_VMASK;
+        for (uint32_t p0 = 0; p0 < ulen; p0 += kWave) {
+            const uint32_t p = p0 + lane;
+            const uint32_t x = p < ulen ? urow[p] : KT_DEAD;
+            bool hit = false;
+            if (!(x & KT_DEAD) && x <= vmax) {
+                const uint32_t j = lower_bound_masked(dst + vb, 0, vlen, x);
+                hit = j < vlen && dst[vb + j] == x;
+            }
+            c += (uint32_t)__popcll(ballot(hit));
+            if (c >= cap) return cap;
+            if (ballot(p < ulen && (x & KT_VMASK) > vmax)) break;
+        }
+        return c;
+    }
+
+    const uint32_t umin = urow[0] & KT_VMASK;
+    uint32_t j0 = 0, jend = vlen, t = 0;
+    const uint32_t S = a.tile_size, Y = a.ytiles;
+    if (TILED) {
+        t = next_class_tile(umin / S, y, Y);
+        j0 = wave_lower_bound(dst + vb, 0, vlen, t * S, lane);
+        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
+    } else if (vlen > 2 * kWave) {
+        j0 = wave_lower_bound(dst + vb, 0, vlen, umin, lane);
+    }
+    while (j0 < vlen) {
+        bool done = false;
+        for (; j0 < jend; j0 += kWave) {
+            const uint32_t j = j0 + lane;
+            const bool inr = j < jend;
+            const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
+            const uint32_t wm = w & KT_VMASK;
+            bool hit = false;
+            if (inr && !(w & KT_DEAD) && wm <= r.umax) {
+                const uint32_t p = lower_bound_masked(urow, 0, ulen, wm);
+                hit = p < ulen && urow[p] == wm;
+            }
+            c += (uint32_t)__popcll(ballot(hit));
+            if (c >= cap) return cap;
+            if (ballot(inr && wm > r.umax)) { done = true; break; }
+        }
+        if (!TILED || done) break;
+        if (jend >= vlen) break;
+        const uint32_t tn = (dst[vb + jend] & KT_VMASK) / S;
+        t = next_class_tile(tn, y, Y);
+        if (t * S > r.umax) break;
+        j0 = t == tn ? jend : wave_lower_bound(dst + vb, jend, vlen, t * S, lane);
+        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
+    }
+    return c < cap ? c : cap;
+}
+
+template <bool DIR, bool TILED>
+__global__ __launch_bounds__(1024) void k_count(CountArgs a) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t nw = blockDim.x / kWave;
+    uint32_t* s_val = lds + wid * (DIR ? 2u : 1u) * a.cap_lds;
+    uint32_t* s_cnt = s_val + a.cap_lds;
+    const uint32_t task = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
+    const uint64_t first = (uint64_t)task * a.chunk;
+    if (first >= a.nitems) return;
+    const uint32_t i0 = (uint32_t)first;
+    const uint32_t i1 = a.nitems - i0 < a.chunk ? a.nitems : i0 + a.chunk;
+    const uint32_t y = TILED ? blockIdx.y : 0;
+    const uint32_t* ptr = a.ptr;
+    const uint32_t* dst = a.dst;
+    const uint32_t S = a.tile_size;
+
+    RowState r;
+    r.u = 0xFFFFFFFFu;
+    r.ub = r.ulen = r.umax = 0;
+    r.in_lds = false;
+    unsigned long long acc_merge = 0, acc_rec = 0;
+
+    for (uint32_t base = i0; base < i1; base += kWave) {
+        const uint32_t my = base + lane;
+        uint32_t e = 0, u = 0, v = KT_DEAD, vb = 0, vlen = 0;
+        if (my < i1) {
+            e = a.wl ? a.wl[my] : my;
+            v = dst[e];
+            if (!(v & KT_DEAD)) {
+                u = a.src[e];
+                vb = ptr[v];
+                vlen = ptr[v + 1] - vb;
+            }
+        }
+        uint64_t todo = ballot(my < i1 && !(v & KT_DEAD));
+        while (todo) {
+            const uint32_t l = (uint32_t)__ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t eu = readlane(u, l), ee = readlane(e, l), ev = readlane(v, l);
+            const uint32_t evb = readlane(vb, l), evlen = readlane(vlen, l);
+            if (eu != r.u) {
+                flush_row<DIR>(r, s_cnt, a.sup, lane);
+                stage_row<DIR>(r, eu, ptr, dst, s_val, s_cnt, a.cap_lds, lane);
+            }
+            const uint32_t q = ee - r.ub;  // position of v in u's row
+            if (y == 0) {
+                acc_rec += 1;
+                uint32_t lu = r.ulen;
+                if (TILED && DIR) {
+                    // u's segments from tile(v) on (kernel_ddt.cu:28)
+                    const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
+                    const uint32_t tv = ev / S;
+                    const uint32_t cut = r.in_lds
+                                             ? lower_bound_masked(urow, 0, r.ulen, tv * S)
+                                             : wave_lower_bound(urow, 0, r.ulen, tv * S, lane);
+                    lu = r.ulen - cut;
+                }
+                acc_merge += (unsigned long long)lu + evlen;
+            }
+            if (DIR) {
+                const uint32_t c = count_directed_item<TILED>(a, r, q + 1, evb, evlen, s_val,
+                                                               s_cnt, y, lane);
+                if (c && lane == 0) {
+                    if (r.in_lds) atomicAdd(s_cnt + q, c);
+                    else atomicAdd(a.sup + ee, c);
+                }
+            } else {
+                uint32_t c = 0;
+                if (!a.prefilter || (r.ulen >= a.cap && evlen >= a.cap))
+                    c = count_undirected_item<TILED>(a, r, evb, evlen, s_val, y, lane);
+                if (lane == 0) {
+                    if (TILED && a.ytiles > 1) {
+                        if (c) atomicAdd(a.sup + ee, c);
+                    } else {
+                        a.sup[ee] = c;
+                    }
+                }
+            }
+        }
+    }
+    flush_row<DIR>(r, s_cnt, a.sup, lane);
+    if (lane == 0 && acc_rec) {
+        atomicAdd(&a.ctr->recount_edges, acc_rec);
+        atomicAdd(&a.ctr->merge_elems, acc_merge);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming kernels: grid-stride, 256 threads.
+
+constexpr uint32_t kBlock = 256;
+
+__device__ __forceinline__ uint32_t gsize() { return gridDim.x * blockDim.x; }
+__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+uint32_t grid_for(uint64_t count) {
+    uint64_t b = (count + kBlock - 1) / kBlock;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return (uint32_t)b;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t value, uint64_t count) {
+    for (uint64_t i = gtid(); i < count; i += gsize()) p[i] = value;
+}
+
+__global__ void k_fill_u8(uint8_t* p, uint8_t value, uint64_t count) {
+    for (uint64_t i = gtid(); i < count; i += gsize()) p[i] = value;
+}
+
+__global__ void k_iota_u32(uint32_t* p, uint64_t count) {
+    for (uint64_t i = gtid(); i < count; i += gsize()) p[i] = (uint32_t)i;
+}
+
+// clearIterInfoOnDevice's numTriangles memset (info.cu:75), live entries only.
+__global__ void k_zero_live(const uint32_t* __restrict__ dst, uint32_t* __restrict__ sup,
+                            uint32_t E) {
+    for (uint32_t i = gtid(); i < E; i += gsize())
+        if (!(dst[i] & KT_DEAD)) sup[i] = 0;
+}
+
+// Wave-aggregated increment of a device counter; returns the wave's base.
+__device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, uint32_t lane) {
+    const uint64_t m = ballot(pred);
+    if (!m) return 0;
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// mark_deleted_edges_kernel (kernel_ddc.cu:56-77 and its three twins): an
+// entry whose support is below k-2 dies; its truss number is k-1.
+__global__ void k_mark_deleted(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                               const uint32_t* __restrict__ eid, const uint32_t* __restrict__ sup,
+                               const uint8_t* __restrict__ state, uint8_t* __restrict__ nflag,
+                               uint32_t* __restrict__ tau, const uint32_t* __restrict__ wl,
+                               uint32_t nitems, uint32_t k, int affected, DevCounters* ctr) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t stride = gsize();
+    // uniform trip count so every lane reaches the ballots
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nitems; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool del = false;
+        if (i < nitems) {
+            const uint32_t e = wl ? wl[i] : i;
+            const uint32_t v = dst[e];
+            if (!(v & KT_DEAD) && (!affected || state[e] == KT_DIRECTLY_AFFECTED) &&
+                sup[e] < k - 2) {
+                del = true;
+                dst[e] = v | KT_DEAD;
+                tau[eid[e]] = k - 1;
+                if (affected) {
+                    nflag[src[e]] = KT_DIRECTLY_AFFECTED;
+                    nflag[v] = KT_DIRECTLY_AFFECTED;
+                }
+            }
+        }
+        const uint64_t m = ballot(del);
+        if (m && lane == (uint32_t)__ffsll((long long)m) - 1)
+            atomicAdd(&ctr->ndel, (unsigned int)__popcll(m));
+    }
+}
+
+// mark_directly_affected_edges_kernel (kernel_ddc.cu:79-98, kernel_duc.cu:79-93).
+__global__ void k_mark_directly(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                uint8_t* __restrict__ state, uint8_t* nflag, uint32_t E,
+                                int directed) {
+    for (uint32_t e = gtid(); e < E; e += gsize()) {
+        uint8_t st = KT_NOT_AFFECTED;
+        const uint32_t v = dst[e];
+        if (!(v & KT_DEAD)) {
+            const uint32_t u = src[e];
+            const bool du = nflag[u] == KT_DIRECTLY_AFFECTED;
+            const bool dv = nflag[v] == KT_DIRECTLY_AFFECTED;
+            if (du || dv) {
+                st = KT_DIRECTLY_AFFECTED;
+                // the endpoint that is not DIRECTLY becomes INDIRECTLY
+                if (directed) {
+                    if (!du) nflag[u] = KT_INDIRECTLY_AFFECTED;
+                    else if (!dv) nflag[v] = KT_INDIRECTLY_AFFECTED;
+                }
+            }
+        }
+        state[e] = st;
+    }
+}
+
+// mark_indirectly_affected_edges_kernel (kernel_ddc.cu:100-114) fused with the
+// worklist build: every DIRECTLY / INDIRECTLY live entry is appended and its
+// support zeroed before the next count.
+__global__ void k_build_worklist(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                 uint8_t* __restrict__ state, const uint8_t* __restrict__ nflag,
+                                 uint32_t* __restrict__ sup, uint32_t* __restrict__ wl, uint32_t E,
+                                 int directed, DevCounters* ctr) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < E; base += gsize()) {
+        const uint32_t e = base + threadIdx.x;
+        bool take = false;
+        if (e < E) {
+            const uint32_t v = dst[e];
+            if (!(v & KT_DEAD)) {
+                uint8_t st = state[e];
+                if (directed && st != KT_DIRECTLY_AFFECTED) {
+                    const uint32_t u = src[e];
+                    if (nflag[u] == KT_INDIRECTLY_AFFECTED || nflag[v] == KT_INDIRECTLY_AFFECTED) {
+                        st = KT_INDIRECTLY_AFFECTED;
+                        state[e] = st;
+                    }
+                }
+                take = st != KT_NOT_AFFECTED;
+            }
+        }
+        const uint32_t pos = wave_append(&ctr->wl_size, take, lane);
+        if (take) {
+            wl[pos] = e;
+            sup[e] = 0;
+        }
+    }
+}
+
+__global__ void k_min_live(const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
+                           uint32_t E, DevCounters* ctr) {
+    uint32_t m = 0xFFFFFFFFu;
+    for (uint32_t e = gtid(); e < E; e += gsize())
+        if (!(dst[e] & KT_DEAD) && sup[e] < m) m = sup[e];
+    // wave min via butterfly, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off);
+        m = o < m ? o : m;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && m != 0xFFFFFFFFu) atomicMin(&ctr->min_sup, m);
+}
+
+// ---------------------------------------------------------------------------
+// Stable compaction (removeCOOCSRDeletedEdgesOnDevice, graph.cu:520-546):
+// tile counts -> single-workgroup scan of tile counts -> scatter with in-tile
+// ballot prefix sums -> row pointers read off the per-position prefix.
+constexpr uint32_t kTile = 2048;  // entries per compaction tile (256 threads x 8)
+
+__global__ void k_tile_count(const uint32_t* __restrict__ dst, uint32_t E,
+                             uint32_t* __restrict__ tile_sums) {
+    __shared__ uint32_t s_w[kBlock / kWave];
+    const uint32_t t = blockIdx.x;
+    const uint32_t base = t * kTile;
+    uint32_t c = 0;
+    for (uint32_t i = base + threadIdx.x; i < base + kTile && i < E; i += kBlock)
+        c += !(dst[i] & KT_DEAD);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; ++w) s += s_w[w];
+        tile_sums[t] = s;
+    }
+}
+
+// Exclusive scan of nt tile counts in place by one workgroup of 1024 threads;
+// the total lands in tile_sums[nt] and ctr->live.
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_sums, uint32_t nt,
+                                                     DevCounters* ctr) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t per = (nt + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < b + per && i < nt; ++i) sum += tile_sums[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partials
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
+    for (uint32_t i = b; i < b + per && i < nt; ++i) {
+        const uint32_t c = tile_sums[i];
+        tile_sums[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) {
+        tile_sums[nt] = s_part[1023];
+        ctr->live = s_part[1023];
+    }
+}
+
+__global__ void k_scatter(uint32_t E, const uint32_t* __restrict__ src_in,
+                          const uint32_t* __restrict__ dst_in, const uint32_t* __restrict__ eid_in,
+                          const uint32_t* __restrict__ sup_in, uint32_t* __restrict__ src_out,
+                          uint32_t* __restrict__ dst_out, uint32_t* __restrict__ eid_out,
+                          uint32_t* __restrict__ sup_out, const uint32_t* __restrict__ tile_sums,
+                          uint32_t* __restrict__ xs) {
+    __shared__ uint32_t s_w[kBlock / kWave];
+    const uint32_t t = blockIdx.x;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    uint32_t run = tile_sums[t];
+    for (uint32_t r = 0; r < kTile / kBlock; ++r) {
+        const uint32_t i = t * kTile + r * kBlock + threadIdx.x;
+        const bool inr = i < E;
+        const uint32_t d = inr ? dst_in[i] : KT_DEAD;
+        const bool live = inr && !(d & KT_DEAD);
+        const uint64_t m = ballot(live);
+        if (lane == 0) s_w[wid] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = run;
+        for (uint32_t w = 0; w < wid; ++w) before += s_w[w];
+        uint32_t round = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; ++w) round += s_w[w];
+        const uint32_t pos = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (inr) xs[i] = pos;
+        if (live) {
+            src_out[pos] = src_in[i];
+            dst_out[pos] = d;
+            eid_out[pos] = eid_in[i];
+            sup_out[pos] = sup_in[i];
+        }
+        run += round;
+        __syncthreads();
+    }
+}
+
+__global__ void k_rebuild_ptr(const uint32_t* __restrict__ ptr_in, const uint32_t* __restrict__ xs,
+                              uint32_t* __restrict__ ptr_out, uint32_t n) {
+    for (uint32_t r = gtid(); r <= n; r += gsize()) ptr_out[r] = xs[ptr_in[r]];
+}
+
+__global__ void k_set_tail(uint32_t* xs, uint32_t E, const uint32_t* tile_sums, uint32_t nt) {
+    if (gtid() == 0) xs[E] = tile_sums[nt];
+}
+
+}  // namespace
+
+namespace dev {
+
+hipError_t fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count)), dim3(kBlock), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t fill_u8(uint8_t* p, uint8_t value, uint64_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(count)), dim3(kBlock), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(count)), dim3(kBlock), 0, s, p, count);
+    return hipGetLastError();
+}
+
+hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t block,
+                          hipStream_t s) {
+    if (!a.nitems) return hipSuccess;
+    const uint32_t nw = block / kWave;
+    const uint64_t tasks = (a.nitems + (uint64_t)a.chunk - 1) / a.chunk;
+    const uint32_t blocks = (uint32_t)((tasks + nw - 1) / nw);
+    const size_t lds = (size_t)nw * (directed ? 2 : 1) * a.cap_lds * sizeof(uint32_t);
+    const dim3 grid(blocks, tiled ? a.ytiles : 1);
+    if (directed) {
+        if (tiled) hipLaunchKernelGGL((k_count<true, true>), grid, dim3(block), lds, s, a);
+        else hipLaunchKernelGGL((k_count<true, false>), grid, dim3(block), lds, s, a);
+    } else {
+        if (tiled) hipLaunchKernelGGL((k_count<false, true>), grid, dim3(block), lds, s, a);
+        else hipLaunchKernelGGL((k_count<false, false>), grid, dim3(block), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t zero_live_supports(const uint32_t* dst, uint32_t* sup, uint32_t E, hipStream_t s) {
+    if (!E) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_live, dim3(grid_for(E)), dim3(kBlock), 0, s, dst, sup, E);
+    return hipGetLastError();
+}
+
+hipError_t mark_deleted(uint32_t* dst, const uint32_t* src, const uint32_t* eid,
+                        const uint32_t* sup, const uint8_t* state, uint8_t* nflag,
+                        uint32_t* tau, const uint32_t* wl, uint32_t nitems, uint32_t k,
+                        bool affected, DevCounters* ctr, hipStream_t s) {
+    if (!nitems) return hipSuccess;
+    hipLaunchKernelGGL(k_mark_deleted, dim3(grid_for(nitems)), dim3(kBlock), 0, s, dst, src, eid,
+                       sup, state, nflag, tau, wl, nitems, k, affected ? 1 : 0, ctr);
+    return hipGetLastError();
+}
+
+hipError_t mark_directly(const uint32_t* src, const uint32_t* dst, uint8_t* state, uint8_t* nflag,
+                         uint32_t E, bool directed, hipStream_t s) {
+    if (!E) return hipSuccess;
+    hipLaunchKernelGGL(k_mark_directly, dim3(grid_for(E)), dim3(kBlock), 0, s, src, dst, state,
+                       nflag, E, directed ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* state,
+                          const uint8_t* nflag, uint32_t* sup, uint32_t* wl, uint32_t E,
+                          bool directed, DevCounters* ctr, hipStream_t s) {
+    if (!E) return hipSuccess;
+    hipLaunchKernelGGL(k_build_worklist, dim3(grid_for(E)), dim3(kBlock), 0, s, src, dst, state,
+                       nflag, sup, wl, E, directed ? 1 : 0, ctr);
+    return hipGetLastError();
+}
+
+hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
+                            DevCounters* ctr, hipStream_t s) {
+    if (!E) return hipSuccess;
+    hipLaunchKernelGGL(k_min_live, dim3(grid_for(E)), dim3(kBlock), 0, s, dst, sup, E, ctr);
+    return hipGetLastError();
+}
+
+uint32_t compact_tiles(uint32_t E) { return (E + kTile - 1) / kTile; }
+
+hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_t* src_in,
+                   const uint32_t* dst_in, const uint32_t* eid_in, const uint32_t* sup_in,
+                   uint32_t* ptr_out, uint32_t* src_out, uint32_t* dst_out, uint32_t* eid_out,
+                   uint32_t* sup_out, uint32_t* tile_sums, uint32_t* xs, DevCounters* ctr,
+                   hipStream_t s) {
+    const uint32_t nt = compact_tiles(E);
+    if (nt) {
+        hipLaunchKernelGGL(k_tile_count, dim3(nt), dim3(kBlock), 0, s, dst_in, E, tile_sums);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_sums, nt, ctr);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    if (nt) {
+        hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kBlock), 0, s, E, src_in, dst_in, eid_in,
+                           sup_in, src_out, dst_out, eid_out, sup_out, tile_sums, xs);
+        err = hipGetLastError();
+        if (err != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(k_set_tail, dim3(1), dim3(64), 0, s, xs, E, tile_sums, nt);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_rebuild_ptr, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, s, ptr_in,
+                       xs, ptr_out, n);
+    return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace kt

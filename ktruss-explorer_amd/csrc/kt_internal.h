@@ -1,0 +1,118 @@
+// kt_internal.h -- shared declarations of the MI355X k-truss engine.
+//
+// Device layout of one plan (all arrays in HBM, uint32 unless noted):
+//   ptr[n+1]            row starts of the stored (oriented or symmetric) CSR
+//   src[E], dst[E]      COO+CSR entries (graph.h:40-47); a deleted entry keeps
+//                       its vertex id and gets bit 31 set (KT_DEAD), so a row
+//                       stays sorted under (x & KT_VMASK) and can be binary
+//                       searched with tombstones in place.  The reference
+//                       overwrites dstIdx with UINT_MAX instead (graph.h:8).
+//   eid[E]              pristine position of each entry (truss numbers are
+//                       written at tau[eid])
+//   sup[E]              support (Info.numTriangles, info.h:16)
+//   state[E] (u8)       NOT / DIRECTLY / INDIRECTLY affected (info.h:10-12)
+//   nflag[n] (u8)       per-vertex affected state (Info.nodeAffected)
+//   wl[E]               worklist of entries to recount (affected mode)
+//   tau[E]              truss number per pristine position
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "ktruss.h"
+
+#define KT_DEAD 0x80000000u
+#define KT_VMASK 0x7FFFFFFFu
+#define KT_NOT_AFFECTED 0
+#define KT_DIRECTLY_AFFECTED 1
+#define KT_INDIRECTLY_AFFECTED 2
+
+namespace kt {
+
+// Error plumbing: every failing path records a message for kt_last_error().
+void set_error(int status, const std::string& msg);
+int fail(int status, const std::string& msg);
+
+#define KT_HIP_TRY(call)                                                              \
+    do {                                                                              \
+        hipError_t kt_e_ = (call);                                                    \
+        if (kt_e_ != hipSuccess)                                                      \
+            return ::kt::fail(KT_ERR_HIP, std::string(#call) + ": " +                 \
+                                              hipGetErrorString(kt_e_));              \
+    } while (0)
+
+// Counters the device updates and the host reads back after each iteration.
+struct DevCounters {
+    unsigned long long merge_elems;   // sum of L_u + L_v over recounted entries
+    unsigned long long recount_edges; // recounted entries
+    unsigned int ndel;                // entries deleted by the last mark pass
+    unsigned int wl_size;             // worklist length
+    unsigned int min_sup;             // minimum live support (JUMP)
+    unsigned int live;                // entries kept by the last compaction
+};
+
+// Arguments of the support-count kernels.
+struct CountArgs {
+    const uint32_t* ptr;
+    const uint32_t* src;
+    const uint32_t* dst;
+    uint32_t* sup;
+    const uint32_t* wl;   // nullptr: items are entry positions [0, nitems)
+    uint32_t nitems;
+    uint32_t chunk;       // items per wave task (multiple of 64)
+    uint32_t cap_lds;     // LDS entries per wave for the staged row
+    uint32_t cap;         // k - 2 (undirected early exit)
+    uint32_t tile_size;   // vertices per tile (tiled format)
+    uint32_t ytiles;      // x-tile classes counted in parallel (tiled); 1 otherwise
+    uint32_t query_factor;// query the shorter list when it is this many times shorter
+    uint32_t prefilter;   // undirected COO+CSR stored-length pre-filter (huc:20)
+    DevCounters* ctr;
+};
+
+// Kernel launchers (kt_device.hip).  All asynchronous on `stream`.
+namespace dev {
+hipError_t fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
+hipError_t fill_u8(uint8_t* p, uint8_t value, uint64_t count, hipStream_t s);
+hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s);
+hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t block,
+                          hipStream_t s);
+hipError_t zero_live_supports(const uint32_t* dst, uint32_t* sup, uint32_t E, hipStream_t s);
+hipError_t mark_deleted(uint32_t* dst, const uint32_t* src, const uint32_t* eid,
+                        const uint32_t* sup, const uint8_t* state, uint8_t* nflag,
+                        uint32_t* tau, const uint32_t* wl, uint32_t nitems, uint32_t k,
+                        bool affected, DevCounters* ctr, hipStream_t s);
+hipError_t mark_directly(const uint32_t* src, const uint32_t* dst, uint8_t* state, uint8_t* nflag,
+                         uint32_t E, bool directed, hipStream_t s);
+// Marks INDIRECTLY entries (directed) and appends every DIRECTLY/INDIRECTLY
+// live entry to wl (zeroing its support); undirected: append only.
+hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* state,
+                          const uint8_t* nflag, uint32_t* sup, uint32_t* wl, uint32_t E,
+                          bool directed, DevCounters* ctr, hipStream_t s);
+hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
+                            DevCounters* ctr, hipStream_t s);
+// Stable compaction of live entries into the *_out arrays and row-pointer
+// rebuild: tile_sums needs compact_tiles(E) + 1 entries, xs needs E + 1.
+uint32_t compact_tiles(uint32_t E);
+hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_t* src_in,
+                   const uint32_t* dst_in, const uint32_t* eid_in, const uint32_t* sup_in,
+                   uint32_t* ptr_out, uint32_t* src_out, uint32_t* dst_out, uint32_t* eid_out,
+                   uint32_t* sup_out, uint32_t* tile_sums, uint32_t* xs, DevCounters* ctr,
+                   hipStream_t s);
+}  // namespace dev
+
+}  // namespace kt
+
+struct kt_graph {
+    uint32_t n = 0;
+    uint64_t m_input = 0;
+    uint64_t m = 0;                 // undirected edges
+    std::vector<uint32_t> ua, ub;   // edge endpoints, ua < ub, sorted by (ua, ub)
+    std::vector<uint32_t> uid;      // per input entry: its edge
+    std::vector<uint32_t> degree;   // undirected degree per vertex
+    uint32_t max_degree = 0;
+    std::atomic<int> refs{1};       // a plan holds a reference
+};

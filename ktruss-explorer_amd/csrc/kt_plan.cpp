@@ -1,0 +1,411 @@
+// kt_plan.cpp -- device-resident plans and the peel / decomposition driver.
+//
+// The host drives one HIP stream per plan.  Per peel iteration it launches
+// [zero supports] -> count -> mark, then reads a 32-byte counter block back
+// (the reference reads its `changed` flag the same way, kernel_ddc.cu:166-167)
+// to decide between compaction (-e), the k == 3 early exit, the affected-edge
+// marking and the next iteration.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kt_host.h"
+#include "kt_internal.h"
+#include "kt_plan.h"
+
+namespace kt {
+
+int DBuf::alloc(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(KT_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes: " +
+                                      hipGetErrorString(e));
+    }
+    size = bytes;
+    return KT_OK;
+}
+
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    size = 0;
+}
+
+namespace {
+
+bool g_device_checked = false;
+int g_device = 0;
+
+int next_pow2(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return (int)p;
+}
+
+}  // namespace
+
+int init_device(int device) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(KT_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    if (device < 0 || device >= count) return fail(KT_ERR_ARG, "device index out of range");
+    KT_HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    KT_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(KT_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                          ", this build targets gfx950 (MI355X)");
+    g_device = device;
+    g_device_checked = true;
+    return KT_OK;
+}
+
+static int ensure_device() {
+    if (g_device_checked) return hipSetDevice(g_device) == hipSuccess ? KT_OK
+                                                                      : fail(KT_ERR_HIP, "hipSetDevice");
+    return init_device(0);
+}
+
+Plan::~Plan() {
+    if (h_ctr) (void)hipHostFree(h_ctr);
+    for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1})
+        if (*ev) (void)hipEventDestroy(*ev);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (graph) graph_release(graph);
+}
+
+int Plan::create(const kt_graph* g, const kt_variant* var) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    v = *var;
+    if (v.directedness < 0 || v.directedness > 2 || v.format < 0 || v.format > 1 || v.recount < 0 ||
+        v.recount > 1)
+        return fail(KT_ERR_ARG, "variant enum out of range");
+    if (v.block_size == 0) v.block_size = 512;
+    if (v.block_size % 64 || v.block_size > 1024) return fail(KT_ERR_ARG, "block_size must be a multiple of 64 up to 1024");
+    if (v.chunk == 0) v.chunk = 128;
+    if (v.chunk % 64) return fail(KT_ERR_ARG, "chunk must be a multiple of 64");
+    directed = v.directedness != KT_UNDIRECTED;
+    tiled = v.format == KT_TILED_COOCSR;
+    affected = v.recount == KT_RECOUNT_AFFECTED;
+    HostLayout L;
+    rc = build_layout(g, v.directedness, L);
+    if (rc) return rc;
+    n = L.n;
+    E0 = L.E;
+    max_row = L.max_row;
+    new2old = std::move(L.new2old);
+    pos_of_edge = std::move(L.pos_of_edge);
+    if (tiled) {
+        if (v.tile_size == 0) {
+            if (v.tiles_per_dim == 0) return fail(KT_ERR_ARG, "tiles_per_dim must be positive");
+            tile_size = (n + v.tiles_per_dim - 1) / v.tiles_per_dim;  // graph.cu:580
+        } else {
+            tile_size = v.tile_size;
+        }
+        if (tile_size == 0) tile_size = 1;
+        const uint32_t ntiles = (n + tile_size - 1) / tile_size;
+        ytiles = std::max<uint32_t>(1, std::min<uint32_t>(v.num_parallel_tiles ? v.num_parallel_tiles : 1,
+                                                          std::max<uint32_t>(ntiles, 1)));
+    } else {
+        tile_size = n ? n : 1;
+        ytiles = 1;
+    }
+    const uint32_t nw = v.block_size / 64;
+    const uint32_t per_entry = directed ? 8 : 4;
+    uint32_t cap_max = 65536 / (nw * per_entry);
+    cap_max = std::min<uint32_t>(cap_max, 4096);
+    cap_lds = std::max<uint32_t>(64, std::min<uint32_t>(cap_max, (uint32_t)next_pow2(max_row)));
+    query_factor = 8;
+
+    KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1}) KT_HIP_TRY(hipEventCreate(ev));
+    KT_HIP_TRY(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocDefault));
+    std::memset(h_ctr, 0, sizeof(DevCounters));
+
+    const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
+    const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
+    if ((rc = ptr0.alloc(Pb)) || (rc = src0.alloc(Eb)) || (rc = dst0.alloc(Eb))) return rc;
+    for (int b = 0; b < 2; ++b)
+        if ((rc = ptr[b].alloc(Pb)) || (rc = src[b].alloc(Eb)) || (rc = dst[b].alloc(Eb)) ||
+            (rc = eid[b].alloc(Eb)) || (rc = sup[b].alloc(Eb)))
+            return rc;
+    if ((rc = state.alloc(std::max<uint32_t>(E0, 1))) || (rc = nflag.alloc(std::max<uint32_t>(n, 1))) ||
+        (rc = wl.alloc(Eb)) || (rc = tau.alloc(Eb)) ||
+        (rc = tiles.alloc(((size_t)dev::compact_tiles(E0) + 2) * sizeof(uint32_t))) ||
+        (rc = xs.alloc(Eb + sizeof(uint32_t))) || (rc = ctr.alloc(sizeof(DevCounters))))
+        return rc;
+    KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
+    if (E0) {
+        KT_HIP_TRY(hipMemcpyAsync(src0.p, L.src.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(dst0.p, L.dst.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+    }
+    KT_HIP_TRY(hipStreamSynchronize(stream));
+    graph = graph_retain(g);
+    return KT_OK;
+}
+
+int Plan::sync_counters() {
+    KT_HIP_TRY(hipMemcpyAsync(h_ctr, ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
+    KT_HIP_TRY(hipStreamSynchronize(stream));
+    return KT_OK;
+}
+
+int Plan::set_counter(size_t offset, uint32_t value) {
+    KT_HIP_TRY(dev::fill_u32((uint32_t*)((char*)ctr.p + offset), value, 1, stream));
+    return KT_OK;
+}
+
+int Plan::reset() {
+    const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
+    cur = 0;
+    E = E0;
+    live = E0;
+    KT_HIP_TRY(hipMemcpyAsync(ptr[0].p, ptr0.p, Pb, hipMemcpyDeviceToDevice, stream));
+    if (E0) {
+        KT_HIP_TRY(hipMemcpyAsync(src[0].p, src0.p, (size_t)E0 * 4, hipMemcpyDeviceToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(dst[0].p, dst0.p, (size_t)E0 * 4, hipMemcpyDeviceToDevice, stream));
+    }
+    KT_HIP_TRY(dev::iota_u32(eid[0].as<uint32_t>(), E0, stream));
+    KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E0, stream));
+    KT_HIP_TRY(dev::fill_u32(ctr.as<uint32_t>(), 0, sizeof(DevCounters) / 4, stream));
+    KT_HIP_TRY(hipStreamSynchronize(stream));
+    std::memset(h_ctr, 0, sizeof(DevCounters));
+    trace.clear();
+    stats = kt_stats{};
+    have_tau = false;
+    return KT_OK;
+}
+
+int Plan::compact_now() {
+    const int o = 1 - cur;
+    stats.compactions++;
+    stats.compact_elems += E;
+    KT_HIP_TRY(dev::compact(n, E, ptr[cur].as<uint32_t>(), src[cur].as<uint32_t>(),
+                            dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
+                            sup[cur].as<uint32_t>(), ptr[o].as<uint32_t>(), src[o].as<uint32_t>(),
+                            dst[o].as<uint32_t>(), eid[o].as<uint32_t>(), sup[o].as<uint32_t>(),
+                            tiles.as<uint32_t>(), xs.as<uint32_t>(), ctr.as<DevCounters>(), stream));
+    int rc = sync_counters();
+    if (rc) return rc;
+    if (h_ctr->live != live)
+        return fail(KT_ERR_STATE, "compaction kept " + std::to_string(h_ctr->live) +
+                                      " entries, expected " + std::to_string(live));
+    cur = o;
+    E = h_ctr->live;
+    return KT_OK;
+}
+
+CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems) {
+    CountArgs a;
+    a.ptr = ptr[cur].as<uint32_t>();
+    a.src = src[cur].as<uint32_t>();
+    a.dst = dst[cur].as<uint32_t>();
+    a.sup = sup[cur].as<uint32_t>();
+    a.wl = use_wl ? wl.as<uint32_t>() : nullptr;
+    a.nitems = nitems;
+    a.chunk = v.chunk;
+    a.cap_lds = cap_lds;
+    a.cap = k - 2;
+    a.tile_size = tile_size;
+    a.ytiles = ytiles;
+    a.query_factor = query_factor;
+    a.prefilter = (!directed && !tiled) ? 1u : 0u;  // kernel_duc.cu:27; kernel_dut.cu has none
+    a.ctr = ctr.as<DevCounters>();
+    return a;
+}
+
+// One phase at k: the peel loop of kernel_ddc.cu:116-214 (and its tiled /
+// undirected twins) on the current graph.
+int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out) {
+    const bool jump = directed && (schedule & KT_SCHED_JUMP);
+    uint32_t iter = 0;
+    bool has_del = false, broke = false, changed = false;
+    bool use_wl = false;   // false: items are all stored entries [0, E)
+    uint32_t nitems = 0;   // worklist length when use_wl
+    int rc;
+    stats.phases++;
+    // initInfoOnDevice (info.cu:55-60): every entry directly affected
+    if (affected) KT_HIP_TRY(dev::fill_u8(state.as<uint8_t>(), KT_DIRECTLY_AFFECTED, E, stream));
+    do {
+        if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
+        if (affected) KT_HIP_TRY(dev::fill_u8(nflag.as<uint8_t>(), KT_NOT_AFFECTED, n, stream));
+        bool counted = false;
+        const uint64_t rec_before = h_ctr->recount_edges;
+        if (!(iter == 0 && reuse)) {
+            if (!use_wl) KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
+            CountArgs a = count_args(k, use_wl, use_wl ? nitems : E);
+            KT_HIP_TRY(hipEventRecord(ev_c0, stream));
+            KT_HIP_TRY(dev::count_supports(a, directed, tiled, v.block_size, stream));
+            KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+            counted = true;
+            stats.count_passes++;
+            stats.live_in_passes += live;
+        }
+        if (iter == 0 && jump) {
+            if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
+            KT_HIP_TRY(dev::min_live_support(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E,
+                                             ctr.as<DevCounters>(), stream));
+            if ((rc = sync_counters())) return rc;
+            const uint32_t s = h_ctr->min_sup;
+            if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
+        }
+        KT_HIP_TRY(dev::mark_deleted(dst[cur].as<uint32_t>(), src[cur].as<uint32_t>(),
+                                     eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
+                                     state.as<uint8_t>(), nflag.as<uint8_t>(), tau.as<uint32_t>(),
+                                     use_wl ? wl.as<uint32_t>() : nullptr, use_wl ? nitems : E, k, affected,
+                                     ctr.as<DevCounters>(), stream));
+        if ((rc = sync_counters())) return rc;
+        if (counted) {
+            float ms = 0;
+            KT_HIP_TRY(hipEventElapsedTime(&ms, ev_c0, ev_c1));
+            stats.ms_count += ms;
+            stats.count_launches++;
+        }
+        const uint32_t ndel = h_ctr->ndel;
+        live -= ndel;
+        stats.iterations++;
+        trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)(directed ? ndel : ndel / 2),
+                         h_ctr->recount_edges - rec_before});
+        if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
+        changed = ndel > 0;
+        if (changed) {
+            if (iter < v.num_edge_remove_iter) {
+                if ((rc = compact_now())) return rc;
+                has_del = false;
+            } else {
+                has_del = true;
+            }
+            if (k == 3) {  // kernel_ddc.cu:183-186
+                broke = true;
+                break;
+            }
+            if (affected) {
+                KT_HIP_TRY(dev::mark_directly(src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                              state.as<uint8_t>(), nflag.as<uint8_t>(), E, directed,
+                                              stream));
+                if ((rc = set_counter(offsetof(DevCounters, wl_size), 0))) return rc;
+                KT_HIP_TRY(dev::build_worklist(src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                               state.as<uint8_t>(), nflag.as<uint8_t>(),
+                                               sup[cur].as<uint32_t>(), wl.as<uint32_t>(), E,
+                                               directed, ctr.as<DevCounters>(), stream));
+                if ((rc = sync_counters())) return rc;
+                nitems = h_ctr->wl_size;
+                use_wl = true;
+            }
+        }
+        ++iter;
+    } while (changed);
+    if (has_del && (rc = compact_now())) return rc;  // kernel_ddc.cu:206-212
+    exact_out = !broke && !changed;
+    stats.k_last = k;
+    return KT_OK;
+}
+
+int Plan::decompose() {
+    int rc = reset();
+    if (rc) return rc;
+    KT_HIP_TRY(hipEventRecord(ev_start, stream));
+    uint32_t k = 3;
+    bool reuse = false;
+    while (live > 0) {
+        bool exact = false;
+        if ((rc = run_phase(k, reuse, v.schedule, exact))) return rc;
+        reuse = directed && !affected && (v.schedule & KT_SCHED_REUSE) && exact;
+        ++k;
+    }
+    KT_HIP_TRY(hipEventRecord(ev_stop, stream));
+    KT_HIP_TRY(hipEventSynchronize(ev_stop));
+    float ms = 0;
+    KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    if ((rc = sync_counters())) return rc;
+    stats.ms_total = ms;
+    finish_stats();
+    have_tau = true;
+    return KT_OK;
+}
+
+void Plan::finish_stats() {
+    stats.merge_elems = h_ctr->merge_elems;
+    stats.recount_edges = h_ctr->recount_edges;
+    stats.balg_count_bytes = 4.0 * (double)stats.merge_elems + 24.0 * (double)stats.recount_edges +
+                             12.0 * (double)stats.live_in_passes;
+}
+
+int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_t>& out_dst) {
+    if (k < 2) return fail(KT_ERR_ARG, "k must be at least 2");
+    int rc = reset();
+    if (rc) return rc;
+    KT_HIP_TRY(hipEventRecord(ev_start, stream));
+    bool exact = false;
+    uint32_t kk = k;
+    if (live > 0 && (rc = run_phase(kk, false, 0, exact))) return rc;
+    KT_HIP_TRY(hipEventRecord(ev_stop, stream));
+    KT_HIP_TRY(hipEventSynchronize(ev_stop));
+    float ms = 0;
+    KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    if ((rc = sync_counters())) return rc;
+    stats.ms_total = ms;
+    finish_stats();
+    std::vector<uint32_t> s(E), d(E);
+    if (E) {
+        KT_HIP_TRY(hipMemcpy(s.data(), src[cur].p, (size_t)E * 4, hipMemcpyDeviceToHost));
+        KT_HIP_TRY(hipMemcpy(d.data(), dst[cur].p, (size_t)E * 4, hipMemcpyDeviceToHost));
+    }
+    // coocsr2coo + directed2undirectedCOO + unsort (kernel_ddc.cu:246-250,
+    // kernel_d.cu:57-61, kernel.cu:31-37)
+    const size_t out = directed ? 2 * (size_t)E : (size_t)E;
+    out_src.resize(out);
+    out_dst.resize(out);
+    for (uint32_t e = 0; e < E; ++e) {
+        if (d[e] & KT_DEAD) return fail(KT_ERR_STATE, "tombstone left after final compaction");
+        out_src[e] = new2old[s[e]];
+        out_dst[e] = new2old[d[e]];
+        if (directed) {
+            out_src[E + e] = new2old[d[e]];
+            out_dst[E + e] = new2old[s[e]];
+        }
+    }
+    return KT_OK;
+}
+
+int Plan::truss_numbers(uint32_t* out, uint64_t count) {
+    if (!have_tau) return fail(KT_ERR_STATE, "no decomposition has run on this plan");
+    if (count != graph->m_input)
+        return fail(KT_ERR_CAPACITY, "need " + std::to_string(graph->m_input) + " entries");
+    std::vector<uint32_t> t(E0);
+    if (E0) KT_HIP_TRY(hipMemcpy(t.data(), tau.p, (size_t)E0 * 4, hipMemcpyDeviceToHost));
+    const uint32_t* uid = graph->uid.data();
+    const uint32_t* pos = pos_of_edge.data();
+    parallel_for(count, 1 << 16, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) out[i] = t[pos[uid[i]]];
+    });
+    return KT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph reference counting (a plan keeps its graph alive)
+
+const kt_graph* graph_retain(const kt_graph* g) {
+    const_cast<kt_graph*>(g)->refs.fetch_add(1);
+    return g;
+}
+
+void graph_release(const kt_graph* g) {
+    if (g && const_cast<kt_graph*>(g)->refs.fetch_sub(1) == 1) delete g;
+}
+
+}  // namespace kt

@@ -1,0 +1,74 @@
+// kt_plan.h -- device-resident plan of one variant.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "kt_internal.h"
+
+namespace kt {
+
+struct DBuf {
+    void* p = nullptr;
+    size_t size = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    int alloc(size_t bytes);
+    void release();
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+int init_device(int device);
+const kt_graph* graph_retain(const kt_graph* g);
+void graph_release(const kt_graph* g);
+
+struct Plan {
+    kt_variant v{};
+    const kt_graph* graph = nullptr;
+    bool directed = true, tiled = false, affected = false;
+    uint32_t n = 0, E0 = 0, E = 0;
+    uint64_t live = 0;
+    uint32_t tile_size = 1, ytiles = 1;
+    uint32_t cap_lds = 64, query_factor = 8;
+    uint32_t max_row = 0;
+    std::vector<uint32_t> new2old, pos_of_edge;
+
+    DBuf ptr0, src0, dst0;                          // pristine layout
+    DBuf ptr[2], src[2], dst[2], eid[2], sup[2];    // working layout, double-buffered
+    int cur = 0;
+    DBuf state, nflag, wl, tau, tiles, xs, ctr;
+    DevCounters* h_ctr = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
+
+    std::vector<std::array<uint64_t, 4>> trace;
+    kt_stats stats{};
+    bool have_tau = false;
+
+    ~Plan();
+    int create(const kt_graph* g, const kt_variant* v);
+    int reset();
+    int decompose();
+    int ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_t>& out_dst);
+    int truss_numbers(uint32_t* out, uint64_t count);
+
+   private:
+    int run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out);
+    int compact_now();
+    int sync_counters();
+    int set_counter(size_t offset, uint32_t value);
+    CountArgs count_args(uint32_t k, bool use_wl, uint32_t nitems);
+    void finish_stats();
+};
+
+}  // namespace kt
+
+struct kt_plan {
+    kt::Plan impl;
+};

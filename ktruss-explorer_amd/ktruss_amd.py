@@ -1,0 +1,339 @@
+"""Python binding of the MI355X k-truss engine (build/libktruss_hip.so).
+
+A thin ctypes layer over the C ABI in include/ktruss.h; every call goes to
+the HIP library, which fails loudly when no gfx950 device is visible.  The
+names follow the reference's interface (KTrussExplorer):
+
+  Graph(n, src, dst)             the COOGraph handed to ktruss()  (kernel.cu:9)
+  Variant(...)                   Config's variant fields          (config.h:163-181)
+  Plan(graph, variant).ktruss(k) one k, ./ktruss -k k             (kernel_ddc.cu:116-214)
+  Plan.decompose()               all k: truss number per edge     (k++ loop over the same peel)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libktruss_hip.so")
+CLI_PATH = os.path.join(_HERE, "build", "ktruss")
+
+OK = 0
+ERR_ARG, ERR_SELF_LOOP, ERR_ASYMMETRIC, ERR_DUPLICATE, ERR_RANGE = -1, -2, -3, -4, -5
+ERR_NOMEM, ERR_CAPACITY, ERR_IO, ERR_HIP, ERR_NO_DEVICE, ERR_STATE = -6, -7, -8, -9, -10, -11
+
+UNDIRECTED, DIRECTED_BY_INDEX, DIRECTED_BY_DEGREE = 0, 1, 2
+COOCSR, TILED_COOCSR = 0, 1
+RECOUNT_ALL, RECOUNT_AFFECTED = 0, 1
+SCHED_JUMP, SCHED_REUSE = 1, 2
+INPUT_SYMMETRIC, INPUT_PAIRS = 0, 1
+
+DIRECTEDNESS = {"undirected": 0, "directed-by-index": 1, "directed-by-degree": 2}
+FORMATS = {"coocsr": 0, "tiled-coocsr": 1}
+RECOUNTS = {"all": 0, "affected": 1}
+
+# C symbols declared in include/ktruss.h
+EXPORTS = [
+    "kt_abi_version", "kt_last_error", "kt_status_string", "kt_init", "kt_free",
+    "kt_variant_default", "kt_graph_create", "kt_graph_destroy", "kt_graph_info",
+    "kt_read_tsv", "kt_write_tsv", "kt_verify_edges", "kt_generate_rmat",
+    "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
+    "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
+    "kt_plan_trace",
+]
+
+
+class KtError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class Variant(ctypes.Structure):
+    _fields_ = [("directedness", ctypes.c_int32), ("format", ctypes.c_int32),
+                ("recount", ctypes.c_int32), ("num_edge_remove_iter", ctypes.c_uint32),
+                ("block_size", ctypes.c_uint32), ("tiles_per_dim", ctypes.c_uint32),
+                ("num_parallel_tiles", ctypes.c_uint32), ("tile_size", ctypes.c_uint32),
+                ("chunk", ctypes.c_uint32), ("schedule", ctypes.c_uint32)]
+
+    def describe(self):
+        names = {v: k for k, v in DIRECTEDNESS.items()}
+        fmt = {v: k for k, v in FORMATS.items()}
+        rec = {v: k for k, v in RECOUNTS.items()}
+        s = f"{names[self.directedness]}/{fmt[self.format]}/{rec[self.recount]}"
+        if self.num_edge_remove_iter:
+            s += f"/e{self.num_edge_remove_iter}"
+        if self.format == TILED_COOCSR:
+            s += f"/t{self.tiles_per_dim}" if not self.tile_size else f"/S{self.tile_size}"
+            s += f"/y{self.num_parallel_tiles}"
+        return s
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("count_passes", ctypes.c_uint64), ("recount_edges", ctypes.c_uint64),
+                ("merge_elems", ctypes.c_uint64), ("live_in_passes", ctypes.c_uint64),
+                ("compactions", ctypes.c_uint64), ("compact_elems", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint64), ("phases", ctypes.c_uint64),
+                ("k_max", ctypes.c_uint32), ("k_last", ctypes.c_uint32),
+                ("ms_total", ctypes.c_double), ("ms_count", ctypes.c_double),
+                ("count_launches", ctypes.c_uint64), ("balg_count_bytes", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def lib():
+    """Load the HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.kt_last_error.restype = ctypes.c_char_p
+        L.kt_status_string.restype = ctypes.c_char_p
+        L.kt_status_string.argtypes = [ctypes.c_int]
+        L.kt_init.argtypes = [ctypes.c_int]
+        L.kt_free.argtypes = [ctypes.c_void_p]
+        L.kt_variant_default.argtypes = [ctypes.POINTER(Variant)]
+        L.kt_variant_default.restype = None
+        L.kt_graph_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p,
+                                      ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+        L.kt_graph_destroy.argtypes = [ctypes.c_void_p]
+        L.kt_graph_destroy.restype = None
+        L.kt_graph_info.argtypes = [ctypes.c_void_p, _u32p, _u64p, _u64p, _u32p]
+        L.kt_read_tsv.argtypes = [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(_u32p),
+                                  ctypes.POINTER(_u32p)]
+        L.kt_write_tsv.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u32p, _u32p]
+        L.kt_verify_edges.argtypes = [ctypes.c_uint64, _u32p, _u32p, ctypes.c_uint64, _u32p, _u32p]
+        L.kt_generate_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _u32p,
+                                       _u64p, ctypes.POINTER(_u32p), ctypes.POINTER(_u32p)]
+        L.kt_generate_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_uint32, ctypes.c_uint64, _u64p,
+                                           ctypes.POINTER(_u32p), ctypes.POINTER(_u32p)]
+        L.kt_generate_erdos_renyi.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
+                                              _u64p, ctypes.POINTER(_u32p), ctypes.POINTER(_u32p)]
+        L.kt_plan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(Variant),
+                                     ctypes.POINTER(ctypes.c_void_p)]
+        L.kt_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.kt_plan_destroy.restype = None
+        L.kt_plan_info.argtypes = [ctypes.c_void_p, _u64p, _u32p]
+        L.kt_plan_ktruss.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _u32p, _u32p,
+                                     ctypes.c_uint64, _u64p, ctypes.POINTER(Stats)]
+        L.kt_plan_decompose.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.kt_plan_truss_numbers.argtypes = [ctypes.c_void_p, _u32p, ctypes.c_uint64]
+        L.kt_plan_trace.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, _u64p]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != OK:
+        msg = lib().kt_last_error().decode(errors="replace")
+        raise KtError(rc, f"{what}: {msg}")
+    return rc
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(_u32p)
+
+
+def _take_array(p, count):
+    """Copy a malloc'ed uint32 array returned by the library, then free it."""
+    out = np.ctypeslib.as_array(p, shape=(count,)).copy() if count else np.zeros(0, np.uint32)
+    lib().kt_free(ctypes.cast(p, ctypes.c_void_p))
+    return out
+
+
+def init(device=0):
+    _check(lib().kt_init(device), "kt_init")
+
+
+def make_variant(directedness="directed-by-degree", fmt="coocsr", recount="all", e_iters=0,
+                 block_size=512, tiles=8, parallel_tiles=8, tile_size=0, chunk=0,
+                 schedule=SCHED_JUMP | SCHED_REUSE):
+    v = Variant()
+    lib().kt_variant_default(ctypes.byref(v))
+    v.directedness = DIRECTEDNESS[directedness] if isinstance(directedness, str) else directedness
+    v.format = FORMATS[fmt] if isinstance(fmt, str) else fmt
+    v.recount = RECOUNTS[recount] if isinstance(recount, str) else recount
+    v.num_edge_remove_iter = e_iters
+    v.block_size = block_size
+    v.tiles_per_dim = tiles
+    v.num_parallel_tiles = parallel_tiles
+    v.tile_size = tile_size
+    v.chunk = chunk
+    v.schedule = schedule
+    return v
+
+
+def default_variant():
+    return make_variant()
+
+
+def all_variants(tiles=8, parallel_tiles=8):
+    """The 12 combinations of the reference's design space (3 orientations x 2
+    layouts x 2 recount policies)."""
+    out = []
+    for d in ("undirected", "directed-by-index", "directed-by-degree"):
+        for f in ("coocsr", "tiled-coocsr"):
+            for r in ("all", "affected"):
+                out.append(make_variant(d, f, r, tiles=tiles, parallel_tiles=parallel_tiles))
+    return out
+
+
+class Graph:
+    """Validated undirected graph.  `pairs=False`: symmetric COO (both
+    directions, as read from a file); `pairs=True`: each edge once."""
+
+    def __init__(self, n, src, dst, pairs=False):
+        s, sp = _u32(src)
+        d, dp = _u32(dst)
+        if len(s) != len(d):
+            raise ValueError("src and dst differ in length")
+        h = ctypes.c_void_p()
+        _check(lib().kt_graph_create(n, len(s), sp, dp, INPUT_PAIRS if pairs else INPUT_SYMMETRIC,
+                                     ctypes.byref(h)), "kt_graph_create")
+        self._h = h
+        self.n = n
+        self.src, self.dst = s, d
+        nn = ctypes.c_uint32()
+        mu = ctypes.c_uint64()
+        mi = ctypes.c_uint64()
+        md = ctypes.c_uint32()
+        _check(lib().kt_graph_info(h, ctypes.byref(nn), ctypes.byref(mu), ctypes.byref(mi),
+                                   ctypes.byref(md)), "kt_graph_info")
+        self.m = mu.value
+        self.m_input = mi.value
+        self.max_degree = md.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.kt_graph_destroy(h)
+            self._h = None
+
+
+class Plan:
+    """Device-resident layout of one variant of one graph."""
+
+    def __init__(self, graph, variant=None):
+        variant = variant or default_variant()
+        h = ctypes.c_void_p()
+        _check(lib().kt_plan_create(graph._h, ctypes.byref(variant), ctypes.byref(h)),
+               "kt_plan_create")
+        self._h = h
+        self.graph = graph
+        self.variant = variant
+        e = ctypes.c_uint64()
+        mr = ctypes.c_uint32()
+        _check(lib().kt_plan_info(h, ctypes.byref(e), ctypes.byref(mr)), "kt_plan_info")
+        self.entries = e.value
+        self.max_row = mr.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.kt_plan_destroy(h)
+            self._h = None
+
+    def ktruss(self, k):
+        """The k-truss as a symmetric COO in the graph's labels, plus stats."""
+        cap = 2 * self.graph.m
+        s = np.zeros(max(cap, 1), np.uint32)
+        d = np.zeros(max(cap, 1), np.uint32)
+        out_m = ctypes.c_uint64()
+        st = Stats()
+        _check(lib().kt_plan_ktruss(self._h, k, s.ctypes.data_as(_u32p), d.ctypes.data_as(_u32p),
+                                    cap, ctypes.byref(out_m), ctypes.byref(st)), "kt_plan_ktruss")
+        m = out_m.value
+        return s[:m].copy(), d[:m].copy(), st.as_dict()
+
+    def decompose(self):
+        st = Stats()
+        _check(lib().kt_plan_decompose(self._h, ctypes.byref(st)), "kt_plan_decompose")
+        return st.as_dict()
+
+    def truss_numbers(self):
+        m = self.graph.m_input
+        t = np.zeros(max(m, 1), np.uint32)
+        _check(lib().kt_plan_truss_numbers(self._h, t.ctypes.data_as(_u32p), m),
+               "kt_plan_truss_numbers")
+        return t[:m].copy()
+
+    def trace(self):
+        n = ctypes.c_uint64()
+        _check(lib().kt_plan_trace(self._h, None, 0, ctypes.byref(n)), "kt_plan_trace")
+        buf = np.zeros(max(4 * n.value, 1), np.uint64)
+        _check(lib().kt_plan_trace(self._h, buf.ctypes.data_as(_u64p), n.value, ctypes.byref(n)),
+               "kt_plan_trace")
+        return [tuple(int(x) for x in buf[4 * i:4 * i + 4]) for i in range(n.value)]
+
+
+def _gen_out():
+    return ctypes.c_uint64(), _u32p(), _u32p()
+
+
+def generate_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+    """R-MAT edge pairs (u < v, sorted, cleaned).  Returns (n, src, dst)."""
+    n = ctypes.c_uint32()
+    m, sp, dp = _gen_out()
+    _check(lib().kt_generate_rmat(scale, edge_factor, a, b, c, seed, ctypes.byref(n),
+                                  ctypes.byref(m), ctypes.byref(sp), ctypes.byref(dp)),
+           "kt_generate_rmat")
+    return n.value, _take_array(sp, m.value), _take_array(dp, m.value)
+
+
+def generate_chung_lu(n, gamma=2.1, avg_degree=16.0, max_weight=0.0, num_cliques=0,
+                      clique_min=20, clique_max=200, seed=1):
+    m, sp, dp = _gen_out()
+    _check(lib().kt_generate_chung_lu(n, gamma, avg_degree, max_weight, num_cliques, clique_min,
+                                      clique_max, seed, ctypes.byref(m), ctypes.byref(sp),
+                                      ctypes.byref(dp)), "kt_generate_chung_lu")
+    return n, _take_array(sp, m.value), _take_array(dp, m.value)
+
+
+def generate_erdos_renyi(n, avg_degree=8.0, seed=1):
+    m, sp, dp = _gen_out()
+    _check(lib().kt_generate_erdos_renyi(n, avg_degree, seed, ctypes.byref(m), ctypes.byref(sp),
+                                         ctypes.byref(dp)), "kt_generate_erdos_renyi")
+    return n, _take_array(sp, m.value), _take_array(dp, m.value)
+
+
+def symmetrize(src, dst):
+    """Pairs -> symmetric COO (both directions), the reference's input shape."""
+    s = np.concatenate([src, dst]).astype(np.uint32)
+    d = np.concatenate([dst, src]).astype(np.uint32)
+    return s, d
+
+
+def read_tsv(path):
+    n = ctypes.c_uint32()
+    m, sp, dp = _gen_out()
+    _check(lib().kt_read_tsv(path.encode(), ctypes.byref(n), ctypes.byref(m), ctypes.byref(sp),
+                             ctypes.byref(dp)), "kt_read_tsv")
+    return n.value, _take_array(sp, m.value), _take_array(dp, m.value)
+
+
+def write_tsv(path, src, dst):
+    s, sp = _u32(src)
+    d, dp = _u32(dst)
+    _check(lib().kt_write_tsv(path.encode(), len(s), sp, dp), "kt_write_tsv")
+
+
+def verify_edges(src_a, dst_a, src_b, dst_b):
+    a, ap = _u32(src_a)
+    b, bp = _u32(dst_a)
+    c, cp = _u32(src_b)
+    d, dp = _u32(dst_b)
+    rc = lib().kt_verify_edges(len(a), ap, bp, len(c), cp, dp)
+    if rc < 0:
+        _check(rc, "kt_verify_edges")
+    return rc == 0
