@@ -109,6 +109,8 @@ const char* kt_status_string(int status);
 /* Select the device (hipSetDevice) and check it is gfx950.  Optional: the
  * first plan calls it with device 0. */
 int kt_init(int device);
+/* hipDeviceSynchronize on the current device (used to bracket timed regions). */
+int kt_device_synchronize(void);
 void kt_free(void* p);
 
 /* ---- graphs (host) -------------------------------------------------------

@@ -46,6 +46,12 @@ const char* kt_status_string(int status) {
 
 int kt_init(int device) { KT_GUARD(return kt::init_device(device);) }
 
+int kt_device_synchronize(void) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return kt::fail(KT_ERR_HIP, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+    return KT_OK;
+}
+
 void kt_free(void* p) { std::free(p); }
 
 void kt_variant_default(kt_variant* v) {

@@ -34,7 +34,8 @@ RECOUNTS = {"all": 0, "affected": 1}
 
 # C symbols declared in include/ktruss.h
 EXPORTS = [
-    "kt_abi_version", "kt_last_error", "kt_status_string", "kt_init", "kt_free",
+    "kt_abi_version", "kt_last_error", "kt_status_string", "kt_init", "kt_device_synchronize",
+    "kt_free",
     "kt_variant_default", "kt_graph_create", "kt_graph_destroy", "kt_graph_info",
     "kt_read_tsv", "kt_write_tsv", "kt_verify_edges", "kt_generate_rmat",
     "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
@@ -154,6 +155,10 @@ def _take_array(p, count):
 
 def init(device=0):
     _check(lib().kt_init(device), "kt_init")
+
+
+def device_synchronize():
+    _check(lib().kt_device_synchronize(), "kt_device_synchronize")
 
 
 def make_variant(directedness="directed-by-degree", fmt="coocsr", recount="all", e_iters=0,
