@@ -52,6 +52,15 @@ enum { KT_RECOUNT_ALL = 0, KT_RECOUNT_AFFECTED = 1 };
  * its exact supports to the next phase. */
 enum { KT_SCHED_JUMP = 1u, KT_SCHED_REUSE = 2u };
 
+/* Support-count strategies (directed COO+CSR; the other layouts use ROW).
+ * ROW   : a wave walks its entries in order, staging each row of u in LDS and
+ *         streaming v's row against it.
+ * SEARCH: 64 entries at a time, all touched row tails staged, the candidate
+ *         windows of v's rows concatenated and binary-searched lane by lane.
+ * MERGE : as SEARCH, but each lane merges a contiguous share of the windows
+ *         against the staged rows. */
+enum { KT_COUNT_AUTO = 0u, KT_COUNT_ROW = 1u, KT_COUNT_SEARCH = 2u, KT_COUNT_MERGE = 3u };
+
 /* Input flags for kt_graph_create. */
 enum {
     KT_INPUT_SYMMETRIC = 0u, /* both (u,v) and (v,u) present, as createCOOFromFile
@@ -59,8 +68,8 @@ enum {
     KT_INPUT_PAIRS = 1u      /* each undirected edge exactly once, either direction */
 };
 
-/* One point of the design space.  Fields mirror Config (config.h:163-181);
- * the last three are this engine's own knobs. */
+/* One point of the design space.  The first seven fields mirror Config
+ * (config.h:163-181); the rest are this engine's own knobs. */
 typedef struct kt_variant {
     int32_t directedness;          /* -d                                                  */
     int32_t format;                /* -f                                                  */
@@ -72,6 +81,8 @@ typedef struct kt_variant {
     uint32_t tile_size;            /* vertices per tile; 0 = ceil(n / tiles_per_dim)      */
     uint32_t chunk;                /* worklist entries per wave task; 0 = default        */
     uint32_t schedule;             /* KT_SCHED_* bits for kt_plan_decompose              */
+    uint32_t count_kernel;         /* KT_COUNT_*: support-count strategy                 */
+    uint32_t cap_lds;              /* LDS slice per wave in entries; 0 = automatic       */
 } kt_variant;
 
 /* The reference's defaults (config.h:185-199): directed-by-degree, coocsr,
@@ -179,8 +190,9 @@ int kt_plan_decompose(kt_plan* p, kt_stats* stats);
  * graph the plan was made from (count = m_input). */
 int kt_plan_truss_numbers(kt_plan* p, uint32_t* tau, uint64_t count);
 /* Per-iteration trace of the last run: (k, iteration, undirected edges
- * deleted, entries recounted) quadruples as uint64; returns the number of
- * records in *len (all of them even when cap is smaller). */
+ * deleted, entries recounted, count-kernel nanoseconds) as 5 uint64 per
+ * record; *len is the number of records (all of them even when cap is
+ * smaller). */
 int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len);
 
 #ifdef __cplusplus

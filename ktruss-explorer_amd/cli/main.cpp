@@ -131,13 +131,14 @@ int main(int argc, char** argv) {
         if (c.verbosity >= 2) {
             uint64_t len = 0;
             kt_plan_trace(plan, nullptr, 0, &len);
-            std::vector<uint64_t> tr(4 * len);
+            std::vector<uint64_t> tr(5 * len);
             kt_plan_trace(plan, tr.data(), len, &len);
             for (uint64_t r = 0; r < len; ++r) {
-                std::printf("        k = %llu, iteration %llu\n", (unsigned long long)tr[4 * r],
-                            (unsigned long long)tr[4 * r + 1]);
+                std::printf("        k = %llu, iteration %llu\n", (unsigned long long)tr[5 * r],
+                            (unsigned long long)tr[5 * r + 1]);
+                Timer::print(tr[5 * r + 4] * 1e-6, "            Triangle counting time");
                 std::printf("            # edges recounted = %llu, # edges deleted = %llu\n",
-                            (unsigned long long)tr[4 * r + 3], (unsigned long long)tr[4 * r + 2]);
+                            (unsigned long long)tr[5 * r + 3], (unsigned long long)tr[5 * r + 2]);
             }
             Timer::print(st.ms_count, "            Triangle counting time (all passes)");
         }

@@ -185,7 +185,7 @@ int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len)
     if (recs) {
         const uint64_t c = std::min<uint64_t>(cap, tr.size());
         for (uint64_t i = 0; i < c; ++i)
-            for (int j = 0; j < 4; ++j) recs[4 * i + j] = tr[i][j];
+            for (int j = 0; j < 5; ++j) recs[5 * i + j] = tr[i][j];
     }
     return KT_OK;
 }

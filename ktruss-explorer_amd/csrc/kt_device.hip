@@ -69,6 +69,21 @@ __device__ __forceinline__ uint32_t lower_bound_masked(const uint32_t* a, uint32
     return lo;
 }
 
+// lower_bound_masked for a range [lo, lo + len) that is the same in every lane
+// of the wave: the trip count is uniform (a scalar loop) and the body is
+// branch-free, so no exec-mask bookkeeping per step.
+__device__ __forceinline__ uint32_t lower_bound_uniform(const uint32_t* a, uint32_t lo,
+                                                        uint32_t len, uint32_t x) {
+    if (len == 0) return lo;
+    uint32_t base = lo;
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        base = (a[base + half] & KT_VMASK) < x ? base + half : base;
+        len -= half;
+    }
+    return base + ((a[base] & KT_VMASK) < x ? 1u : 0u);
+}
+
 // Wave-uniform lower bound in an HBM row; lanes split the range 64-way per
 // step so the search takes log64(len) dependent loads.
 __device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint32_t hi,
@@ -371,6 +386,459 @@ __global__ __launch_bounds__(1024) void k_count(CountArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Directed support count on the COO+CSR layout, flattened.
+//
+// A wave takes blocks of 64 consecutive items; lane i holds item i, the entry
+// (u_i, v_i) at position q_i of u's row.  Consecutive items of one row with
+// increasing positions form a run, and each run's row tail [q_head, ulen) is
+// staged once in the wave's LDS slice (a run longer than the slice is searched
+// in HBM).  Lane i narrows v_i's row to the window of values that u's
+// candidates can match -- [value at q_i+1, last value of u's row] -- by two
+// binary searches.  The 64 windows are concatenated (exclusive scan) and
+// streamed 64 entries per round, one independent load per lane, so no lane
+// idles on a short row and no round waits on another item's latency.  Every
+// streamed entry w is binary-searched among its item's candidates in LDS; a
+// hit credits (u,v) and (u,w) through 16-bit LDS counters (flushed with one
+// atomic per staged entry) and (v,w) with one global atomic.
+
+struct FlatItem {     // per-item LDS record (32 bytes)
+    uint32_t vb;      // v's row start
+    uint32_t lo;      // window start within v's row
+    uint32_t pre;     // exclusive prefix of window lengths
+    uint32_t eslot;   // LDS slot of the item's own entry, or its HBM position
+    uint32_t slo;     // candidate range [slo, shi): LDS slots or HBM positions
+    uint32_t shi;
+    uint32_t hbm;     // 1: candidates searched in HBM
+    uint32_t pad;
+};
+
+constexpr uint32_t kFlatFixedWords = 68 + 68 + 64 * 8;  // s_roff, s_pre, items
+
+__host__ __device__ __forceinline__ uint32_t flat_words(uint32_t cap) {
+    return cap + cap / 2 + kFlatFixedWords;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// First index in [0, n) with a[idx] > t (a non-decreasing).
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t* a, uint32_t n, uint32_t t) {
+    uint32_t lo = 0, len = n;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if (a[lo + half] <= t) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void lds_credit16(uint32_t* s_cnt, uint32_t slot) {
+    atomicAdd(s_cnt + (slot >> 1), 1u << ((slot & 1) << 4));
+}
+
+__device__ __forceinline__ void lds_credit16_n(uint32_t* s_cnt, uint32_t slot, uint32_t n) {
+    atomicAdd(s_cnt + (slot >> 1), n << ((slot & 1) << 4));
+}
+
+// MERGE = false: every streamed entry is binary-searched among its item's
+// candidates (round-robin lanes, coalesced loads).  MERGE = true: each lane
+// takes a contiguous share of the concatenated windows and merges it against
+// the staged candidates, which are sorted like the window, so a lane advances
+// its LDS cursor instead of searching (the merge of kernel_hdc.cu:22-44 with
+// u's row in LDS and the work split evenly over the wave).
+template <bool MERGE>
+__global__ __launch_bounds__(1024) void k_count_flat(CountArgs a) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t nw = blockDim.x / kWave;
+    const uint32_t cap = a.cap_lds;
+    uint32_t* s_val = lds + wid * flat_words(cap);
+    uint32_t* s_cnt = s_val + cap;
+    uint32_t* s_roff = s_cnt + cap / 2;
+    uint32_t* s_pre = s_roff + 68;
+    FlatItem* s_item = reinterpret_cast<FlatItem*>(s_pre + 68);
+    const uint32_t task = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
+    const uint64_t first = (uint64_t)task * a.chunk;
+    if (first >= a.nitems) return;
+    const uint32_t i0 = (uint32_t)first;
+    const uint32_t i1 = a.nitems - i0 < a.chunk ? a.nitems : i0 + a.chunk;
+    const uint32_t* ptr = a.ptr;
+    const uint32_t* dst = a.dst;
+    uint32_t* sup = a.sup;
+    unsigned long long acc_merge = 0, acc_rec = 0;
+
+    for (uint32_t base = i0; base < i1; base += kWave) {
+        const uint32_t my = base + lane;
+        const bool in = my < i1;
+        uint32_t e = 0, u = 0xFFFFFFFFu, v = KT_DEAD, ub = 0, ulen = 0, vb = 0, vlen = 0;
+        if (in) {
+            e = a.wl ? a.wl[my] : my;
+            v = dst[e];
+            u = a.src[e];
+            ub = ptr[u];
+            ulen = ptr[u + 1] - ub;
+            if (!(v & KT_DEAD)) {
+                vb = ptr[v];
+                vlen = ptr[v + 1] - vb;
+            }
+        }
+        const bool live = in && !(v & KT_DEAD);
+        const uint32_t q = e - ub;
+        // runs: same row, increasing positions; lane 0 is always in range
+        const uint32_t pu = __shfl_up(u, 1), pe = __shfl_up(e, 1);
+        const bool head = in && (lane == 0 || u != pu || e < pe);
+        const uint64_t heads = ballot(head);
+        const uint32_t h = 63u - (uint32_t)__clzll(heads & ((2ull << lane) - 1ull));
+        const uint32_t rsize = ulen - q;  // staged length if this lane heads a run
+        const uint32_t gsrc = ub + q;     // HBM position of a run's first staged entry
+
+        uint64_t remaining = heads;
+        while (remaining) {
+            // A pass takes the longest prefix of remaining runs that fits the slice.
+            const bool rem_head = (remaining >> lane) & 1ull;
+            const uint32_t ssz = (rem_head && rsize <= cap) ? rsize : 0u;
+            const uint32_t cum = wave_incl_scan(ssz, lane);
+            const bool take = rem_head && cum <= cap;
+            const uint64_t pass = ballot(take);
+            remaining &= ~pass;
+            const uint32_t psz = take ? ssz : 0u;
+            const uint32_t pcum = wave_incl_scan(psz, lane);
+            const uint32_t poff = pcum - psz;
+            const uint32_t T = readlane(pcum, 63);
+            wave_lds_sync();
+            s_roff[lane] = poff;
+            if (lane == 0) s_roff[64] = T;
+            wave_lds_sync();
+            for (uint32_t t0 = 0; t0 < T; t0 += kWave) {
+                const uint32_t t = t0 + lane;
+                const uint32_t r = upper_bound_u32(s_roff, 65, t < T ? t : T - 1) - 1;
+                const uint32_t g = __shfl(gsrc, r);
+                if (t < T) s_val[t] = dst[g + (t - s_roff[r])];
+            }
+            for (uint32_t w = lane; w < (T + 1) / 2; w += kWave) s_cnt[w] = 0;
+            wave_lds_sync();
+
+            // items of this pass: candidate window in v's row
+            const bool inpass = in && ((pass >> h) & 1ull);
+            const uint32_t h_poff = __shfl(poff, h);
+            const uint32_t h_q = __shfl(q, h);
+            const uint32_t h_rsize = __shfl(rsize, h);
+            const bool staged = h_rsize <= cap;
+            const bool has = inpass && live && q + 1 < ulen && vlen > 0;
+            uint32_t eslot, slo, shi;
+            if (staged) {
+                eslot = h_poff + (q - h_q);
+                slo = eslot + 1;
+                shi = h_poff + (ulen - h_q);
+            } else {
+                eslot = e;
+                slo = ub + q + 1;
+                shi = ub + ulen;
+            }
+            uint32_t lo = 0, hi = 0;
+            if (has) {
+                const uint32_t* rows = staged ? s_val : dst;
+                const uint32_t umin = rows[slo] & KT_VMASK;
+                const uint32_t umax = rows[shi - 1] & KT_VMASK;
+                if (vlen <= 16) {
+                    hi = vlen;
+                } else {
+                    lo = lower_bound_masked(dst + vb, 0, vlen, umin);
+                    hi = lower_bound_masked(dst + vb, lo, vlen, umax + 1);
+                }
+            }
+            if (inpass && live) {
+                acc_rec += 1;
+                acc_merge += (unsigned long long)ulen + vlen;
+            }
+            const uint32_t work = hi - lo;
+            const uint32_t wincl = wave_incl_scan(work, lane);
+            const uint32_t pre = wincl - work;
+            const uint32_t W = readlane(wincl, 63);
+            s_pre[lane] = pre;
+            if (lane == 0) s_pre[64] = W;
+            FlatItem it;
+            it.vb = vb;
+            it.lo = lo;
+            it.pre = pre;
+            it.eslot = eslot;
+            it.slo = slo;
+            it.shi = shi;
+            it.hbm = staged ? 0u : 1u;
+            it.pad = 0;
+            s_item[lane] = it;
+            wave_lds_sync();
+
+            if (MERGE) {
+                // lane-contiguous share [t, tend) of the concatenated windows
+                const uint32_t per = (W + kWave - 1) / kWave;
+                uint32_t t = lane * per;
+                const uint32_t tend = t + per < W ? t + per : W;
+                if (t < tend) {
+                    uint32_t i = upper_bound_u32(s_pre, 65, t) - 1;
+                    FlatItem x = s_item[i];
+                    uint32_t nextb = s_pre[i + 1];
+                    uint32_t p = 0, ecount = 0;
+                    bool fresh = true;
+                    for (; t < tend; ++t) {
+                        while (t >= nextb) {  // next item (empty windows skipped)
+                            if (ecount) {
+                                if (x.hbm) atomicAdd(sup + x.eslot, ecount);
+                                else lds_credit16_n(s_cnt, x.eslot, ecount);
+                            }
+                            ecount = 0;
+                            ++i;
+                            x = s_item[i];
+                            nextb = s_pre[i + 1];
+                            fresh = true;
+                        }
+                        const uint32_t j = x.lo + (t - x.pre);
+                        const uint32_t w = dst[x.vb + j];
+                        if (w & KT_DEAD) continue;
+                        const uint32_t* cand = x.hbm ? dst : s_val;
+                        if (fresh) {
+                            p = lower_bound_masked(cand, x.slo, x.shi, w);
+                            fresh = false;
+                        } else {
+                            // advance the cursor: a few linear steps, then search
+                            uint32_t steps = 0;
+                            while (p < x.shi && (cand[p] & KT_VMASK) < w && steps < 4) {
+                                ++p;
+                                ++steps;
+                            }
+                            if (p < x.shi && (cand[p] & KT_VMASK) < w)
+                                p = lower_bound_masked(cand, p, x.shi, w);
+                        }
+                        if (p < x.shi && cand[p] == w) {
+                            if (x.hbm) atomicAdd(sup + p, 1u);
+                            else lds_credit16(s_cnt, p);
+                            ++ecount;
+                            atomicAdd(sup + x.vb + j, 1u);
+                        }
+                    }
+                    if (ecount) {
+                        if (x.hbm) atomicAdd(sup + x.eslot, ecount);
+                        else lds_credit16_n(s_cnt, x.eslot, ecount);
+                    }
+                }
+            }
+            for (uint32_t t0 = 0; !MERGE && t0 < W; t0 += kWave) {
+                const uint32_t t = t0 + lane;
+                if (t < W) {
+                    const uint32_t i = upper_bound_u32(s_pre, 65, t) - 1;
+                    const FlatItem x = s_item[i];
+                    const uint32_t j = x.lo + (t - x.pre);
+                    const uint32_t w = dst[x.vb + j];
+                    if (!(w & KT_DEAD)) {
+                        if (!x.hbm) {
+                            const uint32_t p = lower_bound_masked(s_val, x.slo, x.shi, w);
+                            if (p < x.shi && s_val[p] == w) {
+                                lds_credit16(s_cnt, p);
+                                lds_credit16(s_cnt, x.eslot);
+                                atomicAdd(sup + x.vb + j, 1u);
+                            }
+                        } else {
+                            const uint32_t p = lower_bound_masked(dst, x.slo, x.shi, w);
+                            if (p < x.shi && dst[p] == w) {
+                                atomicAdd(sup + p, 1u);
+                                atomicAdd(sup + x.eslot, 1u);
+                                atomicAdd(sup + x.vb + j, 1u);
+                            }
+                        }
+                    }
+                }
+            }
+            wave_lds_sync();
+            if (W) {
+                for (uint32_t t0 = 0; t0 < T; t0 += kWave) {
+                    const uint32_t t = t0 + lane;
+                    const uint32_t r = upper_bound_u32(s_roff, 65, t < T ? t : T - 1) - 1;
+                    const uint32_t g = __shfl(gsrc, r);
+                    if (t < T) {
+                        const uint32_t c = (s_cnt[t >> 1] >> ((t & 1) << 4)) & 0xFFFFu;
+                        if (c) atomicAdd(sup + g + (t - s_roff[r]), c);
+                    }
+                }
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc_rec += __shfl_xor(acc_rec, off);
+        acc_merge += __shfl_xor(acc_merge, off);
+    }
+    if (lane == 0 && acc_rec) {
+        atomicAdd(&a.ctr->recount_edges, acc_rec);
+        atomicAdd(&a.ctr->merge_elems, acc_merge);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Directed support count on the COO+CSR layout, row-staged and pipelined.
+//
+// Lanes load 64 items at once and, lane-parallel, narrow each item's v-row to
+// the window of values its u-candidates span: [next entry of u's row, last
+// entry of u's row], found by two binary searches (the first bound is simply
+// the entry after e in HBM).  The wave then walks (item, 64-entry chunk)
+// pairs in order; the load of the next chunk -- of the same item or of the
+// next one -- is issued before the current chunk is searched, so one HBM round
+// trip overlaps the LDS work of the previous chunk.  u's row is staged in LDS
+// once per run of items of that row; credits to (u,v) and (u,w) go to 16-bit
+// LDS counters, the credit to (v,w) is one global atomic.
+__global__ __launch_bounds__(1024) void k_count_dir(CountArgs a) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t nw = blockDim.x / kWave;
+    const uint32_t cap = a.cap_lds;
+    uint32_t* s_val = lds + wid * (cap + cap / 2);
+    uint32_t* s_cnt = s_val + cap;
+    const uint32_t task = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
+    const uint64_t first = (uint64_t)task * a.chunk;
+    if (first >= a.nitems) return;
+    const uint32_t i0 = (uint32_t)first;
+    const uint32_t i1 = a.nitems - i0 < a.chunk ? a.nitems : i0 + a.chunk;
+    const uint32_t* ptr = a.ptr;
+    const uint32_t* dst = a.dst;
+    uint32_t* sup = a.sup;
+    // staged row
+    uint32_t su = 0xFFFFFFFFu, sub = 0, sulen = 0;
+    bool s_in_lds = false;
+    unsigned long long acc_merge = 0, acc_rec = 0;
+
+    auto flush = [&]() {
+        if (su == 0xFFFFFFFFu || !s_in_lds) return;
+        wave_lds_sync();
+        for (uint32_t j = lane; j < sulen; j += kWave) {
+            const uint32_t c = (s_cnt[j >> 1] >> ((j & 1) << 4)) & 0xFFFFu;
+            if (c) atomicAdd(sup + sub + j, c);
+        }
+    };
+    auto stage = [&](uint32_t u) {
+        su = u;
+        sub = ptr[u];
+        sulen = ptr[u + 1] - sub;
+        s_in_lds = sulen <= cap;
+        if (s_in_lds) {
+            wave_lds_sync();
+            for (uint32_t j = lane; j < sulen; j += kWave) s_val[j] = dst[sub + j];
+            for (uint32_t j = lane; j < (sulen + 1) / 2; j += kWave) s_cnt[j] = 0;
+            wave_lds_sync();
+        }
+    };
+
+    for (uint32_t base = i0; base < i1; base += kWave) {
+        const uint32_t my = base + lane;
+        uint32_t e = 0, u = 0, v = KT_DEAD, ub = 0, ulen = 0, vb = 0, vlen = 0;
+        if (my < i1) {
+            e = a.wl ? a.wl[my] : my;
+            v = dst[e];
+            if (!(v & KT_DEAD)) {
+                u = a.src[e];
+                ub = ptr[u];
+                ulen = ptr[u + 1] - ub;
+                vb = ptr[v];
+                vlen = ptr[v + 1] - vb;
+            }
+        }
+        const bool live = my < i1 && !(v & KT_DEAD);
+        if (live) {
+            acc_rec += 1;
+            acc_merge += (unsigned long long)ulen + vlen;
+        }
+        uint32_t lo = 0, hi = 0;
+        if (live && e + 1 < ub + ulen && vlen > 0) {
+            const uint32_t umin = dst[e + 1] & KT_VMASK;
+            const uint32_t umax = dst[ub + ulen - 1] & KT_VMASK;
+            if (vlen <= 16) {
+                hi = vlen;
+            } else {
+                lo = lower_bound_masked(dst + vb, 0, vlen, umin);
+                hi = lower_bound_masked(dst + vb, lo, vlen, umax + 1);
+            }
+        }
+        uint64_t todo = ballot(hi > lo);
+        if (!todo) continue;
+        // current (item, chunk)
+        uint32_t l = (uint32_t)__ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        uint32_t cu = readlane(u, l), ce = readlane(e, l), cvb = readlane(vb, l);
+        uint32_t chi = readlane(hi, l), j0 = readlane(lo, l);
+        uint32_t w = j0 + lane < chi ? dst[cvb + j0 + lane] : KT_DEAD;
+        uint32_t c = 0;
+        while (true) {
+            // issue the next chunk's load before searching this one
+            const bool same = j0 + kWave < chi;
+            uint32_t nl = l, nj0 = j0 + kWave, nu = cu, ne = ce, nvb = cvb, nhi = chi;
+            bool more = same;
+            if (!same && todo) {
+                nl = (uint32_t)__ffsll((long long)todo) - 1;
+                nu = readlane(u, nl);
+                ne = readlane(e, nl);
+                nvb = readlane(vb, nl);
+                nhi = readlane(hi, nl);
+                nj0 = readlane(lo, nl);
+                more = true;
+            }
+            const uint32_t nw_ = more && nj0 + lane < nhi ? dst[nvb + nj0 + lane] : KT_DEAD;
+
+            if (cu != su) {
+                flush();
+                stage(cu);
+            }
+            const uint32_t q = ce - sub;
+            bool hit = false;
+            // candidates are row positions [q+1, sulen): the same range for the whole chunk
+            if (s_in_lds) {
+                const uint32_t p = lower_bound_uniform(s_val, q + 1, sulen - q - 1, w & KT_VMASK);
+                hit = !(w & KT_DEAD) && p < sulen && s_val[p] == w;
+                if (hit) lds_credit16(s_cnt, p);
+            } else {
+                const uint32_t p = lower_bound_uniform(dst, sub + q + 1, sulen - q - 1, w & KT_VMASK);
+                hit = !(w & KT_DEAD) && p < sub + sulen && dst[p] == w;
+                if (hit) atomicAdd(sup + p, 1u);
+            }
+            if (hit) atomicAdd(sup + cvb + j0 + lane, 1u);
+            c += (uint32_t)__popcll(ballot(hit));
+            if (!same) {  // last chunk of this item: credit (u, v)
+                if (c && lane == 0) {
+                    if (s_in_lds) lds_credit16_n(s_cnt, q, c);
+                    else atomicAdd(sup + ce, c);
+                }
+                c = 0;
+            }
+            if (!more) break;
+            if (!same) todo &= todo - 1;
+            l = nl;
+            cu = nu;
+            ce = ne;
+            cvb = nvb;
+            chi = nhi;
+            j0 = nj0;
+            w = nw_;
+        }
+    }
+    flush();
+    for (int off = 32; off > 0; off >>= 1) {
+        acc_rec += __shfl_xor(acc_rec, off);
+        acc_merge += __shfl_xor(acc_merge, off);
+    }
+    if (lane == 0 && acc_rec) {
+        atomicAdd(&a.ctr->recount_edges, acc_rec);
+        atomicAdd(&a.ctr->merge_elems, acc_merge);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Streaming kernels: grid-stride, 256 threads.
 
 constexpr uint32_t kBlock = 256;
@@ -638,17 +1106,27 @@ hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t block,
-                          hipStream_t s) {
+hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t algo,
+                          uint32_t block, hipStream_t s) {
     if (!a.nitems) return hipSuccess;
     const uint32_t nw = block / kWave;
     const uint64_t tasks = (a.nitems + (uint64_t)a.chunk - 1) / a.chunk;
     const uint32_t blocks = (uint32_t)((tasks + nw - 1) / nw);
-    const size_t lds = (size_t)nw * (directed ? 2 : 1) * a.cap_lds * sizeof(uint32_t);
     const dim3 grid(blocks, tiled ? a.ytiles : 1);
+    if (directed && !tiled && algo != KT_COUNT_ROW) {
+        const size_t lds = (size_t)nw * flat_words(a.cap_lds) * sizeof(uint32_t);
+        if (algo == KT_COUNT_SEARCH) hipLaunchKernelGGL((k_count_flat<false>), grid, dim3(block), lds, s, a);
+        else hipLaunchKernelGGL((k_count_flat<true>), grid, dim3(block), lds, s, a);
+        return hipGetLastError();
+    }
+    if (directed && !tiled) {
+        const size_t lds = (size_t)nw * (a.cap_lds + a.cap_lds / 2) * sizeof(uint32_t);
+        hipLaunchKernelGGL(k_count_dir, grid, dim3(block), lds, s, a);
+        return hipGetLastError();
+    }
+    const size_t lds = (size_t)nw * (directed ? 2 : 1) * a.cap_lds * sizeof(uint32_t);
     if (directed) {
-        if (tiled) hipLaunchKernelGGL((k_count<true, true>), grid, dim3(block), lds, s, a);
-        else hipLaunchKernelGGL((k_count<true, false>), grid, dim3(block), lds, s, a);
+        hipLaunchKernelGGL((k_count<true, true>), grid, dim3(block), lds, s, a);
     } else {
         if (tiled) hipLaunchKernelGGL((k_count<false, true>), grid, dim3(block), lds, s, a);
         else hipLaunchKernelGGL((k_count<false, false>), grid, dim3(block), lds, s, a);
@@ -697,6 +1175,14 @@ hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E
 }
 
 uint32_t compact_tiles(uint32_t E) { return (E + kTile - 1) / kTile; }
+
+uint32_t count_lds_bytes(uint32_t cap_lds, uint32_t block, bool directed, bool tiled,
+                         uint32_t algo) {
+    const uint32_t nw = block / kWave;
+    if (directed && !tiled && algo != KT_COUNT_ROW) return nw * flat_words(cap_lds) * 4u;
+    if (directed && !tiled) return nw * (cap_lds + cap_lds / 2) * 4u;
+    return nw * (directed ? 2u : 1u) * cap_lds * 4u;
+}
 
 hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_t* src_in,
                    const uint32_t* dst_in, const uint32_t* eid_in, const uint32_t* sup_in,

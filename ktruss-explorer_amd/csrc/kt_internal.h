@@ -78,8 +78,8 @@ namespace dev {
 hipError_t fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
 hipError_t fill_u8(uint8_t* p, uint8_t value, uint64_t count, hipStream_t s);
 hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s);
-hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t block,
-                          hipStream_t s);
+hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t algo,
+                          uint32_t block, hipStream_t s);
 hipError_t zero_live_supports(const uint32_t* dst, uint32_t* sup, uint32_t E, hipStream_t s);
 hipError_t mark_deleted(uint32_t* dst, const uint32_t* src, const uint32_t* eid,
                         const uint32_t* sup, const uint8_t* state, uint8_t* nflag,
@@ -97,6 +97,9 @@ hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E
 // Stable compaction of live entries into the *_out arrays and row-pointer
 // rebuild: tile_sums needs compact_tiles(E) + 1 entries, xs needs E + 1.
 uint32_t compact_tiles(uint32_t E);
+// Dynamic LDS bytes of one count workgroup.
+uint32_t count_lds_bytes(uint32_t cap_lds, uint32_t block, bool directed, bool tiled,
+                         uint32_t algo);
 hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_t* src_in,
                    const uint32_t* dst_in, const uint32_t* eid_in, const uint32_t* sup_in,
                    uint32_t* ptr_out, uint32_t* src_out, uint32_t* dst_out, uint32_t* eid_out,

@@ -124,11 +124,21 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         tile_size = n ? n : 1;
         ytiles = 1;
     }
-    const uint32_t nw = v.block_size / 64;
-    const uint32_t per_entry = directed ? 8 : 4;
-    uint32_t cap_max = 65536 / (nw * per_entry);
-    cap_max = std::min<uint32_t>(cap_max, 4096);
-    cap_lds = std::max<uint32_t>(64, std::min<uint32_t>(cap_max, (uint32_t)next_pow2(max_row)));
+    // LDS slice per wave: the largest power of two up to the longest row that
+    // keeps a workgroup within 64 KiB
+    if (v.count_kernel > KT_COUNT_MERGE) return fail(KT_ERR_ARG, "unknown count_kernel");
+    if (v.count_kernel == KT_COUNT_AUTO) v.count_kernel = KT_COUNT_ROW;
+    if (!directed || tiled) v.count_kernel = KT_COUNT_ROW;
+    if (v.cap_lds) {
+        if (v.cap_lds < 64 || v.cap_lds > 4096 || (v.cap_lds & (v.cap_lds - 1)))
+            return fail(KT_ERR_ARG, "cap_lds must be a power of two in [64, 4096]");
+        cap_lds = v.cap_lds;
+    } else {
+        cap_lds = std::min<uint32_t>(4096, std::max<uint32_t>(64, (uint32_t)next_pow2(max_row)));
+    }
+    while (cap_lds > 64 &&
+           dev::count_lds_bytes(cap_lds, v.block_size, directed, tiled, v.count_kernel) > 65536)
+        cap_lds >>= 1;
     query_factor = 8;
 
     KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -249,7 +259,7 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
             if (!use_wl) KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
             CountArgs a = count_args(k, use_wl, use_wl ? nitems : E);
             KT_HIP_TRY(hipEventRecord(ev_c0, stream));
-            KT_HIP_TRY(dev::count_supports(a, directed, tiled, v.block_size, stream));
+            KT_HIP_TRY(dev::count_supports(a, directed, tiled, v.count_kernel, v.block_size, stream));
             KT_HIP_TRY(hipEventRecord(ev_c1, stream));
             counted = true;
             stats.count_passes++;
@@ -269,17 +279,17 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
                                      use_wl ? wl.as<uint32_t>() : nullptr, use_wl ? nitems : E, k, affected,
                                      ctr.as<DevCounters>(), stream));
         if ((rc = sync_counters())) return rc;
+        float count_ms = 0;
         if (counted) {
-            float ms = 0;
-            KT_HIP_TRY(hipEventElapsedTime(&ms, ev_c0, ev_c1));
-            stats.ms_count += ms;
+            KT_HIP_TRY(hipEventElapsedTime(&count_ms, ev_c0, ev_c1));
+            stats.ms_count += count_ms;
             stats.count_launches++;
         }
         const uint32_t ndel = h_ctr->ndel;
         live -= ndel;
         stats.iterations++;
         trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)(directed ? ndel : ndel / 2),
-                         h_ctr->recount_edges - rec_before});
+                         h_ctr->recount_edges - rec_before, (uint64_t)(count_ms * 1e6)});
         if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
         changed = ndel > 0;
         if (changed) {

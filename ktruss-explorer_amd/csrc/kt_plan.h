@@ -47,7 +47,7 @@ struct Plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
 
-    std::vector<std::array<uint64_t, 4>> trace;
+    std::vector<std::array<uint64_t, 5>> trace;
     kt_stats stats{};
     bool have_tau = false;
 

@@ -55,7 +55,8 @@ class Variant(ctypes.Structure):
                 ("recount", ctypes.c_int32), ("num_edge_remove_iter", ctypes.c_uint32),
                 ("block_size", ctypes.c_uint32), ("tiles_per_dim", ctypes.c_uint32),
                 ("num_parallel_tiles", ctypes.c_uint32), ("tile_size", ctypes.c_uint32),
-                ("chunk", ctypes.c_uint32), ("schedule", ctypes.c_uint32)]
+                ("chunk", ctypes.c_uint32), ("schedule", ctypes.c_uint32),
+                ("count_kernel", ctypes.c_uint32), ("cap_lds", ctypes.c_uint32)]
 
     def describe(self):
         names = {v: k for k, v in DIRECTEDNESS.items()}
@@ -161,9 +162,13 @@ def device_synchronize():
     _check(lib().kt_device_synchronize(), "kt_device_synchronize")
 
 
+COUNT_AUTO, COUNT_ROW, COUNT_SEARCH, COUNT_MERGE = 0, 1, 2, 3
+COUNT_KERNELS = {"auto": 0, "row": 1, "search": 2, "merge": 3}
+
+
 def make_variant(directedness="directed-by-degree", fmt="coocsr", recount="all", e_iters=0,
                  block_size=512, tiles=8, parallel_tiles=8, tile_size=0, chunk=0,
-                 schedule=SCHED_JUMP | SCHED_REUSE):
+                 schedule=SCHED_JUMP | SCHED_REUSE, count_kernel=COUNT_AUTO, cap_lds=0):
     v = Variant()
     lib().kt_variant_default(ctypes.byref(v))
     v.directedness = DIRECTEDNESS[directedness] if isinstance(directedness, str) else directedness
@@ -176,6 +181,8 @@ def make_variant(directedness="directed-by-degree", fmt="coocsr", recount="all",
     v.tile_size = tile_size
     v.chunk = chunk
     v.schedule = schedule
+    v.count_kernel = COUNT_KERNELS[count_kernel] if isinstance(count_kernel, str) else count_kernel
+    v.cap_lds = cap_lds
     return v
 
 
@@ -273,13 +280,15 @@ class Plan:
                "kt_plan_truss_numbers")
         return t[:m].copy()
 
-    def trace(self):
+    def trace(self, with_time=False):
+        """(k, iteration, edges deleted, entries recounted[, count ns]) per iteration."""
         n = ctypes.c_uint64()
         _check(lib().kt_plan_trace(self._h, None, 0, ctypes.byref(n)), "kt_plan_trace")
-        buf = np.zeros(max(4 * n.value, 1), np.uint64)
+        buf = np.zeros(max(5 * n.value, 1), np.uint64)
         _check(lib().kt_plan_trace(self._h, buf.ctypes.data_as(_u64p), n.value, ctypes.byref(n)),
                "kt_plan_trace")
-        return [tuple(int(x) for x in buf[4 * i:4 * i + 4]) for i in range(n.value)]
+        w = 5 if with_time else 4
+        return [tuple(int(x) for x in buf[5 * i:5 * i + w]) for i in range(n.value)]
 
 
 def _gen_out():

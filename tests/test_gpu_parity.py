@@ -104,3 +104,37 @@ def test_rmat12_all_variants_agree():
         p.decompose()
         tau = p.truss_numbers()
         assert np.array_equal(tau, tau_o[: len(s)]), v.describe()
+
+
+def _golden(scale):
+    import hashlib
+    import json
+    import os
+    from conftest import GOLDEN
+    path = os.path.join(GOLDEN, f"rmat{scale}_tau.json")
+    if not os.path.exists(path):
+        pytest.skip(f"fixture rmat{scale}_tau.json not generated")
+    with open(path) as f:
+        return json.load(f), hashlib
+
+
+@pytest.mark.parametrize("scale", [14, 16, 18])
+def test_rmat_matches_golden_fixture(scale):
+    gold, hashlib = _golden(scale)
+    n, s, d = K.generate_rmat(scale, 16, seed=1)
+    h = hashlib.sha256(np.ascontiguousarray(s, "<u4").tobytes() +
+                       np.ascontiguousarray(d, "<u4").tobytes()).hexdigest()
+    assert h == gold["pairs_sha256"], "generator output changed"
+    g = K.Graph(n, s, d, pairs=True)
+    variants = [K.default_variant(), K.make_variant("directed-by-degree", "coocsr", "affected"),
+                K.make_variant("directed-by-degree", "tiled-coocsr", "all")]
+    if scale <= 16:
+        variants += [K.make_variant("undirected", "coocsr", "all"),
+                     K.make_variant("directed-by-index", "coocsr", "affected", e_iters=4)]
+    for v in variants:
+        p = K.Plan(g, v)
+        st = p.decompose()
+        tau = p.truss_numbers()
+        assert hashlib.sha256(np.ascontiguousarray(tau, "<u4").tobytes()).hexdigest() == \
+            gold["tau_sha256"], v.describe()
+        assert st["k_max"] == gold["k_max"]
