@@ -195,6 +195,58 @@ int kt_plan_truss_numbers(kt_plan* p, uint32_t* tau, uint64_t count);
  * smaller). */
 int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len);
 
+/* ---- learned count-kernel picker -----------------------------------------
+ * No counterpart in the reference (its variant is fixed per run).  A count
+ * configuration is the part of a variant that can change between phases
+ * without touching the layout.  Candidates are tried per phase:
+ *   FIXED  every phase uses the plan's own variant;
+ *   SWEEP  every count pass runs every candidate (each from zeroed supports)
+ *          and times it: the exhaustive baseline and the training labels;
+ *   MODEL  at each phase start a device kernel computes KT_MLP_IN features
+ *          of the surviving graph and the MLP (inference on the device)
+ *          picks the candidate with the lowest predicted cost. */
+#define KT_MLP_IN 16
+#define KT_MLP_HIDDEN 64
+#define KT_MLP_MAX_OUT 16
+#define KT_MAX_CANDIDATES 16
+#define KT_PHASE_REC 36
+
+typedef struct kt_count_config {
+    uint32_t count_kernel; /* KT_COUNT_*                              */
+    uint32_t block_size;   /* threads per workgroup, multiple of 64    */
+    uint32_t cap_lds;      /* LDS slice per wave; 0 = automatic        */
+    uint32_t chunk;        /* items per wave task; 0 = default         */
+} kt_count_config;
+
+enum { KT_PICK_FIXED = 0u, KT_PICK_SWEEP = 1u, KT_PICK_MODEL = 2u };
+
+typedef struct kt_mlp kt_mlp;
+
+int kt_plan_set_candidates(kt_plan* p, const kt_count_config* cands, uint32_t n);
+/* model is read when mode is KT_PICK_MODEL (its parameters are copied). */
+int kt_plan_set_pick(kt_plan* p, uint32_t mode, const kt_mlp* model);
+/* Phase log of the last decomposition, KT_PHASE_REC floats per phase:
+ * [0] k, [1] iterations, [2] chosen candidate, [3] count ms of the chosen,
+ * [4..19] features, [20..35] count ms per candidate (SWEEP; -1 otherwise). */
+int kt_plan_phase_log(const kt_plan* p, float* recs, uint64_t cap, uint64_t* len);
+
+/* MLP KT_MLP_IN -> KT_MLP_HIDDEN -> KT_MLP_HIDDEN -> n_out, ReLU, FP32, on the
+ * device.  Loss: sum_s w_s sum_c mask_sc (y_sc - t_sc)^2 / sum_s w_s. */
+int kt_mlp_create(uint32_t n_out, uint64_t seed, kt_mlp** out);
+void kt_mlp_destroy(kt_mlp* m);
+int kt_mlp_num_params(const kt_mlp* m, uint32_t* n);
+int kt_mlp_get_params(const kt_mlp* m, float* out, uint32_t n);
+int kt_mlp_set_params(kt_mlp* m, const float* in, uint32_t n);
+int kt_mlp_forward(kt_mlp* m, const float* x, uint32_t batch, float* y);
+/* Loss and its gradient (deterministic); mask and w may be NULL. */
+int kt_mlp_loss_grad(kt_mlp* m, const float* x, const float* t, const float* mask, const float* w,
+                     uint32_t batch, float* loss, float* grad);
+/* Adam on mini-batches drawn in a seeded order; *loss_out is the loss over
+ * all n samples after training. */
+int kt_mlp_train(kt_mlp* m, const float* x, const float* t, const float* mask, const float* w,
+                 uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                 float* loss_out);
+
 #ifdef __cplusplus
 }
 #endif

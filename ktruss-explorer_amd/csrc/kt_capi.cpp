@@ -1,6 +1,7 @@
 // kt_capi.cpp -- the extern "C" boundary declared in include/ktruss.h.
 // Every entry point converts C++ exceptions into status codes; nothing
 // throws across the ABI.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -8,6 +9,7 @@
 
 #include "kt_host.h"
 #include "kt_internal.h"
+#include "kt_mlp.h"
 #include "kt_plan.h"
 #include "ktruss.h"
 
@@ -188,6 +190,86 @@ int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len)
             for (int j = 0; j < 5; ++j) recs[5 * i + j] = tr[i][j];
     }
     return KT_OK;
+}
+
+int kt_plan_set_candidates(kt_plan* p, const kt_count_config* cands, uint32_t n) {
+    if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
+    KT_GUARD(return p->impl.set_candidates(cands, n);)
+}
+
+int kt_plan_set_pick(kt_plan* p, uint32_t mode, const kt_mlp* model) {
+    if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
+    if (mode == KT_PICK_MODEL && !model) return kt::fail(KT_ERR_ARG, "model is NULL");
+    KT_GUARD({
+        if (mode == KT_PICK_MODEL)
+            return p->impl.set_pick(mode, model->impl.P, model->impl.total, model->impl.C);
+        return p->impl.set_pick(mode, nullptr, 0, 0);
+    })
+}
+
+int kt_plan_phase_log(const kt_plan* p, float* recs, uint64_t cap, uint64_t* len) {
+    if (!p || !len) return kt::fail(KT_ERR_ARG, "NULL argument");
+    const auto& log = p->impl.phase_log;
+    *len = log.size();
+    if (recs) {
+        const uint64_t c = std::min<uint64_t>(cap, log.size());
+        for (uint64_t i = 0; i < c; ++i)
+            std::memcpy(recs + i * KT_PHASE_REC, log[i].data(), KT_PHASE_REC * sizeof(float));
+    }
+    return KT_OK;
+}
+
+int kt_mlp_create(uint32_t n_out, uint64_t seed, kt_mlp** out) {
+    if (!out) return kt::fail(KT_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    KT_GUARD({
+        int rc = kt::init_device(0);
+        if (rc) return rc;
+        auto* m = new kt_mlp();
+        rc = m->impl.create(n_out, seed);
+        if (rc) {
+            delete m;
+            return rc;
+        }
+        *out = m;
+        return KT_OK;
+    })
+}
+
+void kt_mlp_destroy(kt_mlp* m) { delete m; }
+
+int kt_mlp_num_params(const kt_mlp* m, uint32_t* n) {
+    if (!m || !n) return kt::fail(KT_ERR_ARG, "NULL argument");
+    *n = m->impl.total;
+    return KT_OK;
+}
+
+int kt_mlp_get_params(const kt_mlp* m, float* out, uint32_t n) {
+    if (!m || !out) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return m->impl.get(out, n);)
+}
+
+int kt_mlp_set_params(kt_mlp* m, const float* in, uint32_t n) {
+    if (!m || !in) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return m->impl.set(in, n);)
+}
+
+int kt_mlp_forward(kt_mlp* m, const float* x, uint32_t batch, float* y) {
+    if (!m || (batch && (!x || !y))) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return m->impl.forward(x, batch, y);)
+}
+
+int kt_mlp_loss_grad(kt_mlp* m, const float* x, const float* t, const float* mask, const float* w,
+                     uint32_t batch, float* loss, float* grad) {
+    if (!m || !x || !t) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return m->impl.loss_grad(x, t, mask, w, batch, loss, grad);)
+}
+
+int kt_mlp_train(kt_mlp* m, const float* x, const float* t, const float* mask, const float* w,
+                 uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                 float* loss_out) {
+    if (!m || !x || !t) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return m->impl.train(x, t, mask, w, n, epochs, batch, lr, seed, loss_out);)
 }
 
 }  // extern "C"

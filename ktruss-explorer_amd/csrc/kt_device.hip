@@ -704,9 +704,10 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a) {
     uint32_t* s_cnt = s_val + cap;
     const uint32_t task = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
     const uint64_t first = (uint64_t)task * a.chunk;
-    if (first >= a.nitems) return;
+    const uint32_t nitems = a.nitems_dev ? *a.nitems_dev : a.nitems;
+    if (first >= nitems) return;
     const uint32_t i0 = (uint32_t)first;
-    const uint32_t i1 = a.nitems - i0 < a.chunk ? a.nitems : i0 + a.chunk;
+    const uint32_t i1 = nitems - i0 < a.chunk ? nitems : i0 + a.chunk;
     const uint32_t* ptr = a.ptr;
     const uint32_t* dst = a.dst;
     uint32_t* sup = a.sup;
@@ -870,6 +871,10 @@ __global__ void k_zero_live(const uint32_t* __restrict__ dst, uint32_t* __restri
                             uint32_t E) {
     for (uint32_t i = gtid(); i < E; i += gsize())
         if (!(dst[i] & KT_DEAD)) sup[i] = 0;
+}
+
+__global__ void k_zero_listed(const uint32_t* __restrict__ wl, uint32_t n, uint32_t* __restrict__ sup) {
+    for (uint32_t i = gtid(); i < n; i += gsize()) sup[wl[i]] = 0;
 }
 
 // Wave-aggregated increment of a device counter; returns the wave's base.
@@ -1137,6 +1142,12 @@ hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_
 hipError_t zero_live_supports(const uint32_t* dst, uint32_t* sup, uint32_t E, hipStream_t s) {
     if (!E) return hipSuccess;
     hipLaunchKernelGGL(k_zero_live, dim3(grid_for(E)), dim3(kBlock), 0, s, dst, sup, E);
+    return hipGetLastError();
+}
+
+hipError_t zero_listed_supports(const uint32_t* wl, uint32_t n, uint32_t* sup, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_listed, dim3(grid_for(n)), dim3(kBlock), 0, s, wl, n, sup);
     return hipGetLastError();
 }
 

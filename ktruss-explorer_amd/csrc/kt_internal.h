@@ -53,6 +53,8 @@ struct DevCounters {
     unsigned int wl_size;             // worklist length
     unsigned int min_sup;             // minimum live support (JUMP)
     unsigned int live;                // entries kept by the last compaction
+    unsigned int choice;              // candidate picked by the cost model
+    unsigned int pad[3];
 };
 
 // Arguments of the support-count kernels.
@@ -71,6 +73,7 @@ struct CountArgs {
     uint32_t query_factor;// query the shorter list when it is this many times shorter
     uint32_t prefilter;   // undirected COO+CSR stored-length pre-filter (huc:20)
     DevCounters* ctr;
+    const uint32_t* nitems_dev;  // when set, the item count is read here (graph replay)
 };
 
 // Kernel launchers (kt_device.hip).  All asynchronous on `stream`.
@@ -81,6 +84,7 @@ hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s);
 hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t algo,
                           uint32_t block, hipStream_t s);
 hipError_t zero_live_supports(const uint32_t* dst, uint32_t* sup, uint32_t E, hipStream_t s);
+hipError_t zero_listed_supports(const uint32_t* wl, uint32_t n, uint32_t* sup, hipStream_t s);
 hipError_t mark_deleted(uint32_t* dst, const uint32_t* src, const uint32_t* eid,
                         const uint32_t* sup, const uint8_t* state, uint8_t* nflag,
                         uint32_t* tau, const uint32_t* wl, uint32_t nitems, uint32_t k,

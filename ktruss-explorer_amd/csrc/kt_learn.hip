@@ -1,0 +1,311 @@
+// kt_learn.hip -- the learned count-kernel picker: phase features and a
+// small FP32 MLP (16 -> 64 -> 64 -> C) with forward, backward and Adam
+// kernels.  The network is tiny (about 6K parameters), so every kernel maps
+// one sample to one wave of 64 lanes (lane j owns hidden unit j); gradients
+// are reduced deterministically (per-workgroup partials summed in a fixed
+// order).  No MFMA: these contractions are 64 wide and latency-bound.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kt_internal.h"
+#include "kt_learn.h"
+
+namespace kt {
+namespace {
+
+constexpr int kIn = KT_MLP_IN;
+constexpr int kHid = KT_MLP_HIDDEN;
+
+// Parameter layout (floats): W1[kHid][kIn] b1[kHid] W2[kHid][kHid] b2[kHid] W3[C][kHid] b3[C]
+struct Layout {
+    int w1, b1, w2, b2, w3, b3, total;
+};
+__host__ __device__ inline Layout layout(int C) {
+    Layout L;
+    L.w1 = 0;
+    L.b1 = L.w1 + kHid * kIn;
+    L.w2 = L.b1 + kHid;
+    L.b2 = L.w2 + kHid * kHid;
+    L.w3 = L.b2 + kHid;
+    L.b3 = L.w3 + C * kHid;
+    L.total = L.b3 + C;
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Phase features.  Accumulators (u64): 0 rows, 1 sum len, 2 sum len^2,
+// 3 max len, 4 entries in rows > 256, 5 entries in rows > 1024,
+// 6..10 support histogram bins, 11 entries histogrammed.
+__global__ void k_feat_rows(const uint32_t* __restrict__ ptr, uint32_t n,
+                            unsigned long long* acc) {
+    unsigned long long rows = 0, s1 = 0, s2 = 0, big = 0, huge = 0;
+    unsigned int mx = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const uint32_t len = ptr[r + 1] - ptr[r];
+        if (!len) continue;
+        rows += 1;
+        s1 += len;
+        s2 += (unsigned long long)len * len;
+        if (len > 256) big += len;
+        if (len > 1024) huge += len;
+        mx = len > mx ? len : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rows += __shfl_xor(rows, off);
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+        big += __shfl_xor(big, off);
+        huge += __shfl_xor(huge, off);
+        const unsigned int o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(acc + 0, rows);
+        atomicAdd(acc + 1, s1);
+        atomicAdd(acc + 2, s2);
+        atomicMax(acc + 3, (unsigned long long)mx);
+        atomicAdd(acc + 4, big);
+        atomicAdd(acc + 5, huge);
+    }
+}
+
+__global__ void k_feat_support(const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
+                               uint32_t E, uint32_t k, unsigned long long* acc) {
+    unsigned long long bins[5] = {0, 0, 0, 0, 0};
+    const float denom = (float)(k > 2 ? k - 2 : 1);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        if (dst[e] & KT_DEAD) continue;
+        const float r = (float)sup[e] / denom;
+        const int b = r < 1.f ? 0 : r < 2.f ? 1 : r < 4.f ? 2 : r < 8.f ? 3 : 4;
+        bins[b] += 1;
+    }
+    for (int b = 0; b < 5; ++b)
+        for (int off = 32; off > 0; off >>= 1) bins[b] += __shfl_xor(bins[b], off);
+    if ((threadIdx.x & 63) == 0)
+        for (int b = 0; b < 5; ++b) atomicAdd(acc + 6 + b, bins[b]);
+}
+
+__global__ void k_fill_acc(unsigned long long* acc) {
+    if (threadIdx.x < KT_FEAT_ACC) acc[threadIdx.x] = 0;
+}
+
+__device__ float log2p1(float x) { return log2f(1.f + x); }
+
+__global__ void k_feat_finish(const unsigned long long* acc, uint32_t E0, uint32_t k,
+                              int sup_valid, float* feat) {
+    if (threadIdx.x != 0) return;
+    const float rows = (float)acc[0], s1 = (float)acc[1], s2 = (float)acc[2];
+    const float mean = rows > 0 ? s1 / rows : 0.f;
+    feat[0] = log2p1(s1) / 32.f;
+    feat[1] = log2p1(rows) / 32.f;
+    feat[2] = mean / 256.f;
+    feat[3] = log2p1((float)acc[3]) / 16.f;
+    feat[4] = mean > 0 ? sqrtf(s2 / rows) / mean / 8.f : 0.f;
+    feat[5] = s1 > 0 ? (float)acc[4] / s1 : 0.f;
+    feat[6] = s1 > 0 ? (float)acc[5] / s1 : 0.f;
+    feat[7] = E0 ? s1 / (float)E0 : 0.f;
+    feat[8] = log2f((float)k) / 10.f;
+    feat[9] = s1 > 0 ? s2 / s1 / 1024.f : 0.f;
+    float tot = 0;
+    for (int b = 0; b < 5; ++b) tot += (float)acc[6 + b];
+    for (int b = 0; b < 5; ++b) feat[10 + b] = (sup_valid && tot > 0) ? (float)acc[6 + b] / tot : 0.f;
+    feat[15] = sup_valid ? 1.f : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// MLP forward for one sample in one wave; activations land in LDS.
+__device__ void mlp_forward_wave(const float* __restrict__ P, int C, const float* x, float* s_h1,
+                                 float* s_h2, float* out, int lane) {
+    const Layout L = layout(C);
+    float a = P[L.b1 + lane];
+    for (int i = 0; i < kIn; ++i) a += P[L.w1 + lane * kIn + i] * x[i];
+    s_h1[lane] = a > 0.f ? a : 0.f;
+    __syncthreads();
+    float b = P[L.b2 + lane];
+    for (int i = 0; i < kHid; ++i) b += P[L.w2 + lane * kHid + i] * s_h1[i];
+    s_h2[lane] = b > 0.f ? b : 0.f;
+    __syncthreads();
+    if (lane < C) {
+        float o = P[L.b3 + lane];
+        for (int i = 0; i < kHid; ++i) o += P[L.w3 + lane * kHid + i] * s_h2[i];
+        out[lane] = o;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_mlp_forward(const float* __restrict__ P, int C,
+                                                    const float* __restrict__ X, uint32_t batch,
+                                                    float* __restrict__ Y) {
+    __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
+    const int lane = threadIdx.x;
+    for (uint32_t s = blockIdx.x; s < batch; s += gridDim.x) {
+        if (lane < kIn) s_x[lane] = X[(size_t)s * kIn + lane];
+        __syncthreads();
+        mlp_forward_wave(P, C, s_x, s_h1, s_h2, s_o, lane);
+        if (lane < C) Y[(size_t)s * C + lane] = s_o[lane];
+        __syncthreads();
+    }
+}
+
+// Loss: sum_s w_s * sum_c mask_sc (y_sc - t_sc)^2 / sum_s w_s.  Each
+// workgroup (one wave) accumulates its samples' gradients into a partial row
+// part[blockIdx.x][P] and its loss into part_loss[blockIdx.x].
+__global__ __launch_bounds__(64) void k_mlp_backward(const float* __restrict__ P, int C,
+                                                     const float* __restrict__ X,
+                                                     const float* __restrict__ T,
+                                                     const float* __restrict__ M,
+                                                     const float* __restrict__ Wt, uint32_t batch,
+                                                     float inv_wsum, float* __restrict__ part,
+                                                     float* __restrict__ part_loss) {
+    extern __shared__ float g[];  // Layout(C).total floats of gradient
+    __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT], s_do[KT_MLP_MAX_OUT];
+    __shared__ float s_d2[kHid], s_d1[kHid];
+    const Layout L = layout(C);
+    const int lane = threadIdx.x;
+    for (int p = lane; p < L.total; p += 64) g[p] = 0.f;
+    float loss = 0.f;
+    __syncthreads();
+    for (uint32_t s = blockIdx.x; s < batch; s += gridDim.x) {
+        if (lane < kIn) s_x[lane] = X[(size_t)s * kIn + lane];
+        __syncthreads();
+        mlp_forward_wave(P, C, s_x, s_h1, s_h2, s_o, lane);
+        const float w = Wt ? Wt[s] : 1.f;
+        if (lane < C) {
+            const float m = M ? M[(size_t)s * C + lane] : 1.f;
+            const float diff = (s_o[lane] - T[(size_t)s * C + lane]) * m;
+            loss += w * diff * diff;
+            s_do[lane] = 2.f * w * diff * inv_wsum;
+        }
+        __syncthreads();
+        // layer 3: dW3[c][lane] += do[c] * h2[lane]; dh2[lane]
+        float dh2 = 0.f;
+        for (int c = 0; c < C; ++c) {
+            g[L.w3 + c * kHid + lane] += s_do[c] * s_h2[lane];
+            dh2 += P[L.w3 + c * kHid + lane] * s_do[c];
+        }
+        if (lane < C) g[L.b3 + lane] += s_do[lane];
+        s_d2[lane] = s_h2[lane] > 0.f ? dh2 : 0.f;
+        __syncthreads();
+        // layer 2: dW2[j][i] += d2[j] * h1[i] (lane = i); dh1[i] = sum_j W2[j][i] d2[j]
+        float dh1 = 0.f;
+        for (int j = 0; j < kHid; ++j) {
+            g[L.w2 + j * kHid + lane] += s_d2[j] * s_h1[lane];
+            dh1 += P[L.w2 + j * kHid + lane] * s_d2[j];
+        }
+        g[L.b2 + lane] += s_d2[lane];
+        s_d1[lane] = s_h1[lane] > 0.f ? dh1 : 0.f;
+        __syncthreads();
+        // layer 1: dW1[lane][i] += d1[lane] * x[i]
+        for (int i = 0; i < kIn; ++i) g[L.w1 + lane * kIn + i] += s_d1[lane] * s_x[i];
+        g[L.b1 + lane] += s_d1[lane];
+        __syncthreads();
+    }
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    for (int p = lane; p < L.total; p += 64) part[(size_t)blockIdx.x * L.total + p] = g[p];
+    if (lane == 0) part_loss[blockIdx.x] = loss * inv_wsum;
+}
+
+// Fixed-order reduction of the partial rows.
+__global__ void k_reduce_parts(const float* __restrict__ part, uint32_t nparts, uint32_t total,
+                               float* __restrict__ grad) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (uint32_t b = 0; b < nparts; ++b) s += part[(size_t)b * total + p];
+        grad[p] = s;
+    }
+}
+
+__global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ m1,
+                       float* __restrict__ m2, uint32_t total, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const float g = G[p];
+        const float a = b1 * m1[p] + (1.f - b1) * g;
+        const float b = b2 * m2[p] + (1.f - b2) * g * g;
+        m1[p] = a;
+        m2[p] = b;
+        P[p] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+    }
+}
+
+// Inference inside the peel loop: forward on the phase features, argmin of
+// the predicted costs written to *choice.
+__global__ __launch_bounds__(64) void k_mlp_pick(const float* __restrict__ P, int C,
+                                                 const float* __restrict__ feat,
+                                                 unsigned int* choice) {
+    __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
+    const int lane = threadIdx.x;
+    if (lane < kIn) s_x[lane] = feat[lane];
+    __syncthreads();
+    mlp_forward_wave(P, C, s_x, s_h1, s_h2, s_o, lane);
+    if (lane == 0) {
+        int best = 0;
+        for (int c = 1; c < C; ++c)
+            if (s_o[c] < s_o[best]) best = c;
+        *choice = (unsigned int)best;
+    }
+}
+
+}  // namespace
+
+namespace dev {
+
+uint32_t mlp_num_params(uint32_t C) { return (uint32_t)layout((int)C).total; }
+
+hipError_t phase_features(const uint32_t* ptr, const uint32_t* dst, const uint32_t* sup, uint32_t n,
+                          uint32_t E, uint32_t E0, uint32_t k, bool sup_valid,
+                          unsigned long long* acc, float* feat, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_acc, dim3(1), dim3(64), 0, s, acc);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    const uint32_t gr = n ? std::min<uint32_t>(2048, (n + 255) / 256) : 1;
+    hipLaunchKernelGGL(k_feat_rows, dim3(gr), dim3(256), 0, s, ptr, n, acc);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if (sup_valid && E) {
+        const uint32_t ge = std::min<uint32_t>(2048, (E + 255) / 256);
+        hipLaunchKernelGGL(k_feat_support, dim3(ge), dim3(256), 0, s, dst, sup, E, k, acc);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(k_feat_finish, dim3(1), dim3(64), 0, s, acc, E0, k, sup_valid ? 1 : 0, feat);
+    return hipGetLastError();
+}
+
+hipError_t mlp_forward(const float* P, uint32_t C, const float* X, uint32_t batch, float* Y,
+                       hipStream_t s) {
+    if (!batch) return hipSuccess;
+    const uint32_t gr = std::min<uint32_t>(batch, 4096);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(gr), dim3(64), 0, s, P, (int)C, X, batch, Y);
+    return hipGetLastError();
+}
+
+hipError_t mlp_loss_grad(const float* P, uint32_t C, const float* X, const float* T, const float* M,
+                         const float* W, uint32_t batch, float inv_wsum, float* part,
+                         float* part_loss, uint32_t nparts, float* grad, hipStream_t s) {
+    const uint32_t total = mlp_num_params(C);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(nparts), dim3(64), total * sizeof(float), s, P, (int)C,
+                       X, T, M, W, batch, inv_wsum, part, part_loss);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_reduce_parts, dim3((total + 255) / 256), dim3(256), 0, s, part, nparts,
+                       total, grad);
+    return hipGetLastError();
+}
+
+hipError_t adam_step(float* P, const float* G, float* m1, float* m2, uint32_t total, float lr,
+                     uint32_t t, hipStream_t s) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((total + 255) / 256), dim3(256), 0, s, P, G, m1, m2, total, lr,
+                       b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t mlp_pick(const float* P, uint32_t C, const float* feat, unsigned int* choice,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_pick, dim3(1), dim3(64), 0, s, P, (int)C, feat, choice);
+    return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace kt

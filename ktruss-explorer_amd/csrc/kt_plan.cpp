@@ -17,6 +17,7 @@
 
 #include "kt_host.h"
 #include "kt_internal.h"
+#include "kt_learn.h"
 #include "kt_plan.h"
 
 namespace kt {
@@ -81,6 +82,9 @@ static int ensure_device() {
 
 Plan::~Plan() {
     if (h_ctr) (void)hipHostFree(h_ctr);
+    if (h_feat) (void)hipHostFree(h_feat);
+    for (auto& ev : ev_sw)
+        if (ev) (void)hipEventDestroy(ev);
     for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1})
         if (*ev) (void)hipEventDestroy(*ev);
     if (stream) (void)hipStreamDestroy(stream);
@@ -126,25 +130,21 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     }
     // LDS slice per wave: the largest power of two up to the longest row that
     // keeps a workgroup within 64 KiB
-    if (v.count_kernel > KT_COUNT_MERGE) return fail(KT_ERR_ARG, "unknown count_kernel");
-    if (v.count_kernel == KT_COUNT_AUTO) v.count_kernel = KT_COUNT_ROW;
-    if (!directed || tiled) v.count_kernel = KT_COUNT_ROW;
-    if (v.cap_lds) {
-        if (v.cap_lds < 64 || v.cap_lds > 4096 || (v.cap_lds & (v.cap_lds - 1)))
-            return fail(KT_ERR_ARG, "cap_lds must be a power of two in [64, 4096]");
-        cap_lds = v.cap_lds;
-    } else {
-        cap_lds = std::min<uint32_t>(4096, std::max<uint32_t>(64, (uint32_t)next_pow2(max_row)));
+    {
+        kt_count_config c{v.count_kernel, v.block_size, v.cap_lds, v.chunk};
+        if ((rc = resolve(c, fixed_cfg))) return rc;
+        v.count_kernel = fixed_cfg.kernel;
+        cap_lds = fixed_cfg.cap;
     }
-    while (cap_lds > 64 &&
-           dev::count_lds_bytes(cap_lds, v.block_size, directed, tiled, v.count_kernel) > 65536)
-        cap_lds >>= 1;
     query_factor = 8;
 
     KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1}) KT_HIP_TRY(hipEventCreate(ev));
     KT_HIP_TRY(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocDefault));
     std::memset(h_ctr, 0, sizeof(DevCounters));
+    KT_HIP_TRY(hipHostMalloc((void**)&h_feat, KT_MLP_IN * sizeof(float), hipHostMallocDefault));
+    std::memset(h_feat, 0, KT_MLP_IN * sizeof(float));
+    for (auto& ev : ev_sw) KT_HIP_TRY(hipEventCreate(&ev));
 
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
     const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
@@ -156,7 +156,10 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     if ((rc = state.alloc(std::max<uint32_t>(E0, 1))) || (rc = nflag.alloc(std::max<uint32_t>(n, 1))) ||
         (rc = wl.alloc(Eb)) || (rc = tau.alloc(Eb)) ||
         (rc = tiles.alloc(((size_t)dev::compact_tiles(E0) + 2) * sizeof(uint32_t))) ||
-        (rc = xs.alloc(Eb + sizeof(uint32_t))) || (rc = ctr.alloc(sizeof(DevCounters))))
+        (rc = xs.alloc(Eb + sizeof(uint32_t))) || (rc = ctr.alloc(sizeof(DevCounters))) ||
+        (rc = ctr_scratch.alloc(sizeof(DevCounters))) ||
+        (rc = feat_acc.alloc(KT_FEAT_ACC * sizeof(unsigned long long))) ||
+        (rc = feat.alloc(KT_MLP_IN * sizeof(float))))
         return rc;
     KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
     if (E0) {
@@ -195,6 +198,7 @@ int Plan::reset() {
     KT_HIP_TRY(hipStreamSynchronize(stream));
     std::memset(h_ctr, 0, sizeof(DevCounters));
     trace.clear();
+    phase_log.clear();
     stats = kt_stats{};
     have_tau = false;
     return KT_OK;
@@ -219,7 +223,57 @@ int Plan::compact_now() {
     return KT_OK;
 }
 
-CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems) {
+int Plan::resolve(const kt_count_config& in, CountCfg& out) const {
+    if (in.count_kernel > KT_COUNT_MERGE) return fail(KT_ERR_ARG, "unknown count_kernel");
+    out.kernel = in.count_kernel == KT_COUNT_AUTO ? (uint32_t)KT_COUNT_ROW : in.count_kernel;
+    if (!directed || tiled) out.kernel = (uint32_t)KT_COUNT_ROW;  // the other strategies exist for directed COO+CSR
+    out.block = in.block_size ? in.block_size : 512;
+    if (out.block % 64 || out.block > 1024)
+        return fail(KT_ERR_ARG, "block_size must be a multiple of 64 up to 1024");
+    out.chunk = in.chunk ? in.chunk : 128;
+    if (out.chunk % 64) return fail(KT_ERR_ARG, "chunk must be a multiple of 64");
+    if (in.cap_lds) {
+        if (in.cap_lds < 64 || in.cap_lds > 4096 || (in.cap_lds & (in.cap_lds - 1)))
+            return fail(KT_ERR_ARG, "cap_lds must be a power of two in [64, 4096]");
+        out.cap = in.cap_lds;
+    } else {
+        out.cap = std::min<uint32_t>(4096, std::max<uint32_t>(64, (uint32_t)next_pow2(max_row)));
+    }
+    // the LDS slice per wave shrinks until a workgroup fits in 64 KiB
+    while (out.cap > 64 && dev::count_lds_bytes(out.cap, out.block, directed, tiled, out.kernel) > 65536)
+        out.cap >>= 1;
+    return KT_OK;
+}
+
+int Plan::set_candidates(const kt_count_config* c, uint32_t n) {
+    if (n > KT_MAX_CANDIDATES || (n && !c)) return fail(KT_ERR_ARG, "at most 16 candidates");
+    std::vector<CountCfg> out(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const int rc = resolve(c[i], out[i]);
+        if (rc) return rc;
+    }
+    cands = std::move(out);
+    if (pick != KT_PICK_FIXED && cands.empty()) pick = KT_PICK_FIXED;
+    return KT_OK;
+}
+
+int Plan::set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C) {
+    if (mode > KT_PICK_MODEL) return fail(KT_ERR_ARG, "unknown pick mode");
+    if (mode != KT_PICK_FIXED && cands.empty()) return fail(KT_ERR_STATE, "set candidates first");
+    if (mode == KT_PICK_MODEL) {
+        if (!params || C != cands.size())
+            return fail(KT_ERR_ARG, "the model must have one output per candidate");
+        int rc = model.alloc((size_t)nparams * sizeof(float));
+        if (rc) return rc;
+        KT_HIP_TRY(hipMemcpy(model.p, params, (size_t)nparams * sizeof(float), hipMemcpyDeviceToDevice));
+        model_C = C;
+    }
+    pick = mode;
+    return KT_OK;
+}
+
+CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const CountCfg& cfg,
+                           DevCounters* counters) {
     CountArgs a;
     a.ptr = ptr[cur].as<uint32_t>();
     a.src = src[cur].as<uint32_t>();
@@ -227,14 +281,15 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems) {
     a.sup = sup[cur].as<uint32_t>();
     a.wl = use_wl ? wl.as<uint32_t>() : nullptr;
     a.nitems = nitems;
-    a.chunk = v.chunk;
-    a.cap_lds = cap_lds;
+    a.chunk = cfg.chunk;
+    a.cap_lds = cfg.cap;
     a.cap = k - 2;
     a.tile_size = tile_size;
     a.ytiles = ytiles;
     a.query_factor = query_factor;
     a.prefilter = (!directed && !tiled) ? 1u : 0u;  // kernel_duc.cu:27; kernel_dut.cu has none
-    a.ctr = ctr.as<DevCounters>();
+    a.ctr = counters;
+    a.nitems_dev = nullptr;
     return a;
 }
 
@@ -250,17 +305,51 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     stats.phases++;
     // initInfoOnDevice (info.cu:55-60): every entry directly affected
     if (affected) KT_HIP_TRY(dev::fill_u8(state.as<uint8_t>(), KT_DIRECTLY_AFFECTED, E, stream));
+    // phase features (exact supports only when handed over by REUSE) and the pick
+    const uint32_t nc = (uint32_t)cands.size();
+    std::array<float, KT_PHASE_REC> rec;
+    rec.fill(-1.f);
+    KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                   sup[cur].as<uint32_t>(), n, E, E0, k, reuse,
+                                   feat_acc.as<unsigned long long>(), feat.as<float>(), stream));
+    CountCfg cfg = fixed_cfg;
+    if (pick == KT_PICK_MODEL)
+        KT_HIP_TRY(dev::mlp_pick(model.as<float>(), model_C, feat.as<float>(),
+                                 &ctr.as<DevCounters>()->choice, stream));
+    KT_HIP_TRY(hipMemcpyAsync(h_feat, feat.p, KT_MLP_IN * sizeof(float), hipMemcpyDeviceToHost, stream));
+    uint32_t chosen = 0;
+    if (pick == KT_PICK_MODEL) {
+        if ((rc = sync_counters())) return rc;
+        chosen = std::min<uint32_t>(h_ctr->choice, nc - 1);
+        cfg = cands[chosen];
+    }
+    std::vector<double> cand_ms(nc, 0.0);
+    double phase_ms = 0.0;
     do {
         if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
         if (affected) KT_HIP_TRY(dev::fill_u8(nflag.as<uint8_t>(), KT_NOT_AFFECTED, n, stream));
         bool counted = false;
         const uint64_t rec_before = h_ctr->recount_edges;
         if (!(iter == 0 && reuse)) {
-            if (!use_wl) KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
-            CountArgs a = count_args(k, use_wl, use_wl ? nitems : E);
-            KT_HIP_TRY(hipEventRecord(ev_c0, stream));
-            KT_HIP_TRY(dev::count_supports(a, directed, tiled, v.count_kernel, v.block_size, stream));
-            KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+            const uint32_t items = use_wl ? nitems : E;
+            if (pick == KT_PICK_SWEEP) {
+                // every candidate from zeroed supports; the last one's result is kept
+                for (uint32_t c = 0; c < nc; ++c) {
+                    if (use_wl) KT_HIP_TRY(dev::zero_listed_supports(wl.as<uint32_t>(), nitems, sup[cur].as<uint32_t>(), stream));
+                    else KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
+                    CountArgs a = count_args(k, use_wl, items, cands[c],
+                                             c + 1 == nc ? ctr.as<DevCounters>() : ctr_scratch.as<DevCounters>());
+                    KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
+                    KT_HIP_TRY(dev::count_supports(a, directed, tiled, cands[c].kernel, cands[c].block, stream));
+                    KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
+                }
+            } else {
+                if (!use_wl) KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
+                CountArgs a = count_args(k, use_wl, items, cfg, ctr.as<DevCounters>());
+                KT_HIP_TRY(hipEventRecord(ev_c0, stream));
+                KT_HIP_TRY(dev::count_supports(a, directed, tiled, cfg.kernel, cfg.block, stream));
+                KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+            }
             counted = true;
             stats.count_passes++;
             stats.live_in_passes += live;
@@ -280,11 +369,23 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
                                      ctr.as<DevCounters>(), stream));
         if ((rc = sync_counters())) return rc;
         float count_ms = 0;
-        if (counted) {
+        if (counted && pick == KT_PICK_SWEEP) {
+            float best = 0;
+            for (uint32_t c = 0; c < nc; ++c) {
+                float ms = 0;
+                KT_HIP_TRY(hipEventElapsedTime(&ms, ev_sw[2 * c], ev_sw[2 * c + 1]));
+                cand_ms[c] += ms;
+                if (c == 0 || ms < best) best = ms;
+            }
+            count_ms = best;
+            stats.ms_count += count_ms;
+            stats.count_launches++;
+        } else if (counted) {
             KT_HIP_TRY(hipEventElapsedTime(&count_ms, ev_c0, ev_c1));
             stats.ms_count += count_ms;
             stats.count_launches++;
         }
+        phase_ms += count_ms;
         const uint32_t ndel = h_ctr->ndel;
         live -= ndel;
         stats.iterations++;
@@ -321,6 +422,20 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     } while (changed);
     if (has_del && (rc = compact_now())) return rc;  // kernel_ddc.cu:206-212
     exact_out = !broke && !changed;
+    // phase record
+    if (pick == KT_PICK_SWEEP && nc) {
+        chosen = 0;
+        for (uint32_t c = 1; c < nc; ++c)
+            if (cand_ms[c] < cand_ms[chosen]) chosen = c;
+        for (uint32_t c = 0; c < nc; ++c) rec[20 + c] = (float)cand_ms[c];
+        phase_ms = cand_ms[chosen];
+    }
+    rec[0] = (float)k;
+    rec[1] = (float)iter + (broke ? 1.f : 0.f);
+    rec[2] = pick == KT_PICK_FIXED ? -1.f : (float)chosen;
+    rec[3] = (float)phase_ms;
+    for (int f = 0; f < KT_MLP_IN; ++f) rec[4 + f] = h_feat[f];
+    phase_log.push_back(rec);
     stats.k_last = k;
     return KT_OK;
 }

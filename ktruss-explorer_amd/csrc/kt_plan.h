@@ -25,6 +25,11 @@ struct DBuf {
 };
 
 int init_device(int device);
+
+// Count configuration resolved for a plan's layout.
+struct CountCfg {
+    uint32_t kernel = KT_COUNT_ROW, block = 512, cap = 64, chunk = 128;
+};
 const kt_graph* graph_retain(const kt_graph* g);
 void graph_release(const kt_graph* g);
 
@@ -51,6 +56,21 @@ struct Plan {
     kt_stats stats{};
     bool have_tau = false;
 
+    // count configurations: the variant's own, the candidates, the pick mode
+    CountCfg fixed_cfg;
+    std::vector<CountCfg> cands;
+    uint32_t pick = KT_PICK_FIXED;
+    DBuf model;          // cost-model parameters (KT_PICK_MODEL)
+    uint32_t model_C = 0;
+    DBuf feat_acc, feat, ctr_scratch;
+    float* h_feat = nullptr;
+    hipEvent_t ev_sw[2 * KT_MAX_CANDIDATES] = {};
+    std::vector<std::array<float, KT_PHASE_REC>> phase_log;
+
+    int resolve(const kt_count_config& in, CountCfg& out) const;
+    int set_candidates(const kt_count_config* c, uint32_t n);
+    int set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C);
+
     ~Plan();
     int create(const kt_graph* g, const kt_variant* v);
     int reset();
@@ -63,7 +83,8 @@ struct Plan {
     int compact_now();
     int sync_counters();
     int set_counter(size_t offset, uint32_t value);
-    CountArgs count_args(uint32_t k, bool use_wl, uint32_t nitems);
+    CountArgs count_args(uint32_t k, bool use_wl, uint32_t nitems, const CountCfg& cfg,
+                         DevCounters* counters);
     void finish_stats();
 };
 

@@ -40,8 +40,13 @@ EXPORTS = [
     "kt_read_tsv", "kt_write_tsv", "kt_verify_edges", "kt_generate_rmat",
     "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
     "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
-    "kt_plan_trace",
+    "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
+    "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
+    "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
 ]
+
+MLP_IN, MLP_HIDDEN, MLP_MAX_OUT, MAX_CANDIDATES, PHASE_REC = 16, 64, 16, 16, 36
+PICK_FIXED, PICK_SWEEP, PICK_MODEL = 0, 1, 2
 
 
 class KtError(RuntimeError):
@@ -69,6 +74,15 @@ class Variant(ctypes.Structure):
             s += f"/t{self.tiles_per_dim}" if not self.tile_size else f"/S{self.tile_size}"
             s += f"/y{self.num_parallel_tiles}"
         return s
+
+
+class CountConfig(ctypes.Structure):
+    _fields_ = [("count_kernel", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
+                ("cap_lds", ctypes.c_uint32), ("chunk", ctypes.c_uint32)]
+
+    def describe(self):
+        names = {0: "auto", 1: "row", 2: "search", 3: "merge"}
+        return f"{names[self.count_kernel]}/b{self.block_size}/c{self.cap_lds or 'auto'}/k{self.chunk or 128}"
 
 
 class Stats(ctypes.Structure):
@@ -131,6 +145,22 @@ def lib():
         L.kt_plan_decompose.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.kt_plan_truss_numbers.argtypes = [ctypes.c_void_p, _u32p, ctypes.c_uint64]
         L.kt_plan_trace.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, _u64p]
+        _f32p = ctypes.POINTER(ctypes.c_float)
+        L.kt_plan_set_candidates.argtypes = [ctypes.c_void_p, ctypes.POINTER(CountConfig), ctypes.c_uint32]
+        L.kt_plan_set_pick.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.kt_plan_phase_log.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64, _u64p]
+        L.kt_mlp_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+        L.kt_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.kt_mlp_destroy.restype = None
+        L.kt_mlp_num_params.argtypes = [ctypes.c_void_p, _u32p]
+        L.kt_mlp_get_params.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32]
+        L.kt_mlp_set_params.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32]
+        L.kt_mlp_forward.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, _f32p]
+        L.kt_mlp_loss_grad.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32,
+                                       _f32p, _f32p]
+        L.kt_mlp_train.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64,
+                                   _f32p]
         _lib = L
     return _lib
 
@@ -289,6 +319,102 @@ class Plan:
                "kt_plan_trace")
         w = 5 if with_time else 4
         return [tuple(int(x) for x in buf[5 * i:5 * i + w]) for i in range(n.value)]
+
+
+    def set_candidates(self, configs):
+        """configs: CountConfig list (at most 16) tried per phase by SWEEP / MODEL."""
+        arr = (CountConfig * max(len(configs), 1))(*configs)
+        _check(lib().kt_plan_set_candidates(self._h, arr, len(configs)), "kt_plan_set_candidates")
+        self.candidates = list(configs)
+
+    def set_pick(self, mode, model=None):
+        _check(lib().kt_plan_set_pick(self._h, mode, model._h if model is not None else None),
+               "kt_plan_set_pick")
+
+    def phase_log(self):
+        """(phases, 36) float32: k, iterations, chosen, phase count ms, 16 features,
+        16 per-candidate count ms (SWEEP)."""
+        n = ctypes.c_uint64()
+        _check(lib().kt_plan_phase_log(self._h, None, 0, ctypes.byref(n)), "kt_plan_phase_log")
+        buf = np.zeros((max(n.value, 1), PHASE_REC), np.float32)
+        _check(lib().kt_plan_phase_log(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       n.value, ctypes.byref(n)), "kt_plan_phase_log")
+        return buf[: n.value].copy()
+
+
+def count_config(kernel="row", block_size=512, cap_lds=0, chunk=0):
+    return CountConfig(COUNT_KERNELS[kernel] if isinstance(kernel, str) else kernel, block_size,
+                       cap_lds, chunk)
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The count-kernel cost model: 16 -> 64 -> 64 -> n_out, FP32, on the GPU."""
+
+    def __init__(self, n_out, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().kt_mlp_create(n_out, seed, ctypes.byref(h)), "kt_mlp_create")
+        self._h = h
+        self.n_out = n_out
+        n = ctypes.c_uint32()
+        _check(lib().kt_mlp_num_params(h, ctypes.byref(n)), "kt_mlp_num_params")
+        self.num_params = n.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.kt_mlp_destroy(h)
+            self._h = None
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        _check(lib().kt_mlp_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       self.num_params), "kt_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p, pp = _f32(p)
+        _check(lib().kt_mlp_set_params(self._h, pp, len(p)), "kt_mlp_set_params")
+
+    def forward(self, x):
+        x, xp = _f32(x)
+        b = x.shape[0]
+        y = np.zeros((b, self.n_out), np.float32)
+        _check(lib().kt_mlp_forward(self._h, xp, b, y.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "kt_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t, mask=None, w=None):
+        x, xp = _f32(x)
+        t, tp = _f32(t)
+        mp = _f32(mask)[1] if mask is not None else None
+        m_keep = _f32(mask)[0] if mask is not None else None
+        w_keep = _f32(w)[0] if w is not None else None
+        wp = w_keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if w is not None else None
+        if m_keep is not None:
+            mp = m_keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        loss = ctypes.c_float()
+        g = np.zeros(self.num_params, np.float32)
+        _check(lib().kt_mlp_loss_grad(self._h, xp, tp, mp, wp, x.shape[0], ctypes.byref(loss),
+                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "kt_mlp_loss_grad")
+        return float(loss.value), g
+
+    def train(self, x, t, mask=None, w=None, epochs=50, batch=256, lr=1e-3, seed=1):
+        x, xp = _f32(x)
+        t, tp = _f32(t)
+        m_keep = _f32(mask)[0] if mask is not None else None
+        w_keep = _f32(w)[0] if w is not None else None
+        mp = m_keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if m_keep is not None else None
+        wp = w_keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if w_keep is not None else None
+        loss = ctypes.c_float()
+        _check(lib().kt_mlp_train(self._h, xp, tp, mp, wp, x.shape[0], epochs, batch, lr, seed,
+                                  ctypes.byref(loss)), "kt_mlp_train")
+        return float(loss.value)
 
 
 def _gen_out():

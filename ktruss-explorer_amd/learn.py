@@ -1,0 +1,260 @@
+"""Learned per-phase pick of the support-count configuration.
+
+The peel's results do not depend on how supports are counted, only its speed
+does, and the fastest count configuration changes as the graph shrinks from
+a sparse skewed graph to dense cores.  This module
+
+  collect()   decomposes seeded synthetic graphs in SWEEP mode: every count
+              pass runs every candidate configuration and is timed, and each
+              phase yields one sample (16 features, per-candidate count ms);
+  train()     fits the cost model (16 -> 64 -> 64 -> C MLP, trained on the GPU
+              by the engine's own forward/backward/Adam kernels) to predict
+              each candidate's log cost relative to the others;
+  evaluate()  decomposes a held-out graph with the fixed default variant, with
+              the model picking per phase, and in SWEEP mode (the exhaustive
+              per-phase judge), and reports speed-up, regret and parity.
+
+Command line:
+  python learn.py collect OUT.npz [--graphs N] [--seed S] [--max-scale X]
+  python learn.py train DATA.npz MODEL.npz [--epochs E]
+  python learn.py eval MODEL.npz [--scale 20]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import ktruss_amd as K  # noqa: E402
+
+MODEL_PATH = os.path.join(HERE, "picker", "count_picker.npz")
+
+# Candidate count configurations; index 0 is the default variant's own.
+CANDIDATES = [
+    ("row", 512, 0),
+    ("row", 256, 1024),
+    ("row", 256, 512),
+    ("row", 128, 2048),
+    ("row", 1024, 512),
+    ("row", 256, 2048),
+    ("search", 256, 1024),
+    ("merge", 256, 1024),
+]
+
+
+def candidate_configs():
+    return [K.count_config(k, b, c) for k, b, c in CANDIDATES]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# graphs
+
+def graph_specs(count, seed, max_scale=17, min_scale=10):
+    """Seeded mix of R-MAT, Chung-Lu (with planted cliques) and Erdos-Renyi
+    graphs, sizes log-uniform in [2^min_scale, 2^max_scale] vertices."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        kind = ("rmat", "cl", "er")[i % 3]
+        scale = int(rng.integers(min_scale, max_scale + 1))
+        gseed = int(rng.integers(1, 1 << 30))
+        if kind == "rmat":
+            a = float(rng.uniform(0.45, 0.6))
+            bc = (1 - a) * float(rng.uniform(0.35, 0.45))
+            specs.append({"kind": "rmat", "scale": scale, "ef": int(rng.choice([8, 16, 24])),
+                          "a": a, "b": bc, "c": bc, "seed": gseed})
+        elif kind == "cl":
+            specs.append({"kind": "cl", "n": 1 << scale, "gamma": float(rng.uniform(2.0, 2.8)),
+                          "deg": float(rng.uniform(8, 32)), "maxw": float(2 ** rng.uniform(8, 13)),
+                          "cliques": int(rng.integers(0, 9)), "seed": gseed})
+        else:
+            specs.append({"kind": "er", "n": 1 << scale, "deg": float(2 ** rng.uniform(3, 6)),
+                          "seed": gseed})
+    return specs
+
+
+def make_graph(spec):
+    if spec["kind"] == "rmat":
+        n, s, d = K.generate_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"],
+                                  spec["seed"])
+    elif spec["kind"] == "cl":
+        n, s, d = K.generate_chung_lu(spec["n"], spec["gamma"], spec["deg"], spec["maxw"],
+                                      spec["cliques"], 20, 200, spec["seed"])
+    else:
+        n, s, d = K.generate_erdos_renyi(spec["n"], spec["deg"], spec["seed"])
+    return K.Graph(n, s, d, pairs=True)
+
+
+# ---------------------------------------------------------------------------
+# data
+
+def sweep(graph, variant=None):
+    """SWEEP decomposition: (truss numbers, phase log)."""
+    p = K.Plan(graph, variant or K.default_variant())
+    p.set_candidates(candidate_configs())
+    p.set_pick(K.PICK_SWEEP)
+    p.decompose()
+    return p.truss_numbers(), p.phase_log()
+
+
+def collect(count, seed, max_scale=17, min_scale=10):
+    feats, costs, meta = [], [], []
+    specs = graph_specs(count, seed, max_scale, min_scale)
+    t0 = time.time()
+    for i, spec in enumerate(specs):
+        g = make_graph(spec)
+        _, ph = sweep(g)
+        C = len(CANDIDATES)
+        for r in ph:
+            ms = r[20:20 + C]
+            if np.all(ms > 0):
+                feats.append(r[4:20])
+                costs.append(ms)
+                meta.append((i, r[0], r[1]))
+        log(f"[{i + 1}/{len(specs)}] {spec['kind']} m={g.m} phases={len(ph)} "
+            f"samples={len(feats)} elapsed={time.time() - t0:.0f}s")
+    return (np.array(feats, np.float32), np.array(costs, np.float32),
+            np.array(meta, np.float32), specs)
+
+
+def targets(costs):
+    """Per-candidate log2 cost relative to the sample's mean log2 cost."""
+    lc = np.log2(np.maximum(costs, 1e-4))
+    return (lc - lc.mean(1, keepdims=True)).astype(np.float32)
+
+
+def weights(costs):
+    """Samples weighted by the square root of their default-variant cost, so
+    the phases that dominate the run dominate the fit without a handful of
+    huge phases drowning the rest."""
+    w = np.sqrt(np.maximum(costs[:, 0], 1e-4))
+    return (w / w.mean()).astype(np.float32)
+
+
+def train(feats, costs, epochs=300, batch=256, lr=2e-3, seed=1):
+    C = costs.shape[1]
+    m = K.Mlp(C, seed=seed)
+    t = targets(costs)
+    w = weights(costs)
+    loss = m.train(feats, t, None, w, epochs=epochs, batch=min(batch, len(feats)), lr=lr, seed=seed)
+    return m, loss
+
+
+def picks_regret(model, feats, costs):
+    """On a sample set: cost of the model's picks vs the per-phase best and
+    vs candidate 0 (the default)."""
+    pred = model.forward(feats)
+    choice = pred.argmin(1)
+    picked = costs[np.arange(len(costs)), choice].sum()
+    best = costs.min(1).sum()
+    default = costs[:, 0].sum()
+    single = costs.sum(0).min()
+    return {"picked_ms": float(picked), "best_ms": float(best), "default_ms": float(default),
+            "best_single_ms": float(single), "regret_vs_best": float(picked / best - 1),
+            "speedup_vs_default": float(default / picked)}
+
+
+def save_model(model, path=MODEL_PATH, info=None):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    cands = np.array([[K.COUNT_KERNELS[k], b, c] for k, b, c in CANDIDATES], np.uint32)
+    np.savez(path, params=model.get_params(), candidates=cands,
+             info=np.array(json.dumps(info or {})))
+
+
+def load_model(path=MODEL_PATH):
+    with np.load(path, allow_pickle=False) as z:
+        params = z["params"]
+        cands = z["candidates"]
+    configs = [K.count_config(int(k), int(b), int(c)) for k, b, c in cands]
+    m = K.Mlp(len(configs))
+    m.set_params(params)
+    return m, configs
+
+
+def evaluate(graph, model, configs):
+    """Fixed default vs model-picked vs exhaustive per-phase best."""
+    out = {}
+    p = K.Plan(graph, K.default_variant())
+    st = p.decompose()
+    tau_default = p.truss_numbers()
+    out["default_ms"] = st["ms_total"]
+    out["default_count_ms"] = st["ms_count"]
+    p.set_candidates(configs)
+    p.set_pick(K.PICK_MODEL, model)
+    st = p.decompose()
+    out["model_ms"] = st["ms_total"]
+    out["model_count_ms"] = st["ms_count"]
+    out["model_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
+    ph = p.phase_log()
+    out["model_picks"] = np.bincount(ph[:, 2].astype(int), minlength=len(configs)).tolist()
+    p.set_pick(K.PICK_SWEEP)
+    p.decompose()
+    out["sweep_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
+    ph = p.phase_log()
+    ms = ph[:, 20:20 + len(configs)]
+    out["exhaustive_count_ms"] = float(ms.min(1).sum())
+    out["best_single_count_ms"] = float(ms.sum(0).min())
+    out["best_single"] = int(ms.sum(0).argmin())
+    out["speedup_vs_default"] = out["default_ms"] / out["model_ms"]
+    out["regret_vs_exhaustive"] = out["model_count_ms"] / out["exhaustive_count_ms"] - 1
+    out["regret_vs_best_single"] = out["model_count_ms"] / out["best_single_count_ms"] - 1
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("out")
+    c.add_argument("--graphs", type=int, default=300)
+    c.add_argument("--seed", type=int, default=7)
+    c.add_argument("--max-scale", type=int, default=17)
+    c.add_argument("--min-scale", type=int, default=10)
+    t = sub.add_parser("train")
+    t.add_argument("data", nargs="+")
+    t.add_argument("--out", default=MODEL_PATH)
+    t.add_argument("--epochs", type=int, default=300)
+    e = sub.add_parser("eval")
+    e.add_argument("--model", default=MODEL_PATH)
+    e.add_argument("--scale", type=int, default=20)
+    args = ap.parse_args()
+    K.init(0)
+    if args.cmd == "collect":
+        f, c_, meta, specs = collect(args.graphs, args.seed, args.max_scale, args.min_scale)
+        np.savez(args.out, feats=f, costs=c_, meta=meta, specs=np.array(json.dumps(specs)))
+        log(f"wrote {len(f)} samples to {args.out}")
+    elif args.cmd == "train":
+        fs, cs = [], []
+        for path in args.data:
+            with np.load(path, allow_pickle=False) as z:
+                fs.append(z["feats"])
+                cs.append(z["costs"])
+        f, c_ = np.concatenate(fs), np.concatenate(cs)
+        rng = np.random.default_rng(0)
+        idx = rng.permutation(len(f))
+        cut = int(0.85 * len(f))
+        tr, va = idx[:cut], idx[cut:]
+        model, loss = train(f[tr], c_[tr], epochs=args.epochs)
+        info = {"train_samples": int(len(tr)), "loss": loss,
+                "train": picks_regret(model, f[tr], c_[tr]),
+                "validation": picks_regret(model, f[va], c_[va])}
+        save_model(model, args.out, info)
+        print(json.dumps(info))
+    else:
+        model, configs = load_model(args.model)
+        n, s, d = K.generate_rmat(args.scale, 16, seed=1)
+        g = K.Graph(n, s, d, pairs=True)
+        print(json.dumps(evaluate(g, model, configs)))
+
+
+if __name__ == "__main__":
+    main()
