@@ -247,6 +247,75 @@ int kt_mlp_train(kt_mlp* m, const float* x, const float* t, const float* mask, c
                  uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
                  float* loss_out);
 
+/* ---- batched decomposition ----------------------------------------------
+ * No counterpart in the reference (its ktruss(COOGraph*, COOGraph*, Config),
+ * kernel.h:12, takes one graph per process).  Many independent graphs are
+ * laid out as one disjoint union on the device and decomposed together: every
+ * peel step advances every unfinished graph by one iteration of its own peel
+ * (its own k), and the steps are captured once as a HIP graph and replayed.
+ * Truss numbers are identical to kt_plan_decompose on each graph alone.
+ * Supported variants: directed (by degree or by index), COO+CSR, recount all. */
+typedef struct kt_batch kt_batch;
+
+typedef struct kt_batch_tuning {
+    uint32_t block_size;       /* count workgroup size (64..1024, x64); 0 = 512   */
+    uint32_t chunk;            /* entries per count task (x64); 0 = 128            */
+    uint32_t cap_lds;          /* LDS slice per wave; 0 = automatic                */
+    uint32_t blocks_per_cu;    /* persistent count workgroups per CU; 0 = 2        */
+    uint32_t mark_chunk;       /* entries per mark task (x64); 0 = 1024            */
+    uint32_t steps_per_replay; /* peel steps per captured graph replay; 0 = 16     */
+} kt_batch_tuning;
+
+typedef struct kt_batch_stats {
+    uint64_t graphs;
+    uint64_t entries;          /* stored entries of all graphs                     */
+    uint64_t steps;            /* peel steps (mark passes over the active graphs)  */
+    uint64_t replays;          /* graph replays launched                           */
+    uint64_t count_passes;     /* sum over graphs                                  */
+    uint64_t iterations;       /* sum over graphs                                  */
+    uint64_t phases;           /* sum over graphs                                  */
+    uint64_t recount_edges;
+    uint64_t merge_elems;
+    uint64_t live_in_passes;
+    uint32_t k_max;            /* largest truss number over all graphs             */
+    uint32_t pad;
+    double ms_total;           /* device time from resident layouts to resident truss numbers */
+    double balg_count_bytes;   /* B_alg of the count passes                        */
+    double ms_count;           /* summed count-kernel time (timing mode only)      */
+    uint64_t count_launches;   /* count-kernel launches                            */
+} kt_batch_stats;
+
+/* graphs[count] stay referenced by the batch until kt_batch_destroy. */
+int kt_batch_create(const kt_graph* const* graphs, uint32_t count, const kt_variant* v,
+                    kt_batch** out);
+void kt_batch_destroy(kt_batch* b);
+/* Graphs, stored entries, longest row over the batch. */
+int kt_batch_info(const kt_batch* b, uint32_t* graphs, uint64_t* entries, uint32_t* max_row);
+int kt_batch_set_tuning(kt_batch* b, const kt_batch_tuning* t);
+/* The resolved tuning in use (zeros replaced by the values chosen). */
+int kt_batch_get_tuning(const kt_batch* b, kt_batch_tuning* t);
+/* Autotuner: coordinate descent over block size, LDS slice, count chunk,
+ * count workgroups per CU and mark chunk.  Each trial runs the first
+ * trial_steps peel steps of the batch from its initial state, best of
+ * `repeats`, timed with hipEvents; the fastest tuning is applied.  The trial
+ * log is read with kt_batch_tune_log. */
+int kt_batch_autotune(kt_batch* b, uint32_t trial_steps, uint32_t repeats, kt_batch_tuning* best);
+/* Trials of the last autotune: tunings and milliseconds; *len is the number
+ * of trials (all of them even when cap is smaller). */
+int kt_batch_tune_log(const kt_batch* b, kt_batch_tuning* cfg, float* ms, uint64_t cap,
+                      uint64_t* len);
+/* Timing mode (on != 0): decompose launches the peel steps one by one instead
+ * of replaying the captured graph and brackets every count launch with
+ * hipEvents, filling stats->ms_count.  Results are the same either way. */
+int kt_batch_set_timing(kt_batch* b, int on);
+int kt_batch_decompose(kt_batch* b, kt_batch_stats* stats);
+/* Truss numbers of graph `graph` of the batch, one per input entry of that
+ * graph (count = its m_input). */
+int kt_batch_truss_numbers(kt_batch* b, uint32_t graph, uint32_t* tau, uint64_t count);
+/* Per-graph counters of the last run: info[0] largest truss number,
+ * [1] peel iterations, [2] phases, [3] count passes. */
+int kt_batch_graph_info(kt_batch* b, uint32_t graph, uint32_t* info);
+
 #ifdef __cplusplus
 }
 #endif

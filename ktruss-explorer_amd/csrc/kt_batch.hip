@@ -1,0 +1,304 @@
+// kt_batch.hip -- gfx950 kernels of the batched decomposition.
+//
+// A batch of G graphs is stored as one disjoint union of oriented COO+CSR
+// layouts: graph g owns the entries [gbeg[g], gend[g]) and a vertex range of
+// its own, so the union's rows never cross graphs and the support count of
+// the single-graph engine runs on it unchanged.  What differs per graph is
+// the peel state (k, live entries, phase), kept in device arrays and advanced
+// by a one-workgroup control kernel.  One peel step of every unfinished graph
+// is
+//     mark (active list) -> control -> expand -> zero (dirty list) -> count
+// and every kernel reads its sizes and task lists from device memory, so the
+// host captures a run of steps once as a HIP graph and replays it, reading
+// one counter block per replay to learn whether graphs are left.
+//
+// Per graph the steps follow the single-graph schedule (kernel_ddc.cu:116-214
+// with the JUMP / REUSE schedule of the engine): the mark uses supports that
+// are exact for the graph's current edge set, so each graph deletes exactly
+// the edges it deletes when decomposed alone and truss numbers are identical.
+#include <hip/hip_runtime.h>
+
+#include "kt_batch.h"
+
+namespace kt {
+namespace {
+
+constexpr uint32_t kWave = 64;
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(x, off);
+        x = o < x ? o : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint32_t gwave() {
+    return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+__device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / kWave; }
+
+// clearIterInfoOnDevice's support reset (info.cu:75) for the graphs about to
+// be recounted.
+__global__ void k_b_zero(TaskList tl, const uint32_t* __restrict__ dst, uint32_t* __restrict__ sup) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t n = *tl.ntasks;
+    for (uint32_t t = gwave(); t < n; t += nwaves()) {
+        const uint4 d = tl.desc[t];
+        for (uint32_t i = d.x + lane; i < d.y; i += kWave)
+            if (!(dst[i] & KT_DEAD)) sup[i] = 0;
+    }
+}
+
+// mark_deleted_edges_kernel (kernel_ddc.cu:56-77) with the graph's own k; the
+// minimum support of the entries that stay alive is folded per graph, which
+// is the JUMP target whenever the graph's phase ends with this step.
+__global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
+                         uint32_t* __restrict__ tau, const uint32_t* __restrict__ gk,
+                         uint32_t* gndel, uint32_t* gmin) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t n = *tl.ntasks;
+    for (uint32_t t = gwave(); t < n; t += nwaves()) {
+        const uint4 d = tl.desc[t];
+        const uint32_t k = gk[d.z];
+        uint32_t ndel = 0, smin = 0xFFFFFFFFu;
+        for (uint32_t base = d.x; base < d.y; base += kWave) {
+            const uint32_t i = base + lane;
+            bool del = false;
+            if (i < d.y) {
+                const uint32_t v = dst[i];
+                if (!(v & KT_DEAD)) {
+                    const uint32_t s = sup[i];
+                    if (s < k - 2) {
+                        del = true;
+                        dst[i] = v | KT_DEAD;
+                        tau[i] = k - 1;
+                    } else if (s < smin) {
+                        smin = s;
+                    }
+                }
+            }
+            ndel += (uint32_t)__popcll(ballot(del));
+        }
+        smin = wave_min(smin);
+        if (lane == 0) {
+            if (ndel) atomicAdd(gndel + d.z, ndel);
+            if (smin != 0xFFFFFFFFu) atomicMin(gmin + d.z, smin);
+        }
+    }
+}
+
+// Minimum live support per graph (the first phase's JUMP, after the first
+// full count).
+__global__ void k_b_min(TaskList tl, const uint32_t* __restrict__ dst,
+                        const uint32_t* __restrict__ sup, uint32_t* gmin) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t n = *tl.ntasks;
+    for (uint32_t t = gwave(); t < n; t += nwaves()) {
+        const uint4 d = tl.desc[t];
+        uint32_t smin = 0xFFFFFFFFu;
+        for (uint32_t i = d.x + lane; i < d.y; i += kWave)
+            if (!(dst[i] & KT_DEAD) && sup[i] < smin) smin = sup[i];
+        smin = wave_min(smin);
+        if (lane == 0 && smin != 0xFFFFFFFFu) atomicMin(gmin + d.z, smin);
+    }
+}
+
+// Block-wide exclusive scan of one value per thread (1024 threads).
+__device__ uint32_t block_exscan(uint32_t x, uint32_t* s_wave, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    uint32_t inc = x;
+    for (int off = 1; off < (int)kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += y;
+    }
+    if (lane == kWave - 1) s_wave[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t nw = blockDim.x / kWave;
+        uint32_t w = lane < nw ? s_wave[lane] : 0;
+        uint32_t wi = w;
+        for (int off = 1; off < (int)kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(wi, off);
+            if ((int)lane >= off) wi += y;
+        }
+        if (lane < nw) s_wave[lane] = wi - w;
+        if (lane == nw - 1) s_wave[32] = wi;
+    }
+    __syncthreads();
+    const uint32_t r = s_wave[wid] + inc - x;
+    total = s_wave[32];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t ntasks_of(uint32_t len, uint32_t chunk) {
+    return (len + chunk - 1) / chunk;
+}
+
+// The per-graph state machine and the next step's task lists.  One
+// workgroup of 1024 threads; thread i owns the contiguous graph slice
+// [i * per, (i + 1) * per), so the lists stay in graph order.
+//   mode 0 (init) : k = 3, every non-empty graph dirty and active
+//   mode 1 (jump) : after the first full count: k = max(3, min support + 3)
+//   mode 2 (step) : after a mark pass: close the phase (no deletion, or the
+//                   k == 3 pass, kernel_ddc.cu:183-186) with k = max(k + 1,
+//                   min surviving support + 3), or schedule a recount
+__global__ __launch_bounds__(1024) void k_b_control(BatchDev b, uint32_t mode) {
+    __shared__ uint32_t s_wave[33];
+    const uint32_t G = b.G;
+    const uint32_t prev_active = b.bc->nactive;  // graphs the mark pass just served
+    const uint32_t per = (G + blockDim.x - 1) / blockDim.x;
+    const uint32_t g0 = min(G, threadIdx.x * per), g1 = min(G, g0 + per);
+    const bool jump = (b.schedule & KT_SCHED_JUMP) != 0;
+    const bool reuse = (b.schedule & KT_SCHED_REUSE) != 0;
+    uint32_t nd = 0, na = 0, td = 0, ta = 0;
+    unsigned long long live_counted = 0;
+    for (uint32_t g = g0; g < g1; ++g) {
+        uint32_t f = b.gflag[g];
+        if (mode == 0) {
+            const uint32_t len = b.gend[g] - b.gbeg[g];
+            b.gk[g] = 3;
+            b.glive[g] = len;
+            b.gndel[g] = 0;
+            b.gmin[g] = 0xFFFFFFFFu;
+            b.giter[g] = 0;
+            b.gphase[g] = 0;
+            b.gkmax[g] = 0;
+            b.gpass[g] = 0;
+            f = len ? KT_BF_DIRTY : KT_BF_DONE;
+        } else if (mode == 1) {
+            if (!(f & KT_BF_DONE)) {
+                const uint32_t m = b.gmin[g];
+                if (jump && m != 0xFFFFFFFFu && m + 3 > b.gk[g]) b.gk[g] = m + 3;
+                b.gmin[g] = 0xFFFFFFFFu;
+                f = 0;  // supports exact: the next step marks
+            }
+        } else if (!(f & KT_BF_DONE)) {
+            const uint32_t k = b.gk[g];
+            const uint32_t ndel = b.gndel[g];
+            const uint32_t live = b.glive[g] - ndel;
+            b.glive[g] = live;
+            b.giter[g] += 1;
+            if (ndel) b.gkmax[g] = k - 1;
+            if (live == 0) {
+                f = KT_BF_DONE;
+                b.gphase[g] += 1;
+            } else if (ndel == 0 || k == 3) {
+                const uint32_t m = b.gmin[g];
+                uint32_t kn = k + 1;
+                if (jump && m + 3 > kn) kn = m + 3;
+                b.gk[g] = kn;
+                b.gphase[g] += 1;
+                f = reuse ? 0u : KT_BF_DIRTY;
+            } else {
+                f = KT_BF_DIRTY;
+            }
+            b.gndel[g] = 0;
+            b.gmin[g] = 0xFFFFFFFFu;
+        }
+        b.gflag[g] = f;
+        if (!(f & KT_BF_DONE)) {
+            const uint32_t len = b.gend[g] - b.gbeg[g];
+            ++na;
+            ta += ntasks_of(len, b.mchunk);
+            if ((f & KT_BF_DIRTY) && mode != 1) {
+                ++nd;
+                td += ntasks_of(len, b.cchunk);
+                live_counted += b.glive[g];
+                b.gpass[g] += 1;
+            }
+        }
+    }
+    uint32_t tot_nd, tot_na, tot_td, tot_ta;
+    uint32_t o_nd = block_exscan(nd, s_wave, tot_nd);
+    uint32_t o_na = block_exscan(na, s_wave, tot_na);
+    uint32_t o_td = block_exscan(td, s_wave, tot_td);
+    uint32_t o_ta = block_exscan(ta, s_wave, tot_ta);
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t f = b.gflag[g];
+        if (f & KT_BF_DONE) continue;
+        const uint32_t len = b.gend[g] - b.gbeg[g];
+        b.alist[o_na] = g;
+        b.ascan[o_na++] = o_ta;
+        o_ta += ntasks_of(len, b.mchunk);
+        if ((f & KT_BF_DIRTY) && mode != 1) {
+            b.dlist[o_nd] = g;
+            b.dscan[o_nd++] = o_td;
+            o_td += ntasks_of(len, b.cchunk);
+        }
+    }
+    // live entries of the graphs recounted next, for the algorithmic bytes
+    for (int off = 32; off > 0; off >>= 1) live_counted += __shfl_xor(live_counted, off);
+    if ((threadIdx.x & (kWave - 1)) == 0 && live_counted)
+        atomicAdd(&b.bc->live_in_passes, live_counted);
+    if (threadIdx.x == 0) {
+        b.bc->ndirty = mode == 1 ? 0u : tot_nd;
+        b.bc->nactive = tot_na;
+        b.bc->dtasks = mode == 1 ? 0u : tot_td;
+        b.bc->atasks = tot_ta;
+        if (mode == 2 && prev_active) b.bc->steps += 1;
+    }
+}
+
+// Task descriptors of both lists: one wave per listed graph writes that
+// graph's tasks.
+__global__ void k_b_expand(BatchDev b) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t nd = b.bc->ndirty, na = b.bc->nactive;
+    for (uint32_t j = gwave(); j < nd + na; j += nwaves()) {
+        const bool dirty = j < nd;
+        const uint32_t jj = dirty ? j : j - nd;
+        const uint32_t g = dirty ? b.dlist[jj] : b.alist[jj];
+        const uint32_t base = dirty ? b.dscan[jj] : b.ascan[jj];
+        const uint32_t chunk = dirty ? b.cchunk : b.mchunk;
+        uint4* out = dirty ? b.ddesc : b.adesc;
+        const uint32_t beg = b.gbeg[g], end = b.gend[g];
+        const uint32_t nt = ntasks_of(end - beg, chunk);
+        for (uint32_t i = lane; i < nt; i += kWave) {
+            const uint32_t x = beg + i * chunk;
+            out[base + i] = make_uint4(x, end - x < chunk ? end : x + chunk, g, 0u);
+        }
+    }
+}
+
+constexpr uint32_t kStreamBlock = 256;
+
+}  // namespace
+
+namespace dev {
+
+hipError_t batch_control(const BatchDev& b, uint32_t mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_control, dim3(1), dim3(1024), 0, s, b, mode);
+    return hipGetLastError();
+}
+
+hipError_t batch_expand(const BatchDev& b, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_expand, dim3(blocks), dim3(kStreamBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, uint32_t blocks,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_b_zero, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup);
+    return hipGetLastError();
+}
+
+hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
+                      const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t blocks,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_b_mark, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau, gk,
+                       gndel, gmin);
+    return hipGetLastError();
+}
+
+hipError_t batch_min(const TaskList& tl, const uint32_t* dst, const uint32_t* sup, uint32_t* gmin,
+                     uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_min, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, gmin);
+    return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace kt

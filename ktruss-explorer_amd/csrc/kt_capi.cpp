@@ -7,6 +7,7 @@
 #include <new>
 #include <vector>
 
+#include "kt_batch_host.h"
 #include "kt_host.h"
 #include "kt_internal.h"
 #include "kt_mlp.h"
@@ -270,6 +271,96 @@ int kt_mlp_train(kt_mlp* m, const float* x, const float* t, const float* mask, c
                  float* loss_out) {
     if (!m || !x || !t) return kt::fail(KT_ERR_ARG, "NULL argument");
     KT_GUARD(return m->impl.train(x, t, mask, w, n, epochs, batch, lr, seed, loss_out);)
+}
+
+// ---- batched decomposition --------------------------------------------------
+
+int kt_batch_create(const kt_graph* const* graphs, uint32_t count, const kt_variant* v,
+                    kt_batch** out) {
+    if (!graphs || !out) return kt::fail(KT_ERR_ARG, "graphs or out is NULL");
+    *out = nullptr;
+    kt_variant dv;
+    kt_variant_default(&dv);
+    KT_GUARD({
+        auto* b = new kt_batch();
+        const int rc = b->impl.create(graphs, count, v ? v : &dv);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        *out = b;
+        return KT_OK;
+    })
+}
+
+void kt_batch_destroy(kt_batch* b) { delete b; }
+
+int kt_batch_info(const kt_batch* b, uint32_t* graphs, uint64_t* entries, uint32_t* max_row) {
+    if (!b) return kt::fail(KT_ERR_ARG, "batch is NULL");
+    if (graphs) *graphs = b->impl.G;
+    if (entries) *entries = b->impl.E;
+    if (max_row) *max_row = b->impl.max_row;
+    return KT_OK;
+}
+
+int kt_batch_set_tuning(kt_batch* b, const kt_batch_tuning* t) {
+    if (!b || !t) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return b->impl.set_tuning(*t);)
+}
+
+int kt_batch_get_tuning(const kt_batch* b, kt_batch_tuning* t) {
+    if (!b || !t) return kt::fail(KT_ERR_ARG, "NULL argument");
+    *t = b->impl.tune;
+    return KT_OK;
+}
+
+int kt_batch_autotune(kt_batch* b, uint32_t trial_steps, uint32_t repeats, kt_batch_tuning* best) {
+    if (!b) return kt::fail(KT_ERR_ARG, "batch is NULL");
+    KT_GUARD({
+        const int rc = b->impl.autotune(trial_steps, repeats);
+        if (rc) return rc;
+        if (best) *best = b->impl.tune;
+        return KT_OK;
+    })
+}
+
+int kt_batch_tune_log(const kt_batch* b, kt_batch_tuning* cfg, float* ms, uint64_t cap,
+                      uint64_t* len) {
+    if (!b || !len) return kt::fail(KT_ERR_ARG, "NULL argument");
+    const auto& c = b->impl.tune_log_cfg;
+    *len = c.size();
+    const uint64_t nn = std::min<uint64_t>(cap, c.size());
+    for (uint64_t i = 0; i < nn; ++i) {
+        if (cfg) cfg[i] = c[i];
+        if (ms) ms[i] = b->impl.tune_log_ms[i];
+    }
+    return KT_OK;
+}
+
+int kt_batch_set_timing(kt_batch* b, int on) {
+    if (!b) return kt::fail(KT_ERR_ARG, "batch is NULL");
+    b->impl.timed = on != 0;
+    return KT_OK;
+}
+
+int kt_batch_decompose(kt_batch* b, kt_batch_stats* stats) {
+    if (!b) return kt::fail(KT_ERR_ARG, "batch is NULL");
+    KT_GUARD({
+        const int rc = b->impl.decompose();
+        if (rc) return rc;
+        if (stats) *stats = b->impl.stats;
+        return KT_OK;
+    })
+}
+
+int kt_batch_truss_numbers(kt_batch* b, uint32_t graph, uint32_t* tau, uint64_t count) {
+    if (!b || (count && !tau)) return kt::fail(KT_ERR_ARG, "NULL argument");
+    KT_GUARD(return b->impl.truss_numbers(graph, tau, count);)
+}
+
+int kt_batch_graph_info(kt_batch* b, uint32_t graph, uint32_t* info) {
+    if (!b || !info) return kt::fail(KT_ERR_ARG, "NULL argument");
+    return b->impl.graph_info(graph, info);
 }
 
 }  // extern "C"

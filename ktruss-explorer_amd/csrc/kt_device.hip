@@ -694,20 +694,20 @@ __global__ __launch_bounds__(1024) void k_count_flat(CountArgs a) {
 // trip overlaps the LDS work of the previous chunk.  u's row is staged in LDS
 // once per run of items of that row; credits to (u,v) and (u,w) go to 16-bit
 // LDS counters, the credit to (v,w) is one global atomic.
-__global__ __launch_bounds__(1024) void k_count_dir(CountArgs a) {
+//
+// BATCH: the kernel is persistent over a task list of many graphs stored as
+// one disjoint union; each wave walks tasks wave_id, wave_id + W, ... and a
+// task is `chunk` consecutive entries of one graph.
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_count_dir(CountArgs a, TaskList tl) {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t nw = blockDim.x / kWave;
     const uint32_t cap = a.cap_lds;
     uint32_t* s_val = lds + wid * (cap + cap / 2);
     uint32_t* s_cnt = s_val + cap;
-    const uint32_t task = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
-    const uint64_t first = (uint64_t)task * a.chunk;
-    const uint32_t nitems = a.nitems_dev ? *a.nitems_dev : a.nitems;
-    if (first >= nitems) return;
-    const uint32_t i0 = (uint32_t)first;
-    const uint32_t i1 = nitems - i0 < a.chunk ? nitems : i0 + a.chunk;
+    const uint32_t task0 = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
     const uint32_t* ptr = a.ptr;
     const uint32_t* dst = a.dst;
     uint32_t* sup = a.sup;
@@ -737,6 +737,7 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a) {
         }
     };
 
+    auto run = [&](uint32_t i0, uint32_t i1) {
     for (uint32_t base = i0; base < i1; base += kWave) {
         const uint32_t my = base + lane;
         uint32_t e = 0, u = 0, v = KT_DEAD, ub = 0, ulen = 0, vb = 0, vlen = 0;
@@ -827,6 +828,21 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a) {
             j0 = nj0;
             w = nw_;
         }
+    }
+    };
+    if (BATCH) {
+        const uint32_t ntasks = *tl.ntasks;
+        const uint32_t W = gridDim.x * nw;
+        for (uint32_t t = task0; t < ntasks; t += W) {
+            const uint4 d = tl.desc[t];
+            run(d.x, d.y);
+        }
+    } else {
+        const uint64_t first = (uint64_t)task0 * a.chunk;
+        const uint32_t nitems = a.nitems;
+        if (first >= nitems) return;
+        const uint32_t i0 = (uint32_t)first;
+        run(i0, nitems - i0 < a.chunk ? nitems : i0 + a.chunk);
     }
     flush();
     for (int off = 32; off > 0; off >>= 1) {
@@ -978,17 +994,25 @@ __global__ void k_build_worklist(const uint32_t* __restrict__ src, const uint32_
     }
 }
 
+// Minimum live support.  Grid-stride over at most 2048 workgroups, wave
+// butterflies, then one atomic per workgroup: the counter sees a few
+// thousand atomics instead of one per wave of the whole array.
 __global__ void k_min_live(const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
                            uint32_t E, DevCounters* ctr) {
+    __shared__ uint32_t s_min[kBlock / kWave];
     uint32_t m = 0xFFFFFFFFu;
     for (uint32_t e = gtid(); e < E; e += gsize())
         if (!(dst[e] & KT_DEAD) && sup[e] < m) m = sup[e];
-    // wave min via butterfly, then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t o = __shfl_xor(m, off);
         m = o < m ? o : m;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0 && m != 0xFFFFFFFFu) atomicMin(&ctr->min_sup, m);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_min[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / kWave; ++w) m = s_min[w] < m ? s_min[w] : m;
+        if (m != 0xFFFFFFFFu) atomicMin(&ctr->min_sup, m);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1126,7 +1150,7 @@ hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_
     }
     if (directed && !tiled) {
         const size_t lds = (size_t)nw * (a.cap_lds + a.cap_lds / 2) * sizeof(uint32_t);
-        hipLaunchKernelGGL(k_count_dir, grid, dim3(block), lds, s, a);
+        hipLaunchKernelGGL(k_count_dir<false>, grid, dim3(block), lds, s, a, TaskList{});
         return hipGetLastError();
     }
     const size_t lds = (size_t)nw * (directed ? 2 : 1) * a.cap_lds * sizeof(uint32_t);
@@ -1136,6 +1160,13 @@ hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_
         if (tiled) hipLaunchKernelGGL((k_count<false, true>), grid, dim3(block), lds, s, a);
         else hipLaunchKernelGGL((k_count<false, false>), grid, dim3(block), lds, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t count_supports_batch(const CountArgs& a, const TaskList& tl, uint32_t block,
+                                uint32_t blocks, hipStream_t s) {
+    const size_t lds = (size_t)(block / kWave) * (a.cap_lds + a.cap_lds / 2) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_count_dir<true>, dim3(blocks), dim3(block), lds, s, a, tl);
     return hipGetLastError();
 }
 
@@ -1181,7 +1212,8 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s) {
     if (!E) return hipSuccess;
-    hipLaunchKernelGGL(k_min_live, dim3(grid_for(E)), dim3(kBlock), 0, s, dst, sup, E, ctr);
+    const uint32_t blocks = grid_for(E) < 2048 ? grid_for(E) : 2048;
+    hipLaunchKernelGGL(k_min_live, dim3(blocks), dim3(kBlock), 0, s, dst, sup, E, ctr);
     return hipGetLastError();
 }
 

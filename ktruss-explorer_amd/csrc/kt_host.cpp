@@ -51,9 +51,13 @@ unsigned num_threads() {
     return t;
 }
 
+namespace {
+thread_local bool tl_worker = false;  // nested parallel_for calls run inline
+}
+
 void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_t)>& body) {
     if (n == 0) return;
-    size_t T = num_threads();
+    size_t T = tl_worker ? 1 : num_threads();
     if (grain == 0) grain = 1;
     T = std::min(T, (n + grain - 1) / grain);
     if (T <= 1) {
@@ -64,7 +68,10 @@ void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_
     th.reserve(T);
     for (size_t t = 0; t < T; ++t) {
         const size_t b = n * t / T, e = n * (t + 1) / T;
-        th.emplace_back([&body, b, e] { body(b, e); });
+        th.emplace_back([&body, b, e] {
+            tl_worker = true;
+            body(b, e);
+        });
     }
     for (auto& x : th) x.join();
 }

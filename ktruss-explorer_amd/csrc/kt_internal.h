@@ -22,6 +22,7 @@
 #include <vector>
 
 #include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
 
 #include "ktruss.h"
 
@@ -73,7 +74,15 @@ struct CountArgs {
     uint32_t query_factor;// query the shorter list when it is this many times shorter
     uint32_t prefilter;   // undirected COO+CSR stored-length pre-filter (huc:20)
     DevCounters* ctr;
-    const uint32_t* nitems_dev;  // when set, the item count is read here (graph replay)
+};
+
+// Device-resident task list of a batched decomposition: task t covers the
+// entries [desc[t].x, desc[t].y) of graph desc[t].z of the batch.  Both
+// fields are read on the device, so one captured launch serves whatever list
+// the previous step built.
+struct TaskList {
+    const uint4* desc;
+    const uint32_t* ntasks;
 };
 
 // Kernel launchers (kt_device.hip).  All asynchronous on `stream`.
@@ -83,6 +92,9 @@ hipError_t fill_u8(uint8_t* p, uint8_t value, uint64_t count, hipStream_t s);
 hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s);
 hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t algo,
                           uint32_t block, hipStream_t s);
+// Persistent directed COO+CSR count over a batch task list, `blocks` workgroups.
+hipError_t count_supports_batch(const CountArgs& a, const TaskList& tl, uint32_t block,
+                                uint32_t blocks, hipStream_t s);
 hipError_t zero_live_supports(const uint32_t* dst, uint32_t* sup, uint32_t E, hipStream_t s);
 hipError_t zero_listed_supports(const uint32_t* wl, uint32_t n, uint32_t* sup, hipStream_t s);
 hipError_t mark_deleted(uint32_t* dst, const uint32_t* src, const uint32_t* eid,

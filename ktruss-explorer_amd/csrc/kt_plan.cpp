@@ -74,7 +74,7 @@ int init_device(int device) {
     return KT_OK;
 }
 
-static int ensure_device() {
+int ensure_device() {
     if (g_device_checked) return hipSetDevice(g_device) == hipSuccess ? KT_OK
                                                                       : fail(KT_ERR_HIP, "hipSetDevice");
     return init_device(0);
@@ -289,7 +289,6 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
     a.query_factor = query_factor;
     a.prefilter = (!directed && !tiled) ? 1u : 0u;  // kernel_duc.cu:27; kernel_dut.cu has none
     a.ctr = counters;
-    a.nitems_dev = nullptr;
     return a;
 }
 

@@ -25,6 +25,8 @@ struct DBuf {
 };
 
 int init_device(int device);
+// Makes the engine's device current (initialising it on first use).
+int ensure_device();
 
 // Count configuration resolved for a plan's layout.
 struct CountCfg {

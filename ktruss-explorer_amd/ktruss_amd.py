@@ -43,6 +43,9 @@ EXPORTS = [
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
     "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
     "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
+    "kt_batch_create", "kt_batch_destroy", "kt_batch_info", "kt_batch_set_tuning",
+    "kt_batch_get_tuning", "kt_batch_autotune", "kt_batch_tune_log", "kt_batch_decompose",
+    "kt_batch_truss_numbers", "kt_batch_graph_info", "kt_batch_set_timing",
 ]
 
 MLP_IN, MLP_HIDDEN, MLP_MAX_OUT, MAX_CANDIDATES, PHASE_REC = 16, 64, 16, 16, 36
@@ -98,9 +101,33 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("chunk", ctypes.c_uint32),
+                ("cap_lds", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("mark_chunk", ctypes.c_uint32), ("steps_per_replay", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("graphs", ctypes.c_uint64), ("entries", ctypes.c_uint64),
+                ("steps", ctypes.c_uint64), ("replays", ctypes.c_uint64),
+                ("count_passes", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
+                ("phases", ctypes.c_uint64), ("recount_edges", ctypes.c_uint64),
+                ("merge_elems", ctypes.c_uint64), ("live_in_passes", ctypes.c_uint64),
+                ("k_max", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("ms_total", ctypes.c_double), ("balg_count_bytes", ctypes.c_double),
+                ("ms_count", ctypes.c_double), ("count_launches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 _lib = None
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
 
 
 def lib():
@@ -145,7 +172,6 @@ def lib():
         L.kt_plan_decompose.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.kt_plan_truss_numbers.argtypes = [ctypes.c_void_p, _u32p, ctypes.c_uint64]
         L.kt_plan_trace.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, _u64p]
-        _f32p = ctypes.POINTER(ctypes.c_float)
         L.kt_plan_set_candidates.argtypes = [ctypes.c_void_p, ctypes.POINTER(CountConfig), ctypes.c_uint32]
         L.kt_plan_set_pick.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
         L.kt_plan_phase_log.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64, _u64p]
@@ -161,6 +187,20 @@ def lib():
         L.kt_mlp_train.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64,
                                    _f32p]
+        _vp = ctypes.c_void_p
+        L.kt_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(Variant),
+                                      ctypes.POINTER(_vp)]
+        L.kt_batch_destroy.argtypes = [_vp]
+        L.kt_batch_destroy.restype = None
+        L.kt_batch_info.argtypes = [_vp, _u32p, _u64p, _u32p]
+        L.kt_batch_set_tuning.argtypes = [_vp, ctypes.POINTER(BatchTuning)]
+        L.kt_batch_get_tuning.argtypes = [_vp, ctypes.POINTER(BatchTuning)]
+        L.kt_batch_autotune.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(BatchTuning)]
+        L.kt_batch_tune_log.argtypes = [_vp, ctypes.POINTER(BatchTuning), _f32p, ctypes.c_uint64, _u64p]
+        L.kt_batch_decompose.argtypes = [_vp, ctypes.POINTER(BatchStats)]
+        L.kt_batch_set_timing.argtypes = [_vp, ctypes.c_int]
+        L.kt_batch_truss_numbers.argtypes = [_vp, ctypes.c_uint32, _u32p, ctypes.c_uint64]
+        L.kt_batch_graph_info.argtypes = [_vp, ctypes.c_uint32, _u32p]
         _lib = L
     return _lib
 
@@ -340,6 +380,83 @@ class Plan:
         _check(lib().kt_plan_phase_log(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                        n.value, ctypes.byref(n)), "kt_plan_phase_log")
         return buf[: n.value].copy()
+
+
+class Batch:
+    """Many graphs decomposed together on the device (directed COO+CSR,
+    recount all): per-graph peel state on the device, peel steps replayed
+    from a captured HIP graph."""
+
+    def __init__(self, graphs, variant=None):
+        variant = variant or default_variant()
+        self.graphs = list(graphs)
+        arr = (ctypes.c_void_p * max(len(self.graphs), 1))(*[g._h.value for g in self.graphs])
+        h = ctypes.c_void_p()
+        _check(lib().kt_batch_create(arr, len(self.graphs), ctypes.byref(variant), ctypes.byref(h)),
+               "kt_batch_create")
+        self._h = h
+        self.variant = variant
+        g = ctypes.c_uint32()
+        e = ctypes.c_uint64()
+        mr = ctypes.c_uint32()
+        _check(lib().kt_batch_info(h, ctypes.byref(g), ctypes.byref(e), ctypes.byref(mr)),
+               "kt_batch_info")
+        self.entries = e.value
+        self.max_row = mr.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.kt_batch_destroy(h)
+            self._h = None
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(lib().kt_batch_get_tuning(self._h, ctypes.byref(t)), "kt_batch_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = BatchTuning(**kw)
+        _check(lib().kt_batch_set_tuning(self._h, ctypes.byref(t)), "kt_batch_set_tuning")
+        return self.tuning()
+
+    def autotune(self, trial_steps=8, repeats=2):
+        t = BatchTuning()
+        _check(lib().kt_batch_autotune(self._h, trial_steps, repeats, ctypes.byref(t)),
+               "kt_batch_autotune")
+        return t.as_dict()
+
+    def tune_log(self):
+        n = ctypes.c_uint64()
+        _check(lib().kt_batch_tune_log(self._h, None, None, 0, ctypes.byref(n)), "kt_batch_tune_log")
+        cfg = (BatchTuning * max(n.value, 1))()
+        ms = np.zeros(max(n.value, 1), np.float32)
+        _check(lib().kt_batch_tune_log(self._h, cfg, ms.ctypes.data_as(_f32p), n.value,
+                                       ctypes.byref(n)), "kt_batch_tune_log")
+        return [(cfg[i].as_dict(), float(ms[i])) for i in range(n.value)]
+
+    def set_timing(self, on=True):
+        """Eager steps with every count launch timed (stats["ms_count"])."""
+        _check(lib().kt_batch_set_timing(self._h, 1 if on else 0), "kt_batch_set_timing")
+
+    def decompose(self):
+        st = BatchStats()
+        _check(lib().kt_batch_decompose(self._h, ctypes.byref(st)), "kt_batch_decompose")
+        return st.as_dict()
+
+    def truss_numbers(self, i):
+        m = self.graphs[i].m_input
+        t = np.zeros(max(m, 1), np.uint32)
+        _check(lib().kt_batch_truss_numbers(self._h, i, t.ctypes.data_as(_u32p), m),
+               "kt_batch_truss_numbers")
+        return t[:m].copy()
+
+    def graph_info(self, i):
+        """(largest truss number, iterations, phases, count passes) of graph i."""
+        info = np.zeros(4, np.uint32)
+        _check(lib().kt_batch_graph_info(self._h, i, info.ctypes.data_as(_u32p)),
+               "kt_batch_graph_info")
+        return tuple(int(x) for x in info)
 
 
 def count_config(kernel="row", block_size=512, cap_lds=0, chunk=0):
