@@ -58,8 +58,17 @@ enum { KT_SCHED_JUMP = 1u, KT_SCHED_REUSE = 2u };
  * SEARCH: 64 entries at a time, all touched row tails staged, the candidate
  *         windows of v's rows concatenated and binary-searched lane by lane.
  * MERGE : as SEARCH, but each lane merges a contiguous share of the windows
- *         against the staged rows. */
-enum { KT_COUNT_AUTO = 0u, KT_COUNT_ROW = 1u, KT_COUNT_SEARCH = 2u, KT_COUNT_MERGE = 3u };
+ *         against the staged rows.
+ * DELTA : support maintenance instead of a recount (directed COO+CSR,
+ *         recount all): after a mark pass, every triangle destroyed by the
+ *         deleted set is found by intersecting the full (in + out) adjacency
+ *         lists of each deleted edge's endpoints, once per triangle, and the
+ *         supports of its surviving edges are decremented.  The supports are
+ *         then those a full recount would give, so the next mark deletes the
+ *         same edges.  A pass with nothing to update (the first count) uses
+ *         ROW.  Plans that use DELTA keep entries in place (no compaction). */
+enum { KT_COUNT_AUTO = 0u, KT_COUNT_ROW = 1u, KT_COUNT_SEARCH = 2u, KT_COUNT_MERGE = 3u,
+       KT_COUNT_DELTA = 4u };
 
 /* Input flags for kt_graph_create. */
 enum {
@@ -108,6 +117,10 @@ typedef struct kt_stats {
     double ms_count;          /* summed device time of count-kernel launches      */
     uint64_t count_launches;  /* count-kernel launches timed into ms_count        */
     double balg_count_bytes;  /* B_alg of the count passes only                   */
+    uint64_t delta_passes;    /* support updates done by DELTA (counted in count_passes) */
+    uint64_t delta_edges;     /* deleted edges processed by DELTA                  */
+    uint64_t delta_elems;     /* sum over them of the shorter adjacency list       */
+    uint64_t delta_triangles; /* destroyed triangles found                         */
 } kt_stats;
 
 typedef struct kt_graph kt_graph; /* validated host-side edge set            */

@@ -325,6 +325,48 @@ int build_layout(const kt_graph* g, int directedness, HostLayout& L) {
     return KT_OK;
 }
 
+// Full adjacency of an oriented layout (every stored entry e = (u, v) has
+// u < v): vertex x's list is its in-neighbours (ascending, since entries are
+// stored in source order) followed by its out-neighbours (ascending), which is
+// one sorted list.  seid maps a list slot to its entry, sa / sb map an entry
+// to its slot in u's / v's list.
+int build_adjacency(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t* src,
+                    const uint32_t* dst, std::vector<uint32_t>& sptr, std::vector<uint32_t>& snbr,
+                    std::vector<uint32_t>& seid, std::vector<uint32_t>& sa,
+                    std::vector<uint32_t>& sb) {
+    std::vector<uint32_t> indeg(n, 0);
+    for (uint32_t e = 0; e < E; ++e) {
+        if (src[e] >= dst[e]) return fail(KT_ERR_STATE, "adjacency lists need an oriented layout");
+        indeg[dst[e]]++;
+    }
+    sptr.assign((size_t)n + 1, 0);
+    for (uint32_t x = 0; x < n; ++x) sptr[x + 1] = sptr[x] + indeg[x] + (ptr[x + 1] - ptr[x]);
+    snbr.resize(2 * (size_t)E);
+    seid.resize(2 * (size_t)E);
+    sa.resize(E);
+    sb.resize(E);
+    std::vector<uint32_t> fill(sptr.begin(), sptr.end() - 1);
+    for (uint32_t e = 0; e < E; ++e) {  // in-neighbours, in source order
+        const uint32_t x = dst[e];
+        const uint32_t p = fill[x]++;
+        snbr[p] = src[e];
+        seid[p] = e;
+        sb[e] = p;
+    }
+    parallel_for(n, 4096, [&](size_t x0, size_t x1) {
+        for (size_t x = x0; x < x1; ++x) {
+            const uint32_t base = sptr[x] + indeg[x];
+            for (uint32_t e = ptr[x]; e < ptr[x + 1]; ++e) {
+                const uint32_t p = base + (e - ptr[x]);
+                snbr[p] = dst[e];
+                seid[p] = e;
+                sa[e] = p;
+            }
+        }
+    });
+    return KT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // TSV I/O: createCOOFromFile (graph.cu:37-79), writeCOOGraphToFile
 // (graph.cu:96-104).

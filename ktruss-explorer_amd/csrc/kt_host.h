@@ -30,6 +30,10 @@ struct HostLayout {
 int graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, uint32_t flags,
                  kt_graph** out);
 int build_layout(const kt_graph* g, int directedness, HostLayout& L);
+int build_adjacency(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t* src,
+                    const uint32_t* dst, std::vector<uint32_t>& sptr, std::vector<uint32_t>& snbr,
+                    std::vector<uint32_t>& seid, std::vector<uint32_t>& sa,
+                    std::vector<uint32_t>& sb);
 
 int read_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst);
 int write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst);

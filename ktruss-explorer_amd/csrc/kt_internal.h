@@ -28,6 +28,11 @@
 
 #define KT_DEAD 0x80000000u
 #define KT_VMASK 0x7FFFFFFFu
+// DELTA plans flag an entry deleted by the current mark pass with KT_DYING
+// until its triangles are removed from the surviving supports; vertex ids
+// then stay below 2^30.
+#define KT_DYING 0x40000000u
+#define KT_IDMASK 0x3FFFFFFFu
 #define KT_NOT_AFFECTED 0
 #define KT_DIRECTLY_AFFECTED 1
 #define KT_INDIRECTLY_AFFECTED 2
@@ -56,6 +61,8 @@ struct DevCounters {
     unsigned int live;                // entries kept by the last compaction
     unsigned int choice;              // candidate picked by the cost model
     unsigned int pad[3];
+    unsigned long long delta_elems;   // adjacency entries read by DELTA updates
+    unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
 };
 
 // Arguments of the support-count kernels.
@@ -110,6 +117,14 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
                           bool directed, DevCounters* ctr, hipStream_t s);
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s);
+// DELTA support maintenance (kt_delta.hip).
+hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
+                      uint32_t* dl, uint32_t E, uint32_t k, DevCounters* ctr, hipStream_t s);
+hipError_t delta_update(const uint32_t* dl, uint32_t nd, const uint32_t* src, const uint32_t* dst,
+                        const uint32_t* sptr, const uint32_t* snbr, const uint32_t* seid,
+                        uint32_t* sup, DevCounters* ctr, hipStream_t s);
+hipError_t finalize_deleted(const uint32_t* dl, uint32_t nd, uint32_t* dst, uint32_t* snbr,
+                            const uint32_t* sa, const uint32_t* sb, hipStream_t s);
 // Stable compaction of live entries into the *_out arrays and row-pointer
 // rebuild: tile_sums needs compact_tiles(E) + 1 entries, xs needs E + 1.
 uint32_t compact_tiles(uint32_t E);

@@ -167,6 +167,11 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         KT_HIP_TRY(hipMemcpyAsync(dst0.p, L.dst.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
     }
     KT_HIP_TRY(hipStreamSynchronize(stream));
+    if (directed && !tiled) {  // kept for the adjacency lists DELTA builds on first use
+        h_ptr = std::move(L.ptr);
+        h_src = std::move(L.src);
+        h_dst = std::move(L.dst);
+    }
     graph = graph_retain(g);
     return KT_OK;
 }
@@ -195,6 +200,9 @@ int Plan::reset() {
     KT_HIP_TRY(dev::iota_u32(eid[0].as<uint32_t>(), E0, stream));
     KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E0, stream));
     KT_HIP_TRY(dev::fill_u32(ctr.as<uint32_t>(), 0, sizeof(DevCounters) / 4, stream));
+    if (delta_ready)
+        KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, snbr.size, hipMemcpyDeviceToDevice, stream));
+    pending = 0;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     std::memset(h_ctr, 0, sizeof(DevCounters));
     trace.clear();
@@ -224,7 +232,9 @@ int Plan::compact_now() {
 }
 
 int Plan::resolve(const kt_count_config& in, CountCfg& out) const {
-    if (in.count_kernel > KT_COUNT_MERGE) return fail(KT_ERR_ARG, "unknown count_kernel");
+    if (in.count_kernel > KT_COUNT_DELTA) return fail(KT_ERR_ARG, "unknown count_kernel");
+    if (in.count_kernel == KT_COUNT_DELTA && (!directed || tiled || affected))
+        return fail(KT_ERR_ARG, "DELTA needs a directed COO+CSR variant with recount all");
     out.kernel = in.count_kernel == KT_COUNT_AUTO ? (uint32_t)KT_COUNT_ROW : in.count_kernel;
     if (!directed || tiled) out.kernel = (uint32_t)KT_COUNT_ROW;  // the other strategies exist for directed COO+CSR
     out.block = in.block_size ? in.block_size : 512;
@@ -292,6 +302,27 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
     return a;
 }
 
+// Phase features of the surviving graph (supports used only when `exact`)
+// and, in MODEL mode, the cost model's pick among the candidates.
+int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen) {
+    KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                   sup[cur].as<uint32_t>(), n, E, E0, k, exact,
+                                   feat_acc.as<unsigned long long>(), feat.as<float>(), stream));
+    cfg = fixed_cfg;
+    chosen = 0;
+    if (pick == KT_PICK_MODEL)
+        KT_HIP_TRY(dev::mlp_pick(model.as<float>(), model_C, feat.as<float>(),
+                                 &ctr.as<DevCounters>()->choice, stream));
+    KT_HIP_TRY(hipMemcpyAsync(h_feat, feat.p, KT_MLP_IN * sizeof(float), hipMemcpyDeviceToHost, stream));
+    if (pick == KT_PICK_MODEL) {
+        const int rc = sync_counters();
+        if (rc) return rc;
+        chosen = std::min<uint32_t>(h_ctr->choice, (uint32_t)cands.size() - 1);
+        cfg = cands[chosen];
+    }
+    return KT_OK;
+}
+
 // One phase at k: the peel loop of kernel_ddc.cu:116-214 (and its tiled /
 // undirected twins) on the current graph.
 int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out) {
@@ -308,20 +339,9 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     const uint32_t nc = (uint32_t)cands.size();
     std::array<float, KT_PHASE_REC> rec;
     rec.fill(-1.f);
-    KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                   sup[cur].as<uint32_t>(), n, E, E0, k, reuse,
-                                   feat_acc.as<unsigned long long>(), feat.as<float>(), stream));
     CountCfg cfg = fixed_cfg;
-    if (pick == KT_PICK_MODEL)
-        KT_HIP_TRY(dev::mlp_pick(model.as<float>(), model_C, feat.as<float>(),
-                                 &ctr.as<DevCounters>()->choice, stream));
-    KT_HIP_TRY(hipMemcpyAsync(h_feat, feat.p, KT_MLP_IN * sizeof(float), hipMemcpyDeviceToHost, stream));
     uint32_t chosen = 0;
-    if (pick == KT_PICK_MODEL) {
-        if ((rc = sync_counters())) return rc;
-        chosen = std::min<uint32_t>(h_ctr->choice, nc - 1);
-        cfg = cands[chosen];
-    }
+    if ((rc = phase_pick(k, reuse, cfg, chosen))) return rc;
     std::vector<double> cand_ms(nc, 0.0);
     double phase_ms = 0.0;
     do {
@@ -442,14 +462,27 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
 int Plan::decompose() {
     int rc = reset();
     if (rc) return rc;
+    const bool use_delta = delta_capable();
+    if (use_delta && (rc = ensure_delta())) return rc;
     KT_HIP_TRY(hipEventRecord(ev_start, stream));
     uint32_t k = 3;
     bool reuse = false;
     while (live > 0) {
-        bool exact = false;
-        if ((rc = run_phase(k, reuse, v.schedule, exact))) return rc;
-        reuse = directed && !affected && (v.schedule & KT_SCHED_REUSE) && exact;
+        if (use_delta) {
+            if ((rc = run_phase_delta(k, reuse, v.schedule))) return rc;
+            reuse = (v.schedule & KT_SCHED_REUSE) != 0;
+        } else {
+            bool exact = false;
+            if ((rc = run_phase(k, reuse, v.schedule, exact))) return rc;
+            reuse = directed && !affected && (v.schedule & KT_SCHED_REUSE) && exact;
+        }
         ++k;
+    }
+    if (use_delta && pending) {  // the last pass removed the last edges at k = 3
+        KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), pending, dst[cur].as<uint32_t>(),
+                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                         stream));
+        pending = 0;
     }
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
@@ -465,6 +498,8 @@ int Plan::decompose() {
 void Plan::finish_stats() {
     stats.merge_elems = h_ctr->merge_elems;
     stats.recount_edges = h_ctr->recount_edges;
+    stats.delta_elems = h_ctr->delta_elems;
+    stats.delta_triangles = h_ctr->delta_tris;
     stats.balg_count_bytes = 4.0 * (double)stats.merge_elems + 24.0 * (double)stats.recount_edges +
                              12.0 * (double)stats.live_in_passes;
 }
@@ -476,7 +511,12 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
     KT_HIP_TRY(hipEventRecord(ev_start, stream));
     bool exact = false;
     uint32_t kk = k;
-    if (live > 0 && (rc = run_phase(kk, false, 0, exact))) return rc;
+    const bool use_delta = delta_capable();
+    if (use_delta && (rc = ensure_delta())) return rc;
+    if (live > 0) {
+        rc = use_delta ? run_phase_delta(kk, false, 0) : run_phase(kk, false, 0, exact);
+        if (rc) return rc;
+    }
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
@@ -491,18 +531,246 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
     }
     // coocsr2coo + directed2undirectedCOO + unsort (kernel_ddc.cu:246-250,
     // kernel_d.cu:57-61, kernel.cu:31-37)
-    const size_t out = directed ? 2 * (size_t)E : (size_t)E;
+    // DELTA plans keep tombstones in place (and KT_DYING entries of a last
+    // k == 3 pass); the others compacted them away
+    std::vector<uint32_t> keep;
+    keep.reserve(E);
+    for (uint32_t e = 0; e < E; ++e) {
+        if (d[e] & (KT_DEAD | KT_DYING)) {
+            if (!use_delta) return fail(KT_ERR_STATE, "tombstone left after final compaction");
+            continue;
+        }
+        keep.push_back(e);
+    }
+    const size_t L = keep.size();
+    const size_t out = directed ? 2 * L : L;
     out_src.resize(out);
     out_dst.resize(out);
-    for (uint32_t e = 0; e < E; ++e) {
-        if (d[e] & KT_DEAD) return fail(KT_ERR_STATE, "tombstone left after final compaction");
-        out_src[e] = new2old[s[e]];
-        out_dst[e] = new2old[d[e]];
+    for (size_t i = 0; i < L; ++i) {
+        const uint32_t e = keep[i];
+        out_src[i] = new2old[s[e]];
+        out_dst[i] = new2old[d[e]];
         if (directed) {
-            out_src[E + e] = new2old[d[e]];
-            out_dst[E + e] = new2old[s[e]];
+            out_src[L + i] = new2old[d[e]];
+            out_dst[L + i] = new2old[s[e]];
         }
     }
+    return KT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// DELTA support maintenance
+
+bool Plan::delta_capable() const {
+    if (!directed || tiled || affected) return false;
+    if (fixed_cfg.kernel == KT_COUNT_DELTA) return true;
+    for (const CountCfg& c : cands)
+        if (c.kernel == KT_COUNT_DELTA) return true;
+    return false;
+}
+
+int Plan::ensure_delta() {
+    if (delta_ready) return KT_OK;
+    if (n >= KT_DYING) return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^30 vertices");
+    if (h_ptr.size() != (size_t)n + 1 || h_src.size() != E0)
+        return fail(KT_ERR_STATE, "oriented host layout not kept for this plan");
+    std::vector<uint32_t> hs_ptr, hs_nbr, hs_eid, hs_a, hs_b;
+    int rc = build_adjacency(n, E0, h_ptr.data(), h_src.data(), h_dst.data(), hs_ptr, hs_nbr,
+                             hs_eid, hs_a, hs_b);
+    if (rc) return rc;
+    const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4;
+    if ((rc = sptr.alloc(((size_t)n + 1) * 4)) || (rc = snbr0.alloc(2 * Eb)) ||
+        (rc = snbr.alloc(2 * Eb)) || (rc = seid.alloc(2 * Eb)) || (rc = sa.alloc(Eb)) ||
+        (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)))
+        return rc;
+    KT_HIP_TRY(hipMemcpyAsync(sptr.p, hs_ptr.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream));
+    if (E0) {
+        KT_HIP_TRY(hipMemcpyAsync(snbr0.p, hs_nbr.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, 2 * (size_t)E0 * 4, hipMemcpyDeviceToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(seid.p, hs_eid.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(sa.p, hs_a.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(sb.p, hs_b.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+    }
+    KT_HIP_TRY(hipStreamSynchronize(stream));
+    delta_ready = true;
+    return KT_OK;
+}
+
+// Makes the supports exact for the live entries, as configuration `cfg`
+// says: the DELTA update of the pending deletions, or a full recount (ROW
+// when cfg is DELTA but nothing is pending).  The timed kernel is bracketed
+// by ev_c0 / ev_c1.
+int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
+    (void)ms;
+    uint32_t* dl_p = dl.as<uint32_t>();
+    if (pending && cfg.kernel == KT_COUNT_DELTA) {
+        KT_HIP_TRY(hipEventRecord(ev_c0, stream));
+        KT_HIP_TRY(dev::delta_update(dl_p, pending, src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                     sptr.as<uint32_t>(), snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                     sup[cur].as<uint32_t>(), ctr.as<DevCounters>(), stream));
+        KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+        KT_HIP_TRY(dev::finalize_deleted(dl_p, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
+                                         sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+        stats.delta_passes++;
+        stats.delta_edges += pending;
+    } else {
+        if (pending)
+            KT_HIP_TRY(dev::finalize_deleted(dl_p, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
+                                             sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+        CountCfg rcfg = cfg;
+        if (rcfg.kernel == KT_COUNT_DELTA) rcfg.kernel = KT_COUNT_ROW;
+        KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
+        CountArgs a = count_args(k, false, E, rcfg, ctr.as<DevCounters>());
+        KT_HIP_TRY(hipEventRecord(ev_c0, stream));
+        KT_HIP_TRY(dev::count_supports(a, directed, tiled, rcfg.kernel, rcfg.block, stream));
+        KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+        stats.live_in_passes += live;
+    }
+    pending = 0;
+    return KT_OK;
+}
+
+// SWEEP in a DELTA plan: DELTA candidates first (each from the supports of
+// the previous pass, restored from a snapshot when there are several), then
+// the recount candidates from zeroed supports.  Candidate c is bracketed by
+// ev_sw[2c], ev_sw[2c+1].
+int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
+    (void)cand_ms;
+    (void)best_ms;
+    const uint32_t nc = (uint32_t)cands.size();
+    uint32_t* dl_p = dl.as<uint32_t>();
+    uint32_t ndc = 0;
+    for (const CountCfg& c : cands) ndc += c.kernel == KT_COUNT_DELTA;
+    const bool had_pending = pending != 0;
+    if (had_pending && ndc) {
+        if (ndc > 1) {
+            if (!sup_snap.p) {
+                int rc = sup_snap.alloc((size_t)std::max<uint32_t>(E0, 1) * 4);
+                if (rc) return rc;
+            }
+            KT_HIP_TRY(hipMemcpyAsync(sup_snap.p, sup[cur].p, (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
+        }
+        bool first = true;
+        for (uint32_t c = 0; c < nc; ++c) {
+            if (cands[c].kernel != KT_COUNT_DELTA) continue;
+            if (!first)
+                KT_HIP_TRY(hipMemcpyAsync(sup[cur].p, sup_snap.p, (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
+            first = false;
+            KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
+            KT_HIP_TRY(dev::delta_update(dl_p, pending, src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                         sptr.as<uint32_t>(), snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                         sup[cur].as<uint32_t>(), ctr_scratch.as<DevCounters>(), stream));
+            KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
+        }
+    }
+    if (had_pending)
+        KT_HIP_TRY(dev::finalize_deleted(dl_p, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
+                                         sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+    uint32_t last_recount = nc;
+    for (uint32_t c = 0; c < nc; ++c)
+        if (!(had_pending && cands[c].kernel == KT_COUNT_DELTA)) last_recount = c;
+    for (uint32_t c = 0; c < nc; ++c) {
+        if (had_pending && cands[c].kernel == KT_COUNT_DELTA) continue;
+        CountCfg rcfg = cands[c];
+        if (rcfg.kernel == KT_COUNT_DELTA) rcfg.kernel = KT_COUNT_ROW;
+        KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
+        CountArgs a = count_args(k, false, E, rcfg,
+                                 c == last_recount ? ctr.as<DevCounters>() : ctr_scratch.as<DevCounters>());
+        KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
+        KT_HIP_TRY(dev::count_supports(a, directed, tiled, rcfg.kernel, rcfg.block, stream));
+        KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
+    }
+    pending = 0;
+    return KT_OK;
+}
+
+// One phase at k with DELTA available: the mark pass defers its deletions
+// (KT_DYING + list), and the support pass that opens the next iteration
+// either removes their triangles (DELTA) or recounts.  Entries never move.
+int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
+    const bool jump = (schedule & KT_SCHED_JUMP) != 0;
+    uint32_t iter = 0;
+    bool broke = false, changed = false;
+    int rc;
+    stats.phases++;
+    const uint32_t nc = (uint32_t)cands.size();
+    std::array<float, KT_PHASE_REC> rec;
+    rec.fill(-1.f);
+    CountCfg cfg = fixed_cfg;
+    uint32_t chosen = 0;
+    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen))) return rc;
+    std::vector<double> cand_ms(nc, 0.0);
+    double phase_ms = 0.0;
+    do {
+        if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
+        bool counted = false;
+        const uint64_t rec_before = h_ctr->recount_edges;
+        float dummy = 0;
+        if (pending || !(iter == 0 && reuse)) {
+            if (pick == KT_PICK_SWEEP) rc = sweep_pass(k, cand_ms, dummy);
+            else rc = support_pass(cfg, k, dummy);
+            if (rc) return rc;
+            counted = true;
+            stats.count_passes++;
+        }
+        if (iter == 0 && jump) {
+            if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
+            KT_HIP_TRY(dev::min_live_support(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E,
+                                             ctr.as<DevCounters>(), stream));
+            if ((rc = sync_counters())) return rc;
+            const uint32_t s = h_ctr->min_sup;
+            if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
+        }
+        KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
+                                   sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
+                                   ctr.as<DevCounters>(), stream));
+        if ((rc = sync_counters())) return rc;
+        float count_ms = 0;
+        if (counted && pick == KT_PICK_SWEEP) {
+            float best = 0;
+            for (uint32_t c = 0; c < nc; ++c) {
+                float ms = 0;
+                KT_HIP_TRY(hipEventElapsedTime(&ms, ev_sw[2 * c], ev_sw[2 * c + 1]));
+                cand_ms[c] += ms;
+                if (c == 0 || ms < best) best = ms;
+            }
+            count_ms = best;
+        } else if (counted) {
+            KT_HIP_TRY(hipEventElapsedTime(&count_ms, ev_c0, ev_c1));
+        }
+        if (counted) {
+            stats.ms_count += count_ms;
+            stats.count_launches++;
+        }
+        phase_ms += count_ms;
+        const uint32_t ndel = h_ctr->ndel;
+        pending = ndel;
+        live -= ndel;
+        stats.iterations++;
+        trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)ndel, h_ctr->recount_edges - rec_before,
+                         (uint64_t)(count_ms * 1e6)});
+        if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
+        changed = ndel > 0;
+        if (changed && k == 3) {  // kernel_ddc.cu:183-186
+            broke = true;
+            break;
+        }
+        ++iter;
+    } while (changed);
+    if (pick == KT_PICK_SWEEP && nc) {
+        chosen = 0;
+        for (uint32_t c = 1; c < nc; ++c)
+            if (cand_ms[c] < cand_ms[chosen]) chosen = c;
+        for (uint32_t c = 0; c < nc; ++c) rec[20 + c] = (float)cand_ms[c];
+        phase_ms = cand_ms[chosen];
+    }
+    rec[0] = (float)k;
+    rec[1] = (float)iter + (broke ? 1.f : 0.f);
+    rec[2] = pick == KT_PICK_FIXED ? -1.f : (float)chosen;
+    rec[3] = (float)phase_ms;
+    for (int f = 0; f < KT_MLP_IN; ++f) rec[4 + f] = h_feat[f];
+    phase_log.push_back(rec);
+    stats.k_last = k;
     return KT_OK;
 }
 

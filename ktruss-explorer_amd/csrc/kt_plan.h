@@ -69,6 +69,15 @@ struct Plan {
     hipEvent_t ev_sw[2 * KT_MAX_CANDIDATES] = {};
     std::vector<std::array<float, KT_PHASE_REC>> phase_log;
 
+    // DELTA support maintenance: full adjacency lists (built on first use from
+    // the oriented host layout kept here) and the list of entries deleted by
+    // the last mark pass whose triangles are still counted in the supports
+    std::vector<uint32_t> h_ptr, h_src, h_dst;
+    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, sup_snap;
+    bool delta_ready = false;
+    uint32_t pending = 0;
+    bool delta_capable() const;
+
     int resolve(const kt_count_config& in, CountCfg& out) const;
     int set_candidates(const kt_count_config* c, uint32_t n);
     int set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C);
@@ -82,6 +91,11 @@ struct Plan {
 
    private:
     int run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out);
+    int run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule);
+    int ensure_delta();
+    int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
+    int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
+    int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen);
     int compact_now();
     int sync_counters();
     int set_counter(size_t offset, uint32_t value);

@@ -84,7 +84,7 @@ class CountConfig(ctypes.Structure):
                 ("cap_lds", ctypes.c_uint32), ("chunk", ctypes.c_uint32)]
 
     def describe(self):
-        names = {0: "auto", 1: "row", 2: "search", 3: "merge"}
+        names = {0: "auto", 1: "row", 2: "search", 3: "merge", 4: "delta"}
         return f"{names[self.count_kernel]}/b{self.block_size}/c{self.cap_lds or 'auto'}/k{self.chunk or 128}"
 
 
@@ -95,7 +95,9 @@ class Stats(ctypes.Structure):
                 ("iterations", ctypes.c_uint64), ("phases", ctypes.c_uint64),
                 ("k_max", ctypes.c_uint32), ("k_last", ctypes.c_uint32),
                 ("ms_total", ctypes.c_double), ("ms_count", ctypes.c_double),
-                ("count_launches", ctypes.c_uint64), ("balg_count_bytes", ctypes.c_double)]
+                ("count_launches", ctypes.c_uint64), ("balg_count_bytes", ctypes.c_double),
+                ("delta_passes", ctypes.c_uint64), ("delta_edges", ctypes.c_uint64),
+                ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -232,8 +234,8 @@ def device_synchronize():
     _check(lib().kt_device_synchronize(), "kt_device_synchronize")
 
 
-COUNT_AUTO, COUNT_ROW, COUNT_SEARCH, COUNT_MERGE = 0, 1, 2, 3
-COUNT_KERNELS = {"auto": 0, "row": 1, "search": 2, "merge": 3}
+COUNT_AUTO, COUNT_ROW, COUNT_SEARCH, COUNT_MERGE, COUNT_DELTA = 0, 1, 2, 3, 4
+COUNT_KERNELS = {"auto": 0, "row": 1, "search": 2, "merge": 3, "delta": 4}
 
 
 def make_variant(directedness="directed-by-degree", fmt="coocsr", recount="all", e_iters=0,
