@@ -4,10 +4,14 @@
 // its support sup[e] is the exact number of triangles through it).  After a
 // mark pass the deleted entries carry KT_DYING and sit in the list dl; the
 // supports are brought to those of the surviving graph by
-//   k_delta     one wave per deleted entry d = (u, v): the full adjacency
-//               list of the endpoint with the shorter list is read 64
-//               entries at a time and each neighbour w is binary-searched
-//               (uniform trip count) in the other endpoint's list.  A hit is a
+//   k_dprep, k_dscan_tiles, k_dscan_local
+//               per deleted entry d = (u, v): the shorter and the longer of
+//               the endpoints' full adjacency lists, and the tasks of 64
+//               neighbours of the shorter list, numbered by an exclusive scan;
+//   k_delta     waves take equal ranges of tasks; each neighbour w of a task
+//               is binary-searched (uniform trip count) in the longer list.
+//               Work is split by list length, so a deletion between two hubs
+//               is spread over many waves instead of serialising one.  A hit is a
 //               triangle {u, v, w} that was intact before the pass.  It is
 //               handled by the deleted edge of lowest entry index among its
 //               three edges, which decrements the supports of the triangle's
@@ -73,64 +77,229 @@ __global__ void k_mark_dying(uint32_t* __restrict__ dst, const uint32_t* __restr
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ dl, uint32_t nd,
+// Per pending deletion i (d = dl[i] = (u, v)): the shorter and the longer of
+// the two adjacency lists and the number of 64-candidate tasks over the
+// shorter one (at least 1: the list holds the edge itself); per tile of
+// kTileD deletions the task total.
+constexpr uint32_t kTileD = 2048;
+
+__global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ dl,
+                                                  const uint32_t* __restrict__ nd_dev,
                                                   const uint32_t* __restrict__ src,
                                                   const uint32_t* __restrict__ dst,
                                                   const uint32_t* __restrict__ sptr,
-                                                  const uint32_t* __restrict__ snbr,
-                                                  const uint32_t* __restrict__ seid,
-                                                  uint32_t* sup, DevCounters* ctr) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    unsigned long long acc_elems = 0, acc_tri = 0;
-    for (uint32_t t = gwave(); t < nd; t += nwaves()) {
-        const uint32_t d = dl[t];
-        const uint32_t u = src[d];
-        const uint32_t v = dst[d] & KT_IDMASK;
-        const uint32_t ub = sptr[u], ue = sptr[u + 1];
-        const uint32_t vb = sptr[v], ve = sptr[v + 1];
-        const bool ushort = ue - ub <= ve - vb;
-        const uint32_t qb = ushort ? ub : vb, qe = ushort ? ue : ve;
-        const uint32_t lb = ushort ? vb : ub, ll = ushort ? ve - vb : ue - ub;
-        const uint32_t skip = ushort ? v : u;  // the edge d itself
-        acc_elems += qe - qb;
-        if (ll == 0) continue;
-        for (uint32_t j0 = qb; j0 < qe; j0 += kWave) {
-            const uint32_t j = j0 + lane;
-            const uint32_t x = j < qe ? snbr[j] : KT_DEAD;
-            const bool cand = !(x & KT_DEAD) && x != skip;
-            // lower bound of x in the other list, the same trip count on every lane
-            uint32_t base = lb, len = ll;
-            while (len > 1) {
-                const uint32_t half = len >> 1;
-                if ((snbr[base + half] & KT_VMASK) < x) base += half;
-                len -= half;
-            }
-            const uint32_t p = base + ((snbr[base] & KT_VMASK) < x ? 1u : 0u);
-            const bool hit = cand && p < lb + ll && snbr[p] == x;
-            uint32_t tri = 0;
-            if (hit) {
-                const uint32_t e1 = seid[j], e2 = seid[p];
-                const bool y1 = (dst[e1] & KT_DYING) != 0;
-                const bool y2 = (dst[e2] & KT_DYING) != 0;
-                // the deleted edge of lowest index owns the triangle
-                if (!((y1 && e1 < d) || (y2 && e2 < d))) {
-                    if (!y1) atomicSub(sup + e1, 1u);
-                    if (!y2) atomicSub(sup + e2, 1u);
-                    tri = 1;
-                }
-            }
-            acc_tri += (uint32_t)__popcll(ballot(tri != 0));
+                                                  uint4* __restrict__ meta,
+                                                  uint32_t* __restrict__ ntask,
+                                                  uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t s_w[kBlock / kWave];
+    const uint32_t nd = *nd_dev;
+    const uint32_t ntiles = (nd + kTileD - 1) / kTileD;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t end = min(nd, (t + 1) * kTileD);
+        uint32_t c = 0;
+        for (uint32_t i = t * kTileD + threadIdx.x; i < end; i += kBlock) {
+            const uint32_t d = dl[i];
+            const uint32_t u = src[d], v = dst[d] & KT_IDMASK;
+            const uint32_t ub = sptr[u], ue = sptr[u + 1], vb = sptr[v], ve = sptr[v + 1];
+            const bool us = ue - ub <= ve - vb;
+            const uint4 m = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
+            meta[i] = m;
+            const uint32_t nt = (m.y - m.x + kWave - 1) / kWave;
+            ntask[i] = nt;
+            c += nt;
         }
-    }
-    if (lane == 0 && (acc_elems || acc_tri)) {
-        atomicAdd(&ctr->delta_elems, acc_elems);
-        atomicAdd(&ctr->delta_tris, acc_tri);
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t sum = 0;
+            for (uint32_t w = 0; w < kBlock / kWave; ++w) sum += s_w[w];
+            tsum[t] = sum;
+        }
+        __syncthreads();
     }
 }
 
-__global__ void k_finalize(const uint32_t* __restrict__ dl, uint32_t nd, uint32_t* __restrict__ dst,
-                           uint32_t* __restrict__ snbr, const uint32_t* __restrict__ sa,
-                           const uint32_t* __restrict__ sb) {
+// Exclusive scan of the tile totals in place (one workgroup); the grand
+// total goes to tsum[ntiles] and ctr->dtasks.
+__global__ __launch_bounds__(1024) void k_dscan_tiles(const uint32_t* __restrict__ nd_dev,
+                                                      uint32_t* tsum, DevCounters* ctr) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t nt = (*nd_dev + kTileD - 1) / kTileD;
+    const uint32_t per = (nt + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < b + per && i < nt; ++i) sum += tsum[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
+    for (uint32_t i = b; i < b + per && i < nt; ++i) {
+        const uint32_t c = tsum[i];
+        tsum[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) {
+        tsum[nt] = s_part[1023];
+        ctr->dtasks = s_part[1023];
+    }
+}
+
+// ntask[i] <- first task of deletion i (exclusive scan inside the tile plus
+// the tile's offset).  Thread th owns 8 consecutive deletions of the tile.
+__global__ __launch_bounds__(kBlock) void k_dscan_local(const uint32_t* __restrict__ nd_dev,
+                                                        uint32_t* __restrict__ ntask,
+                                                        const uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t s_w[kBlock / kWave];
+    constexpr uint32_t kPer = kTileD / kBlock;
+    const uint32_t nd = *nd_dev;
+    const uint32_t ntiles = (nd + kTileD - 1) / kTileD;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t i0 = t * kTileD + threadIdx.x * kPer;
+        uint32_t vals[kPer];
+        uint32_t sum = 0;
+        for (uint32_t r = 0; r < kPer; ++r) {
+            vals[r] = i0 + r < nd ? ntask[i0 + r] : 0u;
+            sum += vals[r];
+        }
+        uint32_t inc = sum;
+        for (int off = 1; off < (int)kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off);
+            if ((int)lane >= off) inc += y;
+        }
+        if (lane == kWave - 1) s_w[wid] = inc;
+        __syncthreads();
+        uint32_t run = tsum[t] + inc - sum;
+        for (uint32_t w = 0; w < wid; ++w) run += s_w[w];
+        for (uint32_t r = 0; r < kPer; ++r) {
+            if (i0 + r < nd) ntask[i0 + r] = run;
+            run += vals[r];
+        }
+        __syncthreads();
+    }
+}
+
+// The update proper.  Task t = (deletion i, chunk c) holds the candidates
+// short[64c, 64c + 64) of deletion i; wave w takes the task range
+// [T w / W, T (w + 1) / W), finds its first deletion by a binary search over
+// the task offsets and walks forward.  A task whose candidates are all dead
+// is skipped.  Otherwise the candidates' values span [xmin, xmax]; the window
+// of the longer list holding that range is found by two wave-uniform binary
+// searches.  A window of up to kWin entries is staged in the wave's LDS slice
+// (one coalesced read) and every lane binary-searches its candidate there;
+// a longer window is searched in place.
+constexpr uint32_t kWin = 512;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// First index in [lo, lo + len) whose value (dead bit masked) is >= x; the
+// trip count depends on len only.
+__device__ __forceinline__ uint32_t lower_bound_u(const uint32_t* a, uint32_t lo, uint32_t len,
+                                                  uint32_t x) {
+    if (len == 0) return lo;
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        if ((a[lo + half] & KT_VMASK) < x) lo += half;
+        len -= half;
+    }
+    return lo + ((a[lo] & KT_VMASK) < x ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ dl,
+                                                  const uint32_t* __restrict__ nd_dev,
+                                                  const uint4* __restrict__ meta,
+                                                  const uint32_t* __restrict__ toff,
+                                                  const uint32_t* __restrict__ dst,
+                                                  const uint32_t* __restrict__ snbr,
+                                                  const uint32_t* __restrict__ seid,
+                                                  uint32_t* sup, DevCounters* ctr) {
+    __shared__ uint32_t s_win[(kBlock / kWave) * kWin];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    uint32_t* win = s_win + (threadIdx.x / kWave) * kWin;
+    const uint32_t nd = *nd_dev;
+    const uint32_t T = ctr->dtasks;
+    const uint32_t W = nwaves(), w = gwave();
+    const uint32_t t0 = (uint32_t)((uint64_t)T * w / W), t1 = (uint32_t)((uint64_t)T * (w + 1) / W);
+    if (t0 >= t1 || nd == 0) return;
+    // last deletion whose first task is <= t0
+    uint32_t i = 0, len = nd;
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        if (toff[i + half] <= t0) i += half;
+        len -= half;
+    }
+    unsigned long long acc_elems = 0, acc_tri = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        while (i + 1 < nd && toff[i + 1] <= t) ++i;
+        const uint4 m = meta[i];
+        const uint32_t j0 = m.x + (t - toff[i]) * kWave;
+        const uint32_t j = j0 + lane;
+        acc_elems += min(kWave, m.y - j0);
+        const uint32_t x = j < m.y ? snbr[j] : KT_DEAD;
+        const bool cand = !(x & KT_DEAD);
+        const uint64_t cm = ballot(cand);
+        if (!cm) continue;
+        // candidates ascend with the lane: the window of the longer list
+        const uint32_t xmin = __builtin_amdgcn_readlane(x, (uint32_t)__ffsll((long long)cm) - 1);
+        const uint32_t xmax = __builtin_amdgcn_readlane(x, 63u - (uint32_t)__builtin_clzll(cm));
+        const uint32_t lo = lower_bound_u(snbr, m.z, m.w, xmin);
+        const uint32_t hi = lower_bound_u(snbr, lo, m.z + m.w - lo, xmax + 1);
+        const uint32_t wl = hi - lo;
+        if (wl == 0) continue;
+        uint32_t p;
+        bool hit;
+        if (wl <= kWin) {
+            wave_lds_sync();
+            for (uint32_t q = lane; q < wl; q += kWave) win[q] = snbr[lo + q];
+            wave_lds_sync();
+            uint32_t b = 0, n = wl;
+            while (n > 1) {
+                const uint32_t half = n >> 1;
+                if ((win[b + half] & KT_VMASK) < x) b += half;
+                n -= half;
+            }
+            const uint32_t q = b + ((win[b] & KT_VMASK) < x ? 1u : 0u);
+            hit = cand && q < wl && win[q] == x;
+            p = lo + q;
+        } else {
+            p = lower_bound_u(snbr, lo, wl, x);
+            hit = cand && p < hi && snbr[p] == x;
+        }
+        bool tri = false;
+        if (hit) {
+            const uint32_t d = dl[i];
+            const uint32_t e1 = seid[j], e2 = seid[p];
+            const bool y1 = (dst[e1] & KT_DYING) != 0;
+            const bool y2 = (dst[e2] & KT_DYING) != 0;
+            // the deleted edge of lowest index owns the triangle
+            if (!((y1 && e1 < d) || (y2 && e2 < d))) {
+                if (!y1) atomicSub(sup + e1, 1u);
+                if (!y2) atomicSub(sup + e2, 1u);
+                tri = true;
+            }
+        }
+        acc_tri += (uint32_t)__popcll(ballot(tri));
+    }
+    if (lane == 0) {
+        atomicAdd(&ctr->delta_elems, acc_elems);
+        if (acc_tri) atomicAdd(&ctr->delta_tris, acc_tri);
+    }
+}
+
+__global__ void k_finalize(const uint32_t* __restrict__ dl, const uint32_t* __restrict__ nd_dev,
+                           uint32_t* __restrict__ dst, uint32_t* __restrict__ snbr,
+                           const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb) {
+    const uint32_t nd = *nd_dev;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += gridDim.x * blockDim.x) {
         const uint32_t d = dl[t];
         dst[d] = (dst[d] & ~KT_DYING) | KT_DEAD;
@@ -157,20 +326,47 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, u
     return hipGetLastError();
 }
 
-hipError_t delta_update(const uint32_t* dl, uint32_t nd, const uint32_t* src, const uint32_t* dst,
-                        const uint32_t* sptr, const uint32_t* snbr, const uint32_t* seid,
-                        uint32_t* sup, DevCounters* ctr, hipStream_t s) {
-    if (!nd) return hipSuccess;
-    hipLaunchKernelGGL(k_delta, dim3(blocks_for(nd, kBlock / kWave)), dim3(kBlock), 0, s, dl, nd,
-                       src, dst, sptr, snbr, seid, sup, ctr);
+size_t delta_scratch_bytes(uint32_t cap) {
+    const size_t c = cap ? cap : 1;
+    return 16 * c + 4 * c + 4 * (c / kTileD + 2) + 256;
+}
+
+void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
+    const size_t c = cap ? cap : 1;
+    char* p = (char*)base;
+    s.meta = (uint4*)p;
+    s.ntask = (uint32_t*)(p + 16 * c);
+    s.tsum = (uint32_t*)(p + 20 * c);
+    s.cap = cap;
+}
+
+hipError_t delta_update(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                        const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
+                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
+                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
+    if (!nd_max) return hipSuccess;
+    const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
+    const uint32_t tb = tiles < 2048 ? tiles : 2048;
+    hipLaunchKernelGGL(k_dprep, dim3(tb), dim3(kBlock), 0, s, dl, nd_dev, src, dst, sptr, ws.meta,
+                       ws.ntask, ws.tsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dscan_tiles, dim3(1), dim3(1024), 0, s, nd_dev, ws.tsum, ctr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dscan_local, dim3(tb), dim3(kBlock), 0, s, nd_dev, ws.ntask, ws.tsum);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint32_t db = nd_max < 64 ? 64 : (nd_max > 2048 ? 2048 : nd_max);
+    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, dst,
+                       snbr, seid, sup, ctr);
     return hipGetLastError();
 }
 
-hipError_t finalize_deleted(const uint32_t* dl, uint32_t nd, uint32_t* dst, uint32_t* snbr,
-                            const uint32_t* sa, const uint32_t* sb, hipStream_t s) {
-    if (!nd) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nd, kBlock)), dim3(kBlock), 0, s, dl, nd, dst,
-                       snbr, sa, sb);
+hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                            uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
+                            hipStream_t s) {
+    if (!nd_max) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nd_max, kBlock)), dim3(kBlock), 0, s, dl, nd_dev,
+                       dst, snbr, sa, sb);
     return hipGetLastError();
 }
 

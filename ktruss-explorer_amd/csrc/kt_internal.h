@@ -60,7 +60,8 @@ struct DevCounters {
     unsigned int min_sup;             // minimum live support (JUMP)
     unsigned int live;                // entries kept by the last compaction
     unsigned int choice;              // candidate picked by the cost model
-    unsigned int pad[3];
+    unsigned int dtasks;              // DELTA tasks of the pending deletions
+    unsigned int pad[2];
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
 };
@@ -120,11 +121,24 @@ hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E
 // DELTA support maintenance (kt_delta.hip).
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t E, uint32_t k, DevCounters* ctr, hipStream_t s);
-hipError_t delta_update(const uint32_t* dl, uint32_t nd, const uint32_t* src, const uint32_t* dst,
-                        const uint32_t* sptr, const uint32_t* snbr, const uint32_t* seid,
-                        uint32_t* sup, DevCounters* ctr, hipStream_t s);
-hipError_t finalize_deleted(const uint32_t* dl, uint32_t nd, uint32_t* dst, uint32_t* snbr,
-                            const uint32_t* sa, const uint32_t* sb, hipStream_t s);
+// Scratch of a DELTA update for up to `cap` pending deletions.
+struct DeltaScratch {
+    uint4* meta;        // per deletion: short list [x, y), long list [z, z + w)
+    uint32_t* ntask;    // per deletion: 64-candidate tasks, then their exclusive scan
+    uint32_t* tsum;     // per 2048-deletion tile, plus one
+    uint32_t cap;
+};
+size_t delta_scratch_bytes(uint32_t cap);
+void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
+// nd_dev: the number of pending deletions, read on the device (dl[0, *nd_dev));
+// nd_max bounds it and sizes the grids.
+hipError_t delta_update(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                        const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
+                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
+                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                            uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
+                            hipStream_t s);
 // Stable compaction of live entries into the *_out arrays and row-pointer
 // rebuild: tile_sums needs compact_tiles(E) + 1 entries, xs needs E + 1.
 uint32_t compact_tiles(uint32_t E);

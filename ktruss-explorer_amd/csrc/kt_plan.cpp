@@ -305,6 +305,12 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
 // Phase features of the surviving graph (supports used only when `exact`)
 // and, in MODEL mode, the cost model's pick among the candidates.
 int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen) {
+    cfg = fixed_cfg;
+    chosen = 0;
+    if (pick == KT_PICK_FIXED) {  // no pick to make: the phase log gets no features
+        for (int f = 0; f < KT_MLP_IN; ++f) h_feat[f] = -1.f;
+        return KT_OK;
+    }
     KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
                                    sup[cur].as<uint32_t>(), n, E, E0, k, exact,
                                    feat_acc.as<unsigned long long>(), feat.as<float>(), stream));
@@ -479,9 +485,9 @@ int Plan::decompose() {
         ++k;
     }
     if (use_delta && pending) {  // the last pass removed the last edges at k = 3
-        KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), pending, dst[cur].as<uint32_t>(),
-                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
-                                         stream));
+        KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, pending,
+                                         dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
+                                         sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
         pending = 0;
     }
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
@@ -581,8 +587,10 @@ int Plan::ensure_delta() {
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4;
     if ((rc = sptr.alloc(((size_t)n + 1) * 4)) || (rc = snbr0.alloc(2 * Eb)) ||
         (rc = snbr.alloc(2 * Eb)) || (rc = seid.alloc(2 * Eb)) || (rc = sa.alloc(Eb)) ||
-        (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)))
+        (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)) ||
+        (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))))
         return rc;
+    dev::delta_scratch_carve(dscratch.p, E0, dws);
     KT_HIP_TRY(hipMemcpyAsync(sptr.p, hs_ptr.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream));
     if (E0) {
         KT_HIP_TRY(hipMemcpyAsync(snbr0.p, hs_nbr.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
@@ -603,20 +611,24 @@ int Plan::ensure_delta() {
 int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
     (void)ms;
     uint32_t* dl_p = dl.as<uint32_t>();
+    const uint32_t* nd_dev = &ctr.as<DevCounters>()->ndel;
     if (pending && cfg.kernel == KT_COUNT_DELTA) {
         KT_HIP_TRY(hipEventRecord(ev_c0, stream));
-        KT_HIP_TRY(dev::delta_update(dl_p, pending, src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                     sptr.as<uint32_t>(), snbr.as<uint32_t>(), seid.as<uint32_t>(),
-                                     sup[cur].as<uint32_t>(), ctr.as<DevCounters>(), stream));
+        KT_HIP_TRY(dev::delta_update(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
+                                     dst[cur].as<uint32_t>(), sptr.as<uint32_t>(), snbr.as<uint32_t>(),
+                                     seid.as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
+                                     ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c1, stream));
-        KT_HIP_TRY(dev::finalize_deleted(dl_p, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
-                                         sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+        KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
+                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                         stream));
         stats.delta_passes++;
         stats.delta_edges += pending;
     } else {
         if (pending)
-            KT_HIP_TRY(dev::finalize_deleted(dl_p, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
-                                             sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+            KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
+                                             snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                             stream));
         CountCfg rcfg = cfg;
         if (rcfg.kernel == KT_COUNT_DELTA) rcfg.kernel = KT_COUNT_ROW;
         KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
@@ -639,6 +651,7 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
     (void)best_ms;
     const uint32_t nc = (uint32_t)cands.size();
     uint32_t* dl_p = dl.as<uint32_t>();
+    const uint32_t* nd_dev = &ctr.as<DevCounters>()->ndel;
     uint32_t ndc = 0;
     for (const CountCfg& c : cands) ndc += c.kernel == KT_COUNT_DELTA;
     const bool had_pending = pending != 0;
@@ -657,15 +670,17 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
                 KT_HIP_TRY(hipMemcpyAsync(sup[cur].p, sup_snap.p, (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
             first = false;
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
-            KT_HIP_TRY(dev::delta_update(dl_p, pending, src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                         sptr.as<uint32_t>(), snbr.as<uint32_t>(), seid.as<uint32_t>(),
-                                         sup[cur].as<uint32_t>(), ctr_scratch.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::delta_update(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
+                                         dst[cur].as<uint32_t>(), sptr.as<uint32_t>(),
+                                         snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                         sup[cur].as<uint32_t>(), dws, ctr.as<DevCounters>(), stream));
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
         }
     }
     if (had_pending)
-        KT_HIP_TRY(dev::finalize_deleted(dl_p, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
-                                         sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+        KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
+                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                         stream));
     uint32_t last_recount = nc;
     for (uint32_t c = 0; c < nc; ++c)
         if (!(had_pending && cands[c].kernel == KT_COUNT_DELTA)) last_recount = c;
@@ -702,10 +717,10 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     std::vector<double> cand_ms(nc, 0.0);
     double phase_ms = 0.0;
     do {
-        if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
         bool counted = false;
         const uint64_t rec_before = h_ctr->recount_edges;
         float dummy = 0;
+        // the pending count is ctr->ndel of the last mark: reset only afterwards
         if (pending || !(iter == 0 && reuse)) {
             if (pick == KT_PICK_SWEEP) rc = sweep_pass(k, cand_ms, dummy);
             else rc = support_pass(cfg, k, dummy);
@@ -713,6 +728,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             counted = true;
             stats.count_passes++;
         }
+        if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
         if (iter == 0 && jump) {
             if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
             KT_HIP_TRY(dev::min_live_support(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E,

@@ -73,7 +73,8 @@ struct Plan {
     // the oriented host layout kept here) and the list of entries deleted by
     // the last mark pass whose triangles are still counted in the supports
     std::vector<uint32_t> h_ptr, h_src, h_dst;
-    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, sup_snap;
+    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, sup_snap, dscratch;
+    dev::DeltaScratch dws{};
     bool delta_ready = false;
     uint32_t pending = 0;
     bool delta_capable() const;

@@ -34,16 +34,17 @@ import ktruss_amd as K  # noqa: E402
 
 MODEL_PATH = os.path.join(HERE, "picker", "count_picker.npz")
 
-# Candidate count configurations; index 0 is the default variant's own.
+# Candidate support configurations; index 0 is the default variant's own.
+# "delta" maintains the supports across deletions instead of recounting
+# (its first full count uses the ROW kernel with the same block and slice).
 CANDIDATES = [
     ("row", 512, 0),
     ("row", 256, 1024),
-    ("row", 256, 512),
-    ("row", 128, 2048),
     ("row", 1024, 512),
-    ("row", 256, 2048),
+    ("row", 128, 2048),
     ("search", 256, 1024),
     ("merge", 256, 1024),
+    ("delta", 512, 0),
 ]
 
 
