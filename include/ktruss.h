@@ -267,7 +267,9 @@ int kt_mlp_train(kt_mlp* m, const float* x, const float* t, const float* mask, c
  * peel step advances every unfinished graph by one iteration of its own peel
  * (its own k), and the steps are captured once as a HIP graph and replayed.
  * Truss numbers are identical to kt_plan_decompose on each graph alone.
- * Supported variants: directed (by degree or by index), COO+CSR, recount all. */
+ * Supported variants: directed (by degree or by index), COO+CSR, recount all;
+ * count_kernel KT_COUNT_DELTA maintains the supports with DELTA updates after
+ * the first full count (every step: mark, update, control). */
 typedef struct kt_batch kt_batch;
 
 typedef struct kt_batch_tuning {
@@ -296,6 +298,8 @@ typedef struct kt_batch_stats {
     double balg_count_bytes;   /* B_alg of the count passes                        */
     double ms_count;           /* summed count-kernel time (timing mode only)      */
     uint64_t count_launches;   /* count-kernel launches                            */
+    uint64_t delta_elems;      /* DELTA: adjacency entries read                    */
+    uint64_t delta_triangles;  /* DELTA: destroyed triangles removed               */
 } kt_batch_stats;
 
 /* graphs[count] stay referenced by the batch until kt_batch_destroy. */

@@ -13,6 +13,9 @@
 //   dlist/dscan, alist/ascan   graphs to recount (dirty) / still peeling
 //                              (active) and exclusive scans of their tasks
 //   ddesc, adesc (uint4)       task descriptors {begin, end, graph, 0}
+//   sptr, snbr, seid, sa, sb   DELTA only: the union of the graphs' full
+//                              adjacency lists (see kt_delta.hip), dl the
+//                              entries deleted by the current step
 #pragma once
 
 #include "kt_internal.h"
@@ -43,6 +46,7 @@ struct BatchDev {
     uint32_t cchunk;    // entries per count / zero task
     uint32_t mchunk;    // entries per mark task
     uint32_t schedule;  // KT_SCHED_* bits
+    uint32_t delta;     // supports maintained by DELTA updates: never recount
 };
 
 namespace dev {
@@ -50,9 +54,10 @@ hipError_t batch_control(const BatchDev& b, uint32_t mode, hipStream_t s);
 hipError_t batch_expand(const BatchDev& b, uint32_t blocks, hipStream_t s);
 hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, uint32_t blocks,
                       hipStream_t s);
+// dl != nullptr: DELTA marking (KT_DYING, deleted entries appended to dl, count *dn).
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
-                      const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t blocks,
-                      hipStream_t s);
+                      const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
+                      unsigned int* dn, uint32_t blocks, hipStream_t s);
 hipError_t batch_min(const TaskList& tl, const uint32_t* dst, const uint32_t* sup, uint32_t* gmin,
                      uint32_t blocks, hipStream_t s);
 }  // namespace dev

@@ -52,12 +52,27 @@ __global__ void k_b_zero(TaskList tl, const uint32_t* __restrict__ dst, uint32_t
     }
 }
 
+// Wave-aggregated append to a device counter; returns this lane's slot.
+__device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, uint32_t lane) {
+    const uint64_t m = ballot(pred);
+    if (!m) return 0;
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77) with the graph's own k; the
 // minimum support of the entries that stay alive is folded per graph, which
-// is the JUMP target whenever the graph's phase ends with this step.
+// is the JUMP target whenever the graph's phase ends with this step.  DELTA:
+// a deleted entry is flagged KT_DYING and appended to dl (count *dn) for the
+// support update that follows.
+template <bool DELTA>
 __global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
                          uint32_t* __restrict__ tau, const uint32_t* __restrict__ gk,
-                         uint32_t* gndel, uint32_t* gmin) {
+                         uint32_t* gndel, uint32_t* gmin, uint32_t* __restrict__ dl,
+                         unsigned int* dn) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t n = *tl.ntasks;
     for (uint32_t t = gwave(); t < n; t += nwaves()) {
@@ -67,18 +82,23 @@ __global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t
         for (uint32_t base = d.x; base < d.y; base += kWave) {
             const uint32_t i = base + lane;
             bool del = false;
+            uint32_t v = 0;
             if (i < d.y) {
-                const uint32_t v = dst[i];
+                v = dst[i];
                 if (!(v & KT_DEAD)) {
                     const uint32_t s = sup[i];
                     if (s < k - 2) {
                         del = true;
-                        dst[i] = v | KT_DEAD;
+                        dst[i] = v | (DELTA ? KT_DYING : KT_DEAD);
                         tau[i] = k - 1;
                     } else if (s < smin) {
                         smin = s;
                     }
                 }
+            }
+            if (DELTA) {
+                const uint32_t pos = wave_append(dn, del, lane);
+                if (del) dl[pos] = i;
             }
             ndel += (uint32_t)__popcll(ballot(del));
         }
@@ -194,7 +214,7 @@ __global__ __launch_bounds__(1024) void k_b_control(BatchDev b, uint32_t mode) {
                 b.gphase[g] += 1;
                 f = reuse ? 0u : KT_BF_DIRTY;
             } else {
-                f = KT_BF_DIRTY;
+                f = b.delta ? 0u : KT_BF_DIRTY;  // DELTA keeps the supports exact
             }
             b.gndel[g] = 0;
             b.gmin[g] = 0xFFFFFFFFu;
@@ -235,6 +255,7 @@ __global__ __launch_bounds__(1024) void k_b_control(BatchDev b, uint32_t mode) {
     if ((threadIdx.x & (kWave - 1)) == 0 && live_counted)
         atomicAdd(&b.bc->live_in_passes, live_counted);
     if (threadIdx.x == 0) {
+        b.bc->count.ndel = 0;  // the DELTA list of the next mark pass starts empty
         b.bc->ndirty = mode == 1 ? 0u : tot_nd;
         b.bc->nactive = tot_na;
         b.bc->dtasks = mode == 1 ? 0u : tot_td;
@@ -287,10 +308,14 @@ hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, ui
 }
 
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
-                      const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t blocks,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_b_mark, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau, gk,
-                       gndel, gmin);
+                      const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
+                      unsigned int* dn, uint32_t blocks, hipStream_t s) {
+    if (dl)
+        hipLaunchKernelGGL(k_b_mark<true>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
+                           gk, gndel, gmin, dl, dn);
+    else
+        hipLaunchKernelGGL(k_b_mark<false>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
+                           gk, gndel, gmin, dl, dn);
     return hipGetLastError();
 }
 

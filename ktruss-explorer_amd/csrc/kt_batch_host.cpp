@@ -56,6 +56,8 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
         return fail(KT_ERR_ARG, "batched decomposition supports the directed variants");
     if (v.format != KT_COOCSR || v.recount != KT_RECOUNT_ALL)
         return fail(KT_ERR_ARG, "batched decomposition supports COO+CSR with recount all");
+    if (v.count_kernel > KT_COUNT_DELTA) return fail(KT_ERR_ARG, "unknown count_kernel");
+    delta = v.count_kernel == KT_COUNT_DELTA;
     for (uint32_t i = 0; i < count; ++i)
         if (!gs[i]) return fail(KT_ERR_ARG, "graph " + std::to_string(i) + " is NULL");
     G = count;
@@ -75,8 +77,19 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
     N = (uint32_t)nv;
     E = (uint32_t)ne;
 
-    // per-graph layouts (relabel, orientation, sorted rows) written into the union
+    if (delta && N >= KT_DYING) return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^30 vertices");
+    // per-graph layouts (relabel, orientation, sorted rows) written into the union,
+    // and with DELTA the graphs' full adjacency lists (graph g owns the list
+    // slots [2 gbeg[g], 2 gbeg[g + 1]))
     std::vector<uint32_t> hptr((size_t)N + 1), hsrc(E), hdst(E);
+    std::vector<uint32_t> hs_ptr, hs_nbr, hs_eid, hs_a, hs_b;
+    if (delta) {
+        hs_ptr.resize((size_t)N + 1);
+        hs_nbr.resize(2 * (size_t)E);
+        hs_eid.resize(2 * (size_t)E);
+        hs_a.resize(E);
+        hs_b.resize(E);
+    }
     pos_of_edge.assign(G, {});
     std::atomic<uint32_t> mrow{0};
     std::mutex mu;
@@ -104,11 +117,39 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
             uint32_t cur = mrow.load();
             while (L.max_row > cur && !mrow.compare_exchange_weak(cur, L.max_row)) {
             }
+            if (delta) {
+                std::vector<uint32_t> sp, sn, se, a, bb;
+                const int r2 = build_adjacency(L.n, L.E, L.ptr.data(), L.src.data(), L.dst.data(), sp,
+                                               sn, se, a, bb);
+                if (r2) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (!first_rc) {
+                        first_rc = r2;
+                        first_msg = "graph " + std::to_string(i) + ": " + last_error();
+                    }
+                    continue;
+                }
+                const uint64_t sbase = 2 * (uint64_t)eb;
+                for (uint32_t x = 0; x < L.n; ++x) hs_ptr[(size_t)vb + x] = (uint32_t)(sbase + sp[x]);
+                for (size_t q = 0; q < sn.size(); ++q) {
+                    hs_nbr[sbase + q] = vb + sn[q];
+                    hs_eid[sbase + q] = eb + se[q];
+                }
+                for (uint32_t x = 0; x < L.E; ++x) {
+                    hs_a[(size_t)eb + x] = (uint32_t)(sbase + a[x]);
+                    hs_b[(size_t)eb + x] = (uint32_t)(sbase + bb[x]);
+                }
+            }
             pos_of_edge[i] = std::move(L.pos_of_edge);
         }
     });
     if (first_rc) return fail(first_rc, first_msg);
     hptr[N] = E;
+    if (delta) {
+        if (2 * (uint64_t)E >= 0xFFFFFFFFull)
+            return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^31 entries");
+        hs_ptr[N] = 2 * E;
+    }
     max_row = mrow.load();
 
     hipDeviceProp_t prop;
@@ -139,6 +180,21 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
     }
     KT_HIP_TRY(hipMemcpyAsync(d_gbeg.p, gbeg.data(), ((size_t)G + 1) * 4, hipMemcpyHostToDevice, stream));
     KT_HIP_TRY(dev::fill_u32(gstate.as<uint32_t>(), 0, (size_t)kStateArrays * G, stream));
+    if (delta) {
+        if ((rc = sptr.alloc(((size_t)N + 1) * 4)) || (rc = snbr0.alloc(2 * Eb)) ||
+            (rc = snbr.alloc(2 * Eb)) || (rc = seid.alloc(2 * Eb)) || (rc = sa.alloc(Eb)) ||
+            (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)) ||
+            (rc = dscratch.alloc(dev::delta_scratch_bytes(E))))
+            return rc;
+        dev::delta_scratch_carve(dscratch.p, E, dws);
+        KT_HIP_TRY(hipMemcpyAsync(sptr.p, hs_ptr.data(), ((size_t)N + 1) * 4, hipMemcpyHostToDevice, stream));
+        if (E) {
+            KT_HIP_TRY(hipMemcpyAsync(snbr0.p, hs_nbr.data(), 2 * (size_t)E * 4, hipMemcpyHostToDevice, stream));
+            KT_HIP_TRY(hipMemcpyAsync(seid.p, hs_eid.data(), 2 * (size_t)E * 4, hipMemcpyHostToDevice, stream));
+            KT_HIP_TRY(hipMemcpyAsync(sa.p, hs_a.data(), (size_t)E * 4, hipMemcpyHostToDevice, stream));
+            KT_HIP_TRY(hipMemcpyAsync(sb.p, hs_b.data(), (size_t)E * 4, hipMemcpyHostToDevice, stream));
+        }
+    }
     KT_HIP_TRY(hipStreamSynchronize(stream));
 
     uint32_t* st = gstate.as<uint32_t>();
@@ -163,6 +219,7 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
     bd.adesc = adesc.as<uint4>();
     bd.bc = bc.as<BatchCounters>();
     bd.schedule = v.schedule;
+    bd.delta = delta ? 1u : 0u;
     graphs.assign(gs, gs + G);
     for (const kt_graph* g : graphs) graph_retain(g);
     kt_batch_tuning t{};
@@ -219,6 +276,8 @@ uint32_t Batch::count_blocks() const { return num_cu * tune.blocks_per_cu; }
 
 int Batch::reset() {
     if (E) KT_HIP_TRY(hipMemcpyAsync(dst.p, dst0.p, (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
+    if (delta && E)
+        KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, 2 * (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
     KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E, stream));
     KT_HIP_TRY(dev::fill_u32(bc.as<uint32_t>(), 0, sizeof(BatchCounters) / 4, stream));
     std::memset(h_bc, 0, sizeof(BatchCounters));
@@ -240,7 +299,7 @@ int Batch::event_pair(size_t i, hipEvent_t& a, hipEvent_t& b) {
 
 // First full count and the first JUMP.
 int Batch::prestep(hipEvent_t c0, hipEvent_t c1) {
-    const TaskList dl{bd.ddesc, &bd.bc->dtasks}, al{bd.adesc, &bd.bc->atasks};
+    const TaskList dtl{bd.ddesc, &bd.bc->dtasks}, atl{bd.adesc, &bd.bc->atasks};
     CountArgs a{};
     a.ptr = ptr.as<uint32_t>();
     a.src = src.as<uint32_t>();
@@ -252,11 +311,11 @@ int Batch::prestep(hipEvent_t c0, hipEvent_t c1) {
     a.ctr = &bd.bc->count;
     KT_HIP_TRY(dev::batch_control(bd, 0, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
-    KT_HIP_TRY(dev::batch_zero(dl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
+    KT_HIP_TRY(dev::batch_zero(dtl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
     if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
-    KT_HIP_TRY(dev::count_supports_batch(a, dl, tune.block_size, count_blocks(), stream));
+    KT_HIP_TRY(dev::count_supports_batch(a, dtl, tune.block_size, count_blocks(), stream));
     if (c1) KT_HIP_TRY(hipEventRecord(c1, stream));
-    KT_HIP_TRY(dev::batch_min(al, dst.as<uint32_t>(), sup.as<uint32_t>(), bd.gmin, stream_blocks(), stream));
+    KT_HIP_TRY(dev::batch_min(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), bd.gmin, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_control(bd, 1, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
     return KT_OK;
@@ -264,7 +323,7 @@ int Batch::prestep(hipEvent_t c0, hipEvent_t c1) {
 
 // mark -> control -> expand -> zero -> count
 int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
-    const TaskList dl{bd.ddesc, &bd.bc->dtasks}, al{bd.adesc, &bd.bc->atasks};
+    const TaskList dtl{bd.ddesc, &bd.bc->dtasks}, atl{bd.adesc, &bd.bc->atasks};
     CountArgs a{};
     a.ptr = ptr.as<uint32_t>();
     a.src = src.as<uint32_t>();
@@ -274,13 +333,29 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
     a.cap_lds = cap;
     a.ytiles = 1;
     a.ctr = &bd.bc->count;
-    KT_HIP_TRY(dev::batch_mark(al, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
-                               bd.gndel, bd.gmin, stream_blocks(), stream));
+    if (delta) {
+        // mark -> DELTA update of this step's deletions -> finalize -> control -> expand
+        unsigned int* dn = &bd.bc->count.ndel;
+        KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
+                                   bd.gndel, bd.gmin, dl.as<uint32_t>(), dn, stream_blocks(), stream));
+        if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
+        KT_HIP_TRY(dev::delta_update(dl.as<uint32_t>(), dn, E, src.as<uint32_t>(), dst.as<uint32_t>(),
+                                     sptr.as<uint32_t>(), snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                     sup.as<uint32_t>(), dws, &bd.bc->count, stream));
+        if (c1) KT_HIP_TRY(hipEventRecord(c1, stream));
+        KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), dn, E, dst.as<uint32_t>(),
+                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+        KT_HIP_TRY(dev::batch_control(bd, 2, stream));
+        KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
+        return KT_OK;
+    }
+    KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
+                               bd.gndel, bd.gmin, nullptr, nullptr, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_control(bd, 2, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
-    KT_HIP_TRY(dev::batch_zero(dl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
+    KT_HIP_TRY(dev::batch_zero(dtl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
     if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
-    KT_HIP_TRY(dev::count_supports_batch(a, dl, tune.block_size, count_blocks(), stream));
+    KT_HIP_TRY(dev::count_supports_batch(a, dtl, tune.block_size, count_blocks(), stream));
     if (c1) KT_HIP_TRY(hipEventRecord(c1, stream));
     return KT_OK;
 }
@@ -373,6 +448,8 @@ int Batch::decompose() {
     stats.replays = replays;
     stats.recount_edges = h_bc->count.recount_edges;
     stats.merge_elems = h_bc->count.merge_elems;
+    stats.delta_elems = h_bc->count.delta_elems;
+    stats.delta_triangles = h_bc->count.delta_tris;
     stats.live_in_passes = h_bc->live_in_passes;
     stats.ms_total = ms;
     stats.balg_count_bytes = 4.0 * (double)stats.merge_elems + 24.0 * (double)stats.recount_edges +

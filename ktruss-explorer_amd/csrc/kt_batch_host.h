@@ -20,6 +20,9 @@ struct Batch {
 
     DBuf ptr, src, dst0, dst, sup, tau;
     DBuf d_gbeg, gstate, lists, ddesc, adesc, bc;
+    bool delta = false;                               // DELTA support maintenance
+    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, dscratch;
+    dev::DeltaScratch dws{};
     BatchCounters* h_bc = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;

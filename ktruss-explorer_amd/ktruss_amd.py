@@ -120,7 +120,8 @@ class BatchStats(ctypes.Structure):
                 ("merge_elems", ctypes.c_uint64), ("live_in_passes", ctypes.c_uint64),
                 ("k_max", ctypes.c_uint32), ("pad", ctypes.c_uint32),
                 ("ms_total", ctypes.c_double), ("balg_count_bytes", ctypes.c_double),
-                ("ms_count", ctypes.c_double), ("count_launches", ctypes.c_uint64)]
+                ("ms_count", ctypes.c_double), ("count_launches", ctypes.c_uint64),
+                ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}

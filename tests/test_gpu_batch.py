@@ -164,3 +164,38 @@ def test_batch_timing_mode_same_results(batch_data):
     assert st["count_launches"] >= 1
     for key in ("steps", "iterations", "phases", "recount_edges", "merge_elems"):
         assert st[key] == st_graph[key], key
+
+
+@pytest.mark.parametrize("d", ["directed-by-degree", "directed-by-index"])
+def test_batch_delta_matches_oracle(batch_data, d):
+    raw, graphs, taus = batch_data
+    b = K.Batch(graphs, K.make_variant(d, count_kernel="delta"))
+    st = b.decompose()
+    _check_all(b, graphs, taus)
+    # every triangle of every graph is removed exactly once
+    assert st["delta_triangles"] == sum(t[1]["triangles"] for t in taus)
+    for i, (tau_o, st_o) in enumerate(taus):
+        kmax, iters, phases, passes = b.graph_info(i)
+        assert kmax == st_o["k_max"] and phases == st_o["phases"], i
+        assert passes == 1, i  # the first full count only; DELTA keeps the rest exact
+    st2 = b.decompose()  # a second run restores the adjacency flags
+    _check_all(b, graphs, taus)
+    assert st2["delta_triangles"] == st["delta_triangles"]
+
+
+def test_batch_delta_tunings_and_timing(batch_data):
+    raw, graphs, taus = batch_data
+    b = K.Batch(graphs, K.make_variant(count_kernel="delta"))
+    b.set_tuning(block_size=256, chunk=64, cap_lds=64, blocks_per_cu=1, mark_chunk=64,
+                 steps_per_replay=3)
+    b.decompose()
+    _check_all(b, graphs, taus)
+    b.set_timing(True)
+    st = b.decompose()
+    _check_all(b, graphs, taus)
+    assert st["ms_count"] > 0
+    best = b.autotune(trial_steps=3, repeats=1)
+    assert best == b.tuning()
+    b.set_timing(False)
+    b.decompose()
+    _check_all(b, graphs, taus)
