@@ -5,15 +5,20 @@ A step is one full decomposition (k++ loop over the peel) of a synthetic
 R-MAT graph whose oriented COO+CSR layout is already resident in HBM; the
 step ends with the truss number of every edge resident in HBM.  Workload:
 R-MAT scale 20, edge factor 16, (a, b, c, d) = (.57, .19, .19, .05), seed 1,
-symmetrised, self-loops and duplicates removed; default variant
-(directed-by-degree / coocsr / recount all).
+symmetrised, self-loops and duplicates removed.
+
+The measured steps use the learned per-phase pick (the cost model in
+ktruss-explorer_amd/picker/ chooses, at every phase, among recount
+configurations and DELTA support maintenance); one run of the fixed default
+variant (directed-by-degree / coocsr / recount all, the reference's method)
+is timed beside it, and the two are checked bit-exact against each other.
 
 Prints ONE JSON line.  value = undirected edges / seconds per step (wall time
 of K steps between two device synchronisations).  The roofline object is for
-the support-count kernel: algorithmic bytes of all its launches in the timed
-steps divided by their summed HIP-event durations on the library's stream.
-cpu_baseline times the CPU oracle (single thread) on a bounded sample of the
-same workload, R-MAT scale 14 with the same parameters.
+the kernel with the most device time in the measured steps: its algorithmic
+bytes divided by the summed HIP-event durations of its launches on the
+library's stream.  cpu_baseline times the CPU oracle (single thread) on a
+bounded sample of the same workload, R-MAT scale 14 with the same parameters.
 """
 import argparse
 import json
@@ -179,6 +184,8 @@ def bench_batch(args, K):
 def parse_variant(spec, K):
     if spec == "default":
         return K.default_variant()
+    if spec == "delta":
+        return K.make_variant(count_kernel="delta")
     parts = spec.split("/")
     return K.make_variant(*parts[:3])
 
@@ -191,8 +198,12 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--mode", choices=["learned", "fixed"], default="learned",
+                    help="learned: the cost model picks per phase (default); fixed: --variant")
     ap.add_argument("--variant", default="default",
-                    help="default or directedness/format/recount, e.g. undirected/coocsr/affected")
+                    help="with --mode fixed: default, delta, or directedness/format/recount")
+    ap.add_argument("--skip-default", action="store_true",
+                    help="skip the comparison run of the fixed default variant")
     ap.add_argument("--cpu-scale", type=int, default=14)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["rmat", "batch"], default="rmat",
@@ -214,54 +225,104 @@ def main():
     if args.workload == "batch":
         bench_batch(args, K)
         return
-    log(f"generating R-MAT scale {args.scale} ef {args.edge_factor} seed {args.seed}")
-    n, s, d = K.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
-    g = K.Graph(n, s, d, pairs=True)
-    v = parse_variant(args.variant, K)
-    log(f"graph: n={n} m={g.m} max degree={g.max_degree}; building plan {v.describe()}")
-    plan = K.Plan(g, v)
+    bench_rmat(args, K)
 
-    for i in range(args.warmup):
-        st = plan.decompose()
-        log(f"warmup {i}: {st['ms_total']:.1f} ms device, k_max {st['k_max']}, "
-            f"{st['iterations']} iterations")
 
-    K.device_synchronize()
-    per_step = []
-    ms_count = 0.0
-    launches = 0
-    balg = 0.0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = plan.decompose()
-        per_step.append(st["ms_total"])
-        ms_count += st["ms_count"]
-        launches += st["count_launches"]
-        balg += st["balg_count_bytes"]
-        log(f"step {i}: {st['ms_total']:.1f} ms device, count kernel {st['ms_count']:.1f} ms "
-            f"in {st['count_launches']} launches")
-    K.device_synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / max(args.steps, 1)
-
-    achieved = balg / (ms_count * 1e-3) / 1e9 if ms_count > 0 else 0.0
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-        if t.get("workload_scale") == args.scale and t.get("variant") == v.describe():
-            traffic = t.get("hbm_bytes_per_launch")
-    roofline = {
+def roofline_obj(balg, ms, launches, kernel, traffic=None):
+    achieved = balg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "k_count (support count)",
+        "kernel": kernel,
         "alg_bytes_per_launch": balg / max(launches, 1),
-        "avg_launch_ms": ms_count / max(launches, 1),
+        "avg_launch_ms": ms / max(launches, 1),
     }
+
+
+def bench_rmat(args, K):
+    """One R-MAT graph: the learned per-phase pick (the measured steps) beside
+    one run of the fixed default variant (the reference's method)."""
+    log(f"generating R-MAT scale {args.scale} ef {args.edge_factor} seed {args.seed}")
+    n, s, d = K.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
+    g = K.Graph(n, s, d, pairs=True)
+    log(f"graph: n={n} m={g.m} max degree={g.max_degree}")
+
+    default = None
+    tau_default = None
+    if not args.skip_default:
+        pd = K.Plan(g, K.default_variant())
+        K.device_synchronize()
+        t0 = time.perf_counter()
+        st = pd.decompose()
+        K.device_synchronize()
+        wall = time.perf_counter() - t0
+        tau_default = pd.truss_numbers()
+        default = {"ms_per_step": 1e3 * wall, "device_ms": st["ms_total"],
+                   "edges_per_s": g.m / wall, "k_max": st["k_max"],
+                   "iterations": st["iterations"],
+                   "count_roofline": roofline_obj(st["balg_count_bytes"], st["ms_count"],
+                                                  st["count_launches"], "k_count_dir (support count)")}
+        log(f"default variant: {1e3 * wall:.1f} ms, k_max {st['k_max']}, {st['iterations']} iterations")
+        del pd
+
+    if args.mode == "learned":
+        import learn
+        model, configs = learn.load_model()
+        plan = K.Plan(g, K.default_variant())
+        plan.set_candidates(configs)
+        plan.set_pick(K.PICK_MODEL, model)
+        label = "learned per-phase pick over " + ", ".join(c.describe() for c in configs)
+    else:
+        v = parse_variant(args.variant, K)
+        plan = K.Plan(g, v)
+        label = "fixed " + v.describe()
+    log(f"measured: {label}")
+
+    for i in range(args.warmup):
+        st = plan.decompose()
+        log(f"warmup {i}: {st['ms_total']:.1f} ms device, k_max {st['k_max']}")
+    K.device_synchronize()
+    per_step = []
+    acc = {"ms_count": 0.0, "count_launches": 0, "balg_count_bytes": 0.0, "ms_delta": 0.0,
+           "delta_passes": 0, "balg_delta_bytes": 0.0}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = plan.decompose()
+        per_step.append(st["ms_total"])
+        for key in acc:
+            acc[key] += st[key]
+        log(f"step {i}: {st['ms_total']:.1f} ms device (count {st['ms_count']:.1f} ms, "
+            f"DELTA {st['ms_delta']:.1f} ms)")
+    K.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / max(args.steps, 1)
+    tau = plan.truss_numbers()
+
+    count_rl = roofline_obj(acc["balg_count_bytes"], acc["ms_count"], acc["count_launches"],
+                            "k_count_dir (support count)")
+    delta_rl = roofline_obj(acc["balg_delta_bytes"], acc["ms_delta"], acc["delta_passes"],
+                            "k_delta (DELTA support update)")
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        for rl in (count_rl, delta_rl):
+            key = rl["kernel"].split()[0]
+            if t.get("workload_scale") == args.scale and key in t.get("hbm_bytes_per_launch", {}):
+                rl["traffic"] = t["hbm_bytes_per_launch"][key]
+    dominant = delta_rl if acc["ms_delta"] > acc["ms_count"] else count_rl
+    other = count_rl if dominant is delta_rl else delta_rl
+    picks = None
+    if args.mode == "learned":
+        ph = plan.phase_log()
+        names = [c.describe() for c in plan.candidates]
+        counts = {}
+        for c in ph[:, 2].astype(int):
+            counts[names[c]] = counts.get(names[c], 0) + 1
+        picks = counts
     out = {
         "metric": METRIC,
         "value": g.m / (ms_per_step * 1e-3),
@@ -278,14 +339,20 @@ def main():
         "config": {
             "workload": (f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, a/b/c/d = "
                          f".57/.19/.19/.05, seed {args.seed}, cleaned: full k-truss decomposition"),
-            "variant": v.describe(),
+            "measured": label,
             "n": n,
             "m": g.m,
             "k_max": st["k_max"],
             "iterations": st["iterations"],
             "median_device_ms": statistics.median(per_step) if per_step else None,
+            "phase_picks": picks,
+            "default_variant": default,
+            "speedup_vs_default": (default["ms_per_step"] / ms_per_step) if default else None,
+            "bit_exact_vs_default": (bool((tau == tau_default).all())
+                                     if tau_default is not None else None),
+            "other_kernel_roofline": other,
         },
-        "roofline": roofline,
+        "roofline": dominant,
     }
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on R-MAT scale {args.cpu_scale}")

@@ -114,13 +114,16 @@ typedef struct kt_stats {
     uint32_t k_max;           /* largest truss number             */
     uint32_t k_last;          /* k of the last phase              */
     double ms_total;          /* device time of the whole run (hipEvents)        */
-    double ms_count;          /* summed device time of count-kernel launches      */
+    double ms_count;          /* summed device time of count-kernel launches (recounts) */
     uint64_t count_launches;  /* count-kernel launches timed into ms_count        */
     double balg_count_bytes;  /* B_alg of the count passes only                   */
     uint64_t delta_passes;    /* support updates done by DELTA (counted in count_passes) */
     uint64_t delta_edges;     /* deleted edges processed by DELTA                  */
     uint64_t delta_elems;     /* sum over them of the shorter adjacency list       */
     uint64_t delta_triangles; /* destroyed triangles found                         */
+    double ms_delta;          /* summed device time of the DELTA update kernel     */
+    double balg_delta_bytes;  /* DELTA algorithmic bytes: 4 (candidates + window
+                                 entries read) + 32 per destroyed triangle          */
 } kt_stats;
 
 typedef struct kt_graph kt_graph; /* validated host-side edge set            */
@@ -300,6 +303,8 @@ typedef struct kt_batch_stats {
     uint64_t count_launches;   /* count-kernel launches                            */
     uint64_t delta_elems;      /* DELTA: adjacency entries read                    */
     uint64_t delta_triangles;  /* DELTA: destroyed triangles removed               */
+    double ms_delta;           /* summed DELTA update time (timing mode only)      */
+    double balg_delta_bytes;   /* DELTA algorithmic bytes (as kt_stats)            */
 } kt_batch_stats;
 
 /* graphs[count] stay referenced by the batch until kt_batch_destroy. */

@@ -338,10 +338,11 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
         unsigned int* dn = &bd.bc->count.ndel;
         KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
                                    bd.gndel, bd.gmin, dl.as<uint32_t>(), dn, stream_blocks(), stream));
+        KT_HIP_TRY(dev::delta_prepare(dl.as<uint32_t>(), dn, E, src.as<uint32_t>(), dst.as<uint32_t>(),
+                                      sptr.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
-        KT_HIP_TRY(dev::delta_update(dl.as<uint32_t>(), dn, E, src.as<uint32_t>(), dst.as<uint32_t>(),
-                                     sptr.as<uint32_t>(), snbr.as<uint32_t>(), seid.as<uint32_t>(),
-                                     sup.as<uint32_t>(), dws, &bd.bc->count, stream));
+        KT_HIP_TRY(dev::delta_apply(dl.as<uint32_t>(), dn, E, dst.as<uint32_t>(), snbr.as<uint32_t>(),
+                                    seid.as<uint32_t>(), sup.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c1) KT_HIP_TRY(hipEventRecord(c1, stream));
         KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), dn, E, dst.as<uint32_t>(),
                                          snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
@@ -450,16 +451,23 @@ int Batch::decompose() {
     stats.merge_elems = h_bc->count.merge_elems;
     stats.delta_elems = h_bc->count.delta_elems;
     stats.delta_triangles = h_bc->count.delta_tris;
+    stats.balg_delta_bytes = 4.0 * (double)h_bc->count.delta_elems +
+                             4.0 * (double)h_bc->count.delta_win +
+                             32.0 * (double)h_bc->count.delta_tris;
     stats.live_in_passes = h_bc->live_in_passes;
     stats.ms_total = ms;
     stats.balg_count_bytes = 4.0 * (double)stats.merge_elems + 24.0 * (double)stats.recount_edges +
                              12.0 * (double)stats.live_in_passes;
     stats.count_launches = 1 + replays * tune.steps_per_replay;
+    // timing mode: pair 0 brackets the first full count, the others the
+    // steps' count launches (recount mode) or DELTA updates (DELTA mode)
     for (size_t i = 0; timed && i < nev; ++i) {
         float x = 0;
         KT_HIP_TRY(hipEventElapsedTime(&x, tev[2 * i], tev[2 * i + 1]));
-        stats.ms_count += x;
+        if (i > 0 && delta) stats.ms_delta += x;
+        else stats.ms_count += x;
     }
+    if (delta) stats.count_launches = 1;
     have_tau = true;
     return KT_OK;
 }

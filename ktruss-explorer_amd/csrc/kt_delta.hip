@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         if (toff[i + half] <= t0) i += half;
         len -= half;
     }
-    unsigned long long acc_elems = 0, acc_tri = 0;
+    unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0;
     for (uint32_t t = t0; t < t1; ++t) {
         while (i + 1 < nd && toff[i + 1] <= t) ++i;
         const uint4 m = meta[i];
@@ -256,6 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         const uint32_t hi = lower_bound_u(snbr, lo, m.z + m.w - lo, xmax + 1);
         const uint32_t wl = hi - lo;
         if (wl == 0) continue;
+        acc_win += wl;
         uint32_t p;
         bool hit;
         if (wl <= kWin) {
@@ -292,6 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
     }
     if (lane == 0) {
         atomicAdd(&ctr->delta_elems, acc_elems);
+        if (acc_win) atomicAdd(&ctr->delta_win, acc_win);
         if (acc_tri) atomicAdd(&ctr->delta_tris, acc_tri);
     }
 }
@@ -340,10 +342,9 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
     s.cap = cap;
 }
 
-hipError_t delta_update(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
-                        const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
-                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
-                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
+hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                         const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
+                         const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
     if (!nd_max) return hipSuccess;
     const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
     const uint32_t tb = tiles < 2048 ? tiles : 2048;
@@ -354,7 +355,13 @@ hipError_t delta_update(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_
     hipLaunchKernelGGL(k_dscan_tiles, dim3(1), dim3(1024), 0, s, nd_dev, ws.tsum, ctr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dscan_local, dim3(tb), dim3(kBlock), 0, s, nd_dev, ws.ntask, ws.tsum);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                       const uint32_t* dst, const uint32_t* snbr, const uint32_t* seid,
+                       uint32_t* sup, const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
+    if (!nd_max) return hipSuccess;
     const uint32_t db = nd_max < 64 ? 64 : (nd_max > 2048 ? 2048 : nd_max);
     hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, dst,
                        snbr, seid, sup, ctr);

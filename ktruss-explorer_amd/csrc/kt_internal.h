@@ -64,6 +64,7 @@ struct DevCounters {
     unsigned int pad[2];
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
+    unsigned long long delta_win;     // longer-list window entries read by DELTA updates
 };
 
 // Arguments of the support-count kernels.
@@ -131,11 +132,14 @@ struct DeltaScratch {
 size_t delta_scratch_bytes(uint32_t cap);
 void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
 // nd_dev: the number of pending deletions, read on the device (dl[0, *nd_dev));
-// nd_max bounds it and sizes the grids.
-hipError_t delta_update(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
-                        const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
-                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
-                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+// nd_max bounds it and sizes the grids.  delta_prepare lists the tasks (three
+// small kernels), delta_apply is the update proper (k_delta).
+hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                         const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
+                         const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
+                       const uint32_t* dst, const uint32_t* snbr, const uint32_t* seid,
+                       uint32_t* sup, const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
 hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                             uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                             hipStream_t s);

@@ -7,11 +7,13 @@
 
 #include "ktruss.h"
 
-#define KT_FEAT_ACC 12  // u64 accumulators of the phase-feature kernels
+#define KT_FEAT_ACC 12     // u64 partials per workgroup of the phase-feature kernels
+#define KT_FEAT_BLOCKS 256 // workgroups of the phase-feature kernels
 
 namespace kt {
 namespace dev {
 uint32_t mlp_num_params(uint32_t n_out);
+// acc: KT_FEAT_BLOCKS * KT_FEAT_ACC u64 partials.
 hipError_t phase_features(const uint32_t* ptr, const uint32_t* dst, const uint32_t* sup, uint32_t n,
                           uint32_t E, uint32_t E0, uint32_t k, bool sup_valid,
                           unsigned long long* acc, float* feat, hipStream_t s);

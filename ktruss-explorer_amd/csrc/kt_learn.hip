@@ -34,66 +34,85 @@ __host__ __device__ inline Layout layout(int C) {
 }
 
 // ---------------------------------------------------------------------------
-// Phase features.  Accumulators (u64): 0 rows, 1 sum len, 2 sum len^2,
-// 3 max len, 4 entries in rows > 256, 5 entries in rows > 1024,
-// 6..10 support histogram bins, 11 entries histogrammed.
-__global__ void k_feat_rows(const uint32_t* __restrict__ ptr, uint32_t n,
-                            unsigned long long* acc) {
-    unsigned long long rows = 0, s1 = 0, s2 = 0, big = 0, huge = 0;
-    unsigned int mx = 0;
+// Phase features.  Every workgroup of the two feature kernels (kFeatBlocks of
+// them, grid-stride) writes its own partials; k_feat_finish sums them in a
+// fixed order (no atomics, deterministic).  Partials (u64) per workgroup:
+// 0 rows, 1 sum len, 2 sum len^2, 3 max len, 4 entries in rows > 256,
+// 5 entries in rows > 1024 (stored rows), 6..10 support histogram bins,
+// 11 live entries.
+constexpr uint32_t kFeatBlocks = KT_FEAT_BLOCKS;
+constexpr uint32_t kFeatThreads = 256;
+
+template <int N>
+__device__ void block_sum_store(unsigned long long (&v)[N], bool is_max_3,
+                                unsigned long long* out) {
+    __shared__ unsigned long long s[kFeatThreads / 64][N];
+    for (int j = 0; j < N; ++j)
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(v[j], off);
+            v[j] = (is_max_3 && j == 3) ? (o > v[j] ? o : v[j]) : v[j] + o;
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int j = 0; j < N; ++j) s[threadIdx.x / 64][j] = v[j];
+    __syncthreads();
+    if (threadIdx.x < (unsigned)N) {
+        const int j = threadIdx.x;
+        unsigned long long r = s[0][j];
+        for (uint32_t w = 1; w < kFeatThreads / 64; ++w)
+            r = (is_max_3 && j == 3) ? (s[w][j] > r ? s[w][j] : r) : r + s[w][j];
+        out[j] = r;
+    }
+}
+
+__global__ __launch_bounds__(kFeatThreads) void k_feat_rows(const uint32_t* __restrict__ ptr,
+                                                           uint32_t n, unsigned long long* part) {
+    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // rows, s1, s2, max, big, huge
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         const uint32_t len = ptr[r + 1] - ptr[r];
         if (!len) continue;
-        rows += 1;
-        s1 += len;
-        s2 += (unsigned long long)len * len;
-        if (len > 256) big += len;
-        if (len > 1024) huge += len;
-        mx = len > mx ? len : mx;
+        v[0] += 1;
+        v[1] += len;
+        v[2] += (unsigned long long)len * len;
+        v[3] = len > v[3] ? len : v[3];
+        if (len > 256) v[4] += len;
+        if (len > 1024) v[5] += len;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        rows += __shfl_xor(rows, off);
-        s1 += __shfl_xor(s1, off);
-        s2 += __shfl_xor(s2, off);
-        big += __shfl_xor(big, off);
-        huge += __shfl_xor(huge, off);
-        const unsigned int o = __shfl_xor(mx, off);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(acc + 0, rows);
-        atomicAdd(acc + 1, s1);
-        atomicAdd(acc + 2, s2);
-        atomicMax(acc + 3, (unsigned long long)mx);
-        atomicAdd(acc + 4, big);
-        atomicAdd(acc + 5, huge);
-    }
+    block_sum_store<6>(v, true, part + (size_t)blockIdx.x * KT_FEAT_ACC);
 }
 
-__global__ void k_feat_support(const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
-                               uint32_t E, uint32_t k, unsigned long long* acc) {
-    unsigned long long bins[5] = {0, 0, 0, 0, 0};
+__global__ __launch_bounds__(kFeatThreads) void k_feat_support(const uint32_t* __restrict__ dst,
+                                                              const uint32_t* __restrict__ sup,
+                                                              uint32_t E, uint32_t k, int sup_valid,
+                                                              unsigned long long* part) {
+    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries
     const float denom = (float)(k > 2 ? k - 2 : 1);
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        if (dst[e] & KT_DEAD) continue;
-        const float r = (float)sup[e] / denom;
-        const int b = r < 1.f ? 0 : r < 2.f ? 1 : r < 4.f ? 2 : r < 8.f ? 3 : 4;
-        bins[b] += 1;
+        if (dst[e] & (KT_DEAD | KT_DYING)) continue;
+        v[5] += 1;
+        if (sup_valid) {
+            const float r = (float)sup[e] / denom;
+            const int b = r < 1.f ? 0 : r < 2.f ? 1 : r < 4.f ? 2 : r < 8.f ? 3 : 4;
+            v[b] += 1;
+        }
     }
-    for (int b = 0; b < 5; ++b)
-        for (int off = 32; off > 0; off >>= 1) bins[b] += __shfl_xor(bins[b], off);
-    if ((threadIdx.x & 63) == 0)
-        for (int b = 0; b < 5; ++b) atomicAdd(acc + 6 + b, bins[b]);
-}
-
-__global__ void k_fill_acc(unsigned long long* acc) {
-    if (threadIdx.x < KT_FEAT_ACC) acc[threadIdx.x] = 0;
+    block_sum_store<6>(v, false, part + (size_t)blockIdx.x * KT_FEAT_ACC + 6);
 }
 
 __device__ float log2p1(float x) { return log2f(1.f + x); }
 
-__global__ void k_feat_finish(const unsigned long long* acc, uint32_t E0, uint32_t k,
+__global__ void k_feat_finish(const unsigned long long* part, uint32_t E0, uint32_t k,
                               int sup_valid, float* feat) {
+    __shared__ unsigned long long acc[KT_FEAT_ACC];
+    if (threadIdx.x < KT_FEAT_ACC) {
+        const int j = threadIdx.x;
+        unsigned long long r = 0;
+        for (uint32_t b = 0; b < kFeatBlocks; ++b) {
+            const unsigned long long x = part[(size_t)b * KT_FEAT_ACC + j];
+            r = j == 3 ? (x > r ? x : r) : r + x;
+        }
+        acc[j] = r;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     const float rows = (float)acc[0], s1 = (float)acc[1], s2 = (float)acc[2];
     const float mean = rows > 0 ? s1 / rows : 0.f;
@@ -104,7 +123,7 @@ __global__ void k_feat_finish(const unsigned long long* acc, uint32_t E0, uint32
     feat[4] = mean > 0 ? sqrtf(s2 / rows) / mean / 8.f : 0.f;
     feat[5] = s1 > 0 ? (float)acc[4] / s1 : 0.f;
     feat[6] = s1 > 0 ? (float)acc[5] / s1 : 0.f;
-    feat[7] = E0 ? s1 / (float)E0 : 0.f;
+    feat[7] = E0 ? (float)acc[11] / (float)E0 : 0.f;  // live fraction
     feat[8] = log2f((float)k) / 10.f;
     feat[9] = s1 > 0 ? s2 / s1 / 1024.f : 0.f;
     float tot = 0;
@@ -256,17 +275,12 @@ uint32_t mlp_num_params(uint32_t C) { return (uint32_t)layout((int)C).total; }
 hipError_t phase_features(const uint32_t* ptr, const uint32_t* dst, const uint32_t* sup, uint32_t n,
                           uint32_t E, uint32_t E0, uint32_t k, bool sup_valid,
                           unsigned long long* acc, float* feat, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_acc, dim3(1), dim3(64), 0, s, acc);
+    hipLaunchKernelGGL(k_feat_rows, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, ptr, n, acc);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    const uint32_t gr = n ? std::min<uint32_t>(2048, (n + 255) / 256) : 1;
-    hipLaunchKernelGGL(k_feat_rows, dim3(gr), dim3(256), 0, s, ptr, n, acc);
+    hipLaunchKernelGGL(k_feat_support, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, dst, sup, E, k,
+                       sup_valid ? 1 : 0, acc);
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    if (sup_valid && E) {
-        const uint32_t ge = std::min<uint32_t>(2048, (E + 255) / 256);
-        hipLaunchKernelGGL(k_feat_support, dim3(ge), dim3(256), 0, s, dst, sup, E, k, acc);
-        if ((err = hipGetLastError()) != hipSuccess) return err;
-    }
     hipLaunchKernelGGL(k_feat_finish, dim3(1), dim3(64), 0, s, acc, E0, k, sup_valid ? 1 : 0, feat);
     return hipGetLastError();
 }

@@ -158,7 +158,7 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = tiles.alloc(((size_t)dev::compact_tiles(E0) + 2) * sizeof(uint32_t))) ||
         (rc = xs.alloc(Eb + sizeof(uint32_t))) || (rc = ctr.alloc(sizeof(DevCounters))) ||
         (rc = ctr_scratch.alloc(sizeof(DevCounters))) ||
-        (rc = feat_acc.alloc(KT_FEAT_ACC * sizeof(unsigned long long))) ||
+        (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))) ||
         (rc = feat.alloc(KT_MLP_IN * sizeof(float))))
         return rc;
     KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
@@ -506,6 +506,11 @@ void Plan::finish_stats() {
     stats.recount_edges = h_ctr->recount_edges;
     stats.delta_elems = h_ctr->delta_elems;
     stats.delta_triangles = h_ctr->delta_tris;
+    // DELTA's algorithmic bytes: candidates and longer-list windows read,
+    // and per destroyed triangle two slot->entry reads, two flag reads and
+    // two support read-modify-writes
+    stats.balg_delta_bytes = 4.0 * (double)h_ctr->delta_elems + 4.0 * (double)h_ctr->delta_win +
+                             32.0 * (double)h_ctr->delta_tris;
     stats.balg_count_bytes = 4.0 * (double)stats.merge_elems + 24.0 * (double)stats.recount_edges +
                              12.0 * (double)stats.live_in_passes;
 }
@@ -613,12 +618,15 @@ int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
     uint32_t* dl_p = dl.as<uint32_t>();
     const uint32_t* nd_dev = &ctr.as<DevCounters>()->ndel;
     if (pending && cfg.kernel == KT_COUNT_DELTA) {
+        KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
+                                      dst[cur].as<uint32_t>(), sptr.as<uint32_t>(), dws,
+                                      ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c0, stream));
-        KT_HIP_TRY(dev::delta_update(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
-                                     dst[cur].as<uint32_t>(), sptr.as<uint32_t>(), snbr.as<uint32_t>(),
-                                     seid.as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
-                                     ctr.as<DevCounters>(), stream));
+        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
+                                    seid.as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
+                                    ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+        last_pass_delta = true;
         KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
                                          snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                          stream));
@@ -637,6 +645,7 @@ int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
         KT_HIP_TRY(dev::count_supports(a, directed, tiled, rcfg.kernel, rcfg.block, stream));
         KT_HIP_TRY(hipEventRecord(ev_c1, stream));
         stats.live_in_passes += live;
+        last_pass_delta = false;
     }
     pending = 0;
     return KT_OK;
@@ -670,10 +679,12 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
                 KT_HIP_TRY(hipMemcpyAsync(sup[cur].p, sup_snap.p, (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
             first = false;
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
-            KT_HIP_TRY(dev::delta_update(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
-                                         dst[cur].as<uint32_t>(), sptr.as<uint32_t>(),
-                                         snbr.as<uint32_t>(), seid.as<uint32_t>(),
-                                         sup[cur].as<uint32_t>(), dws, ctr.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
+                                          dst[cur].as<uint32_t>(), sptr.as<uint32_t>(), dws,
+                                          ctr.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
+                                        snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                        sup[cur].as<uint32_t>(), dws, ctr.as<DevCounters>(), stream));
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
         }
     }
@@ -754,7 +765,9 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         } else if (counted) {
             KT_HIP_TRY(hipEventElapsedTime(&count_ms, ev_c0, ev_c1));
         }
-        if (counted) {
+        if (counted && pick != KT_PICK_SWEEP && last_pass_delta) {
+            stats.ms_delta += count_ms;
+        } else if (counted) {
             stats.ms_count += count_ms;
             stats.count_launches++;
         }

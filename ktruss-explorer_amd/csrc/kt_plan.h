@@ -76,6 +76,7 @@ struct Plan {
     DBuf sptr, snbr0, snbr, seid, sa, sb, dl, sup_snap, dscratch;
     dev::DeltaScratch dws{};
     bool delta_ready = false;
+    bool last_pass_delta = false;  // the last support pass was a DELTA update
     uint32_t pending = 0;
     bool delta_capable() const;
 

@@ -97,7 +97,8 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_count", ctypes.c_double),
                 ("count_launches", ctypes.c_uint64), ("balg_count_bytes", ctypes.c_double),
                 ("delta_passes", ctypes.c_uint64), ("delta_edges", ctypes.c_uint64),
-                ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64)]
+                ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64),
+                ("ms_delta", ctypes.c_double), ("balg_delta_bytes", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -121,7 +122,8 @@ class BatchStats(ctypes.Structure):
                 ("k_max", ctypes.c_uint32), ("pad", ctypes.c_uint32),
                 ("ms_total", ctypes.c_double), ("balg_count_bytes", ctypes.c_double),
                 ("ms_count", ctypes.c_double), ("count_launches", ctypes.c_uint64),
-                ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64)]
+                ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64),
+                ("ms_delta", ctypes.c_double), ("balg_delta_bytes", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}

@@ -11,13 +11,14 @@ a sparse skewed graph to dense cores.  This module
               by the engine's own forward/backward/Adam kernels) to predict
               each candidate's log cost relative to the others;
   evaluate()  decomposes a held-out graph with the fixed default variant, with
-              the model picking per phase, and in SWEEP mode (the exhaustive
-              per-phase judge), and reports speed-up, regret and parity.
+              the model picking per phase, with single candidates for the whole
+              run, and in SWEEP mode (the exhaustive per-phase judge), and
+              reports speed-up, regret and parity.
 
 Command line:
   python learn.py collect OUT.npz [--graphs N] [--seed S] [--max-scale X]
-  python learn.py train DATA.npz MODEL.npz [--epochs E]
-  python learn.py eval MODEL.npz [--scale 20]
+  python learn.py train DATA.npz [DATA.npz ...] [--out MODEL.npz] [--epochs E]
+  python learn.py eval [--model MODEL.npz] [--scale 20] [--sweep] [--singles I ...]
 """
 import argparse
 import json
@@ -181,33 +182,59 @@ def load_model(path=MODEL_PATH):
     return m, configs
 
 
-def evaluate(graph, model, configs):
-    """Fixed default vs model-picked vs exhaustive per-phase best."""
-    out = {}
+def evaluate(graph, model, configs, sweep=False, singles=()):
+    """The fixed default variant, the model's per-phase pick, each candidate
+    in `singles` used alone for the whole run, and (sweep) the exhaustive
+    per-phase best; wall-clock milliseconds of full decompositions."""
+    def timed(plan):
+        K.device_synchronize()
+        t0 = time.perf_counter()
+        st = plan.decompose()
+        K.device_synchronize()
+        return 1e3 * (time.perf_counter() - t0), st
+
+    out = {"m": graph.m}
     p = K.Plan(graph, K.default_variant())
-    st = p.decompose()
+    out["default_ms"], st = timed(p)
     tau_default = p.truss_numbers()
-    out["default_ms"] = st["ms_total"]
-    out["default_count_ms"] = st["ms_count"]
+    out["k_max"] = st["k_max"]
+    del p
+    p = K.Plan(graph, K.default_variant())
     p.set_candidates(configs)
     p.set_pick(K.PICK_MODEL, model)
-    st = p.decompose()
-    out["model_ms"] = st["ms_total"]
-    out["model_count_ms"] = st["ms_count"]
+    timed(p)  # warm
+    out["model_ms"], st = timed(p)
+    out["model_support_ms"] = st["ms_count"] + st["ms_delta"]
     out["model_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
     ph = p.phase_log()
     out["model_picks"] = np.bincount(ph[:, 2].astype(int), minlength=len(configs)).tolist()
-    p.set_pick(K.PICK_SWEEP)
-    p.decompose()
-    out["sweep_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
-    ph = p.phase_log()
-    ms = ph[:, 20:20 + len(configs)]
-    out["exhaustive_count_ms"] = float(ms.min(1).sum())
-    out["best_single_count_ms"] = float(ms.sum(0).min())
-    out["best_single"] = int(ms.sum(0).argmin())
+    singles_ms = {}
+    for c in singles:
+        cfg = configs[c]
+        v = K.make_variant(count_kernel=cfg.count_kernel, block_size=cfg.block_size,
+                           cap_lds=cfg.cap_lds, chunk=cfg.chunk)
+        q = K.Plan(graph, v)
+        timed(q)
+        ms, _ = timed(q)
+        singles_ms[cfg.describe()] = ms
+        assert np.array_equal(q.truss_numbers(), tau_default), cfg.describe()
+        del q
+    out["singles_ms"] = singles_ms
     out["speedup_vs_default"] = out["default_ms"] / out["model_ms"]
-    out["regret_vs_exhaustive"] = out["model_count_ms"] / out["exhaustive_count_ms"] - 1
-    out["regret_vs_best_single"] = out["model_count_ms"] / out["best_single_count_ms"] - 1
+    if singles_ms:
+        best = min(singles_ms.values())
+        out["best_single_ms"] = best
+        out["regret_vs_best_single"] = out["model_ms"] / best - 1
+    if sweep:
+        p.set_pick(K.PICK_SWEEP)
+        p.decompose()
+        out["sweep_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
+        ph = p.phase_log()
+        ms = ph[:, 20:20 + len(configs)]
+        out["exhaustive_support_ms"] = float(ms.min(1).sum())
+        out["best_single_support_ms"] = float(ms.sum(0).min())
+        out["best_single_candidate"] = configs[int(ms.sum(0).argmin())].describe()
+        out["regret_vs_exhaustive"] = out["model_support_ms"] / out["exhaustive_support_ms"] - 1
     return out
 
 
@@ -227,6 +254,8 @@ def main():
     e = sub.add_parser("eval")
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--scale", type=int, default=20)
+    e.add_argument("--sweep", action="store_true")
+    e.add_argument("--singles", type=int, nargs="*", default=[])
     args = ap.parse_args()
     K.init(0)
     if args.cmd == "collect":
@@ -254,7 +283,9 @@ def main():
         model, configs = load_model(args.model)
         n, s, d = K.generate_rmat(args.scale, 16, seed=1)
         g = K.Graph(n, s, d, pairs=True)
-        print(json.dumps(evaluate(g, model, configs)))
+        res = evaluate(g, model, configs, args.sweep, args.singles)
+        res["scale"] = args.scale
+        print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

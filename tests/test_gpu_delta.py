@@ -70,6 +70,8 @@ def test_delta_rmat12_and_repeat():
         assert np.array_equal(p.truss_numbers(), tau_o[: len(s)])
         assert st["delta_triangles"] == st_o["triangles"]
         assert [t[:3] for t in p.trace()] == [tuple(x[:3]) for x in tr_o]
+        assert st["ms_delta"] > 0 and st["balg_delta_bytes"] >= 4 * st["delta_elems"]
+        assert st["count_launches"] == 1  # the first full count only
 
 
 def test_delta_sweep_and_model_pick(small_graphs):
