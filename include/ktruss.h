@@ -253,6 +253,12 @@ void kt_mlp_destroy(kt_mlp* m);
 int kt_mlp_num_params(const kt_mlp* m, uint32_t* n);
 int kt_mlp_get_params(const kt_mlp* m, float* out, uint32_t n);
 int kt_mlp_set_params(kt_mlp* m, const float* in, uint32_t n);
+/* Inference inputs (kt_mlp_forward and the picks of kt_plan_set_pick) are
+ * clamped to [lo, hi] per feature -- set it to the training features' range so
+ * the network never extrapolates; NULL bounds mean unbounded.  Training does
+ * not clamp. */
+int kt_mlp_set_input_range(kt_mlp* m, const float* lo, const float* hi);
+int kt_mlp_get_input_range(const kt_mlp* m, float* lo, float* hi);
 int kt_mlp_forward(kt_mlp* m, const float* x, uint32_t batch, float* y);
 /* Loss and its gradient (deterministic); mask and w may be NULL. */
 int kt_mlp_loss_grad(kt_mlp* m, const float* x, const float* t, const float* mask, const float* w,

@@ -203,8 +203,9 @@ int kt_plan_set_pick(kt_plan* p, uint32_t mode, const kt_mlp* model) {
     if (mode == KT_PICK_MODEL && !model) return kt::fail(KT_ERR_ARG, "model is NULL");
     KT_GUARD({
         if (mode == KT_PICK_MODEL)
-            return p->impl.set_pick(mode, model->impl.P, model->impl.total, model->impl.C);
-        return p->impl.set_pick(mode, nullptr, 0, 0);
+            return p->impl.set_pick(mode, model->impl.P, model->impl.total, model->impl.C,
+                                    model->impl.R);
+        return p->impl.set_pick(mode, nullptr, 0, 0, nullptr);
     })
 }
 
@@ -253,6 +254,16 @@ int kt_mlp_get_params(const kt_mlp* m, float* out, uint32_t n) {
 int kt_mlp_set_params(kt_mlp* m, const float* in, uint32_t n) {
     if (!m || !in) return kt::fail(KT_ERR_ARG, "NULL argument");
     KT_GUARD(return m->impl.set(in, n);)
+}
+
+int kt_mlp_set_input_range(kt_mlp* m, const float* lo, const float* hi) {
+    if (!m) return kt::fail(KT_ERR_ARG, "model is NULL");
+    KT_GUARD(return m->impl.set_range(lo, hi);)
+}
+
+int kt_mlp_get_input_range(const kt_mlp* m, float* lo, float* hi) {
+    if (!m) return kt::fail(KT_ERR_ARG, "model is NULL");
+    KT_GUARD(return m->impl.get_range(lo, hi);)
 }
 
 int kt_mlp_forward(kt_mlp* m, const float* x, uint32_t batch, float* y) {

@@ -215,6 +215,22 @@ __device__ __forceinline__ uint32_t lower_bound_u(const uint32_t* a, uint32_t lo
     return lo + ((a[lo] & KT_VMASK) < x ? 1u : 0u);
 }
 
+// Lower bounds of x0 and x1 in [lo, lo + len), interleaved (same trip count).
+__device__ __forceinline__ void lower_bounds_2(const uint32_t* a, uint32_t lo, uint32_t len,
+                                               uint32_t x0, uint32_t x1, uint32_t& r0,
+                                               uint32_t& r1) {
+    uint32_t b0 = lo, b1 = lo;
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        const uint32_t v0 = a[b0 + half] & KT_VMASK, v1 = a[b1 + half] & KT_VMASK;
+        if (v0 < x0) b0 += half;
+        if (v1 < x1) b1 += half;
+        len -= half;
+    }
+    r0 = b0 + ((a[b0] & KT_VMASK) < x0 ? 1u : 0u);
+    r1 = b1 + ((a[b1] & KT_VMASK) < x1 ? 1u : 0u);
+}
+
 __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
@@ -249,11 +265,13 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         const bool cand = !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
         if (!cm) continue;
-        // candidates ascend with the lane: the window of the longer list
+        // candidates ascend with the lane: the window of the longer list is
+        // [lower bound of xmin, lower bound of xmax + 1), two wave-uniform
+        // searches run in the same loop so their loads overlap
         const uint32_t xmin = __builtin_amdgcn_readlane(x, (uint32_t)__ffsll((long long)cm) - 1);
         const uint32_t xmax = __builtin_amdgcn_readlane(x, 63u - (uint32_t)__builtin_clzll(cm));
-        const uint32_t lo = lower_bound_u(snbr, m.z, m.w, xmin);
-        const uint32_t hi = lower_bound_u(snbr, lo, m.z + m.w - lo, xmax + 1);
+        uint32_t lo, hi;
+        lower_bounds_2(snbr, m.z, m.w, xmin, xmax + 1, lo, hi);
         const uint32_t wl = hi - lo;
         if (wl == 0) continue;
         acc_win += wl;

@@ -80,13 +80,20 @@ __global__ __launch_bounds__(kFeatThreads) void k_feat_rows(const uint32_t* __re
     block_sum_store<6>(v, true, part + (size_t)blockIdx.x * KT_FEAT_ACC);
 }
 
+// Entries are sampled with a fixed stride (every entry up to kFeatSample
+// entries; the histogram is normalised and the live count rescaled), so the
+// features cost a bounded pass however large the graph.
+constexpr uint32_t kFeatSample = 1u << 22;
+
 __global__ __launch_bounds__(kFeatThreads) void k_feat_support(const uint32_t* __restrict__ dst,
                                                               const uint32_t* __restrict__ sup,
-                                                              uint32_t E, uint32_t k, int sup_valid,
+                                                              uint32_t E, uint32_t stride, uint32_t k,
+                                                              int sup_valid,
                                                               unsigned long long* part) {
-    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries
+    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries (sampled)
     const float denom = (float)(k > 2 ? k - 2 : 1);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x * stride;
+    for (uint64_t e = (uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * stride; e < E; e += step) {
         if (dst[e] & (KT_DEAD | KT_DYING)) continue;
         v[5] += 1;
         if (sup_valid) {
@@ -100,8 +107,8 @@ __global__ __launch_bounds__(kFeatThreads) void k_feat_support(const uint32_t* _
 
 __device__ float log2p1(float x) { return log2f(1.f + x); }
 
-__global__ void k_feat_finish(const unsigned long long* part, uint32_t E0, uint32_t k,
-                              int sup_valid, float* feat) {
+__global__ void k_feat_finish(const unsigned long long* part, uint32_t E0, uint32_t stride,
+                              uint32_t k, int sup_valid, float* feat) {
     __shared__ unsigned long long acc[KT_FEAT_ACC];
     if (threadIdx.x < KT_FEAT_ACC) {
         const int j = threadIdx.x;
@@ -123,7 +130,7 @@ __global__ void k_feat_finish(const unsigned long long* part, uint32_t E0, uint3
     feat[4] = mean > 0 ? sqrtf(s2 / rows) / mean / 8.f : 0.f;
     feat[5] = s1 > 0 ? (float)acc[4] / s1 : 0.f;
     feat[6] = s1 > 0 ? (float)acc[5] / s1 : 0.f;
-    feat[7] = E0 ? (float)acc[11] / (float)E0 : 0.f;  // live fraction
+    feat[7] = E0 ? fminf(1.f, (float)acc[11] * (float)stride / (float)E0) : 0.f;  // live fraction
     feat[8] = log2f((float)k) / 10.f;
     feat[9] = s1 > 0 ? s2 / s1 / 1024.f : 0.f;
     float tot = 0;
@@ -153,13 +160,20 @@ __device__ void mlp_forward_wave(const float* __restrict__ P, int C, const float
     __syncthreads();
 }
 
+// Inference inputs are clamped to the model's input range R = [lo[16], hi[16]]
+// (the range of its training features), so the network never extrapolates.
+__device__ __forceinline__ float clamp_in(float x, const float* R, int i) {
+    return fminf(fmaxf(x, R[i]), R[kIn + i]);
+}
+
 __global__ __launch_bounds__(64) void k_mlp_forward(const float* __restrict__ P, int C,
+                                                    const float* __restrict__ R,
                                                     const float* __restrict__ X, uint32_t batch,
                                                     float* __restrict__ Y) {
     __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
     const int lane = threadIdx.x;
     for (uint32_t s = blockIdx.x; s < batch; s += gridDim.x) {
-        if (lane < kIn) s_x[lane] = X[(size_t)s * kIn + lane];
+        if (lane < kIn) s_x[lane] = clamp_in(X[(size_t)s * kIn + lane], R, lane);
         __syncthreads();
         mlp_forward_wave(P, C, s_x, s_h1, s_h2, s_o, lane);
         if (lane < C) Y[(size_t)s * C + lane] = s_o[lane];
@@ -251,11 +265,12 @@ __global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float
 // Inference inside the peel loop: forward on the phase features, argmin of
 // the predicted costs written to *choice.
 __global__ __launch_bounds__(64) void k_mlp_pick(const float* __restrict__ P, int C,
+                                                 const float* __restrict__ R,
                                                  const float* __restrict__ feat,
                                                  unsigned int* choice) {
     __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
     const int lane = threadIdx.x;
-    if (lane < kIn) s_x[lane] = feat[lane];
+    if (lane < kIn) s_x[lane] = clamp_in(feat[lane], R, lane);
     __syncthreads();
     mlp_forward_wave(P, C, s_x, s_h1, s_h2, s_o, lane);
     if (lane == 0) {
@@ -278,18 +293,20 @@ hipError_t phase_features(const uint32_t* ptr, const uint32_t* dst, const uint32
     hipLaunchKernelGGL(k_feat_rows, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, ptr, n, acc);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_feat_support, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, dst, sup, E, k,
-                       sup_valid ? 1 : 0, acc);
+    const uint32_t stride = E > kFeatSample ? (E + kFeatSample - 1) / kFeatSample : 1u;
+    hipLaunchKernelGGL(k_feat_support, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, dst, sup, E, stride,
+                       k, sup_valid ? 1 : 0, acc);
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    hipLaunchKernelGGL(k_feat_finish, dim3(1), dim3(64), 0, s, acc, E0, k, sup_valid ? 1 : 0, feat);
+    hipLaunchKernelGGL(k_feat_finish, dim3(1), dim3(64), 0, s, acc, E0, stride, k, sup_valid ? 1 : 0,
+                       feat);
     return hipGetLastError();
 }
 
-hipError_t mlp_forward(const float* P, uint32_t C, const float* X, uint32_t batch, float* Y,
-                       hipStream_t s) {
+hipError_t mlp_forward(const float* P, uint32_t C, const float* R, const float* X, uint32_t batch,
+                       float* Y, hipStream_t s) {
     if (!batch) return hipSuccess;
     const uint32_t gr = std::min<uint32_t>(batch, 4096);
-    hipLaunchKernelGGL(k_mlp_forward, dim3(gr), dim3(64), 0, s, P, (int)C, X, batch, Y);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(gr), dim3(64), 0, s, P, (int)C, R, X, batch, Y);
     return hipGetLastError();
 }
 
@@ -315,9 +332,9 @@ hipError_t adam_step(float* P, const float* G, float* m1, float* m2, uint32_t to
     return hipGetLastError();
 }
 
-hipError_t mlp_pick(const float* P, uint32_t C, const float* feat, unsigned int* choice,
+hipError_t mlp_pick(const float* P, uint32_t C, const float* R, const float* feat, unsigned int* choice,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_pick, dim3(1), dim3(64), 0, s, P, (int)C, feat, choice);
+    hipLaunchKernelGGL(k_mlp_pick, dim3(1), dim3(64), 0, s, P, (int)C, R, feat, choice);
     return hipGetLastError();
 }
 

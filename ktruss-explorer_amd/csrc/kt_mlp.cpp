@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -19,7 +20,7 @@ namespace kt {
 
 Mlp::~Mlp() {
     for (void* p : {(void*)P, (void*)G, (void*)m1, (void*)m2, (void*)part, (void*)part_loss,
-                    (void*)X, (void*)T, (void*)M, (void*)W, (void*)Y})
+                    (void*)X, (void*)T, (void*)M, (void*)W, (void*)Y, (void*)R})
         if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -35,6 +36,15 @@ int Mlp::create(uint32_t n_out, uint64_t seed) {
     KT_HIP_TRY(hipMalloc((void**)&m2, total * 4));
     KT_HIP_TRY(hipMalloc((void**)&part, (size_t)kParts * total * 4));
     KT_HIP_TRY(hipMalloc((void**)&part_loss, kParts * 4));
+    KT_HIP_TRY(hipMalloc((void**)&R, 2 * KT_MLP_IN * 4));
+    {
+        std::vector<float> r(2 * KT_MLP_IN);
+        for (int i = 0; i < KT_MLP_IN; ++i) {
+            r[i] = -FLT_MAX;
+            r[KT_MLP_IN + i] = FLT_MAX;
+        }
+        KT_HIP_TRY(hipMemcpy(R, r.data(), r.size() * 4, hipMemcpyHostToDevice));
+    }
     // He-uniform weights from a seeded mt19937_64, zero biases
     std::vector<float> h(total, 0.f);
     std::mt19937_64 rng(seed);
@@ -79,7 +89,7 @@ int Mlp::forward(const float* x, uint32_t batch, float* y) {
     int rc = ensure_batch(batch);
     if (rc) return rc;
     KT_HIP_TRY(hipMemcpyAsync(X, x, (size_t)batch * KT_MLP_IN * 4, hipMemcpyHostToDevice, stream));
-    KT_HIP_TRY(dev::mlp_forward(P, C, X, batch, Y, stream));
+    KT_HIP_TRY(dev::mlp_forward(P, C, R, X, batch, Y, stream));
     KT_HIP_TRY(hipMemcpyAsync(y, Y, (size_t)batch * C * 4, hipMemcpyDeviceToHost, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
     return KT_OK;
@@ -176,6 +186,27 @@ int Mlp::set(const float* in, uint32_t n) {
     if (n != total) return fail(KT_ERR_CAPACITY, "model has " + std::to_string(total) + " parameters");
     KT_HIP_TRY(hipMemcpy(P, in, total * 4, hipMemcpyHostToDevice));
     return reset_adam();
+}
+
+int Mlp::set_range(const float* lo, const float* hi) {
+    std::vector<float> r(2 * KT_MLP_IN);
+    for (int i = 0; i < KT_MLP_IN; ++i) {
+        r[i] = lo ? lo[i] : -FLT_MAX;
+        r[KT_MLP_IN + i] = hi ? hi[i] : FLT_MAX;
+        if (!(r[i] <= r[KT_MLP_IN + i])) return fail(KT_ERR_ARG, "input range has lo > hi");
+    }
+    KT_HIP_TRY(hipMemcpy(R, r.data(), r.size() * 4, hipMemcpyHostToDevice));
+    return KT_OK;
+}
+
+int Mlp::get_range(float* lo, float* hi) const {
+    std::vector<float> r(2 * KT_MLP_IN);
+    KT_HIP_TRY(hipMemcpy(r.data(), R, r.size() * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < KT_MLP_IN; ++i) {
+        if (lo) lo[i] = r[i];
+        if (hi) hi[i] = r[KT_MLP_IN + i];
+    }
+    return KT_OK;
 }
 
 }  // namespace kt

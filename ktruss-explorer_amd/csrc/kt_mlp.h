@@ -15,6 +15,7 @@ struct Mlp {
     float *P = nullptr, *G = nullptr, *m1 = nullptr, *m2 = nullptr;
     float *part = nullptr, *part_loss = nullptr;
     float *X = nullptr, *T = nullptr, *M = nullptr, *W = nullptr, *Y = nullptr;
+    float* R = nullptr;  // input range: lo[KT_MLP_IN], hi[KT_MLP_IN]
     hipStream_t stream = nullptr;
 
     ~Mlp();
@@ -26,6 +27,8 @@ struct Mlp {
               uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float* loss_out);
     int get(float* out, uint32_t n) const;
     int set(const float* in, uint32_t n);
+    int set_range(const float* lo, const float* hi);
+    int get_range(float* lo, float* hi) const;
 
    private:
     int reset_adam();

@@ -267,16 +267,21 @@ int Plan::set_candidates(const kt_count_config* c, uint32_t n) {
     return KT_OK;
 }
 
-int Plan::set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C) {
+int Plan::set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C,
+                   const float* range) {
     if (mode > KT_PICK_MODEL) return fail(KT_ERR_ARG, "unknown pick mode");
     if (mode != KT_PICK_FIXED && cands.empty()) return fail(KT_ERR_STATE, "set candidates first");
     if (mode == KT_PICK_MODEL) {
-        if (!params || C != cands.size())
+        if (!params || !range || C != cands.size())
             return fail(KT_ERR_ARG, "the model must have one output per candidate");
-        int rc = model.alloc((size_t)nparams * sizeof(float));
+        // parameters, then the input range (2 * KT_MLP_IN floats)
+        int rc = model.alloc(((size_t)nparams + 2 * KT_MLP_IN) * sizeof(float));
         if (rc) return rc;
         KT_HIP_TRY(hipMemcpy(model.p, params, (size_t)nparams * sizeof(float), hipMemcpyDeviceToDevice));
+        KT_HIP_TRY(hipMemcpy(model.as<float>() + nparams, range, 2 * KT_MLP_IN * sizeof(float),
+                             hipMemcpyDeviceToDevice));
         model_C = C;
+        model_n = nparams;
     }
     pick = mode;
     return KT_OK;
@@ -304,7 +309,7 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
 
 // Phase features of the surviving graph (supports used only when `exact`)
 // and, in MODEL mode, the cost model's pick among the candidates.
-int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen) {
+int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer) {
     cfg = fixed_cfg;
     chosen = 0;
     if (pick == KT_PICK_FIXED) {  // no pick to make: the phase log gets no features
@@ -317,10 +322,10 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen) {
     cfg = fixed_cfg;
     chosen = 0;
     if (pick == KT_PICK_MODEL)
-        KT_HIP_TRY(dev::mlp_pick(model.as<float>(), model_C, feat.as<float>(),
-                                 &ctr.as<DevCounters>()->choice, stream));
+        KT_HIP_TRY(dev::mlp_pick(model.as<float>(), model_C, model.as<float>() + model_n,
+                                 feat.as<float>(), &ctr.as<DevCounters>()->choice, stream));
     KT_HIP_TRY(hipMemcpyAsync(h_feat, feat.p, KT_MLP_IN * sizeof(float), hipMemcpyDeviceToHost, stream));
-    if (pick == KT_PICK_MODEL) {
+    if (pick == KT_PICK_MODEL && !defer) {
         const int rc = sync_counters();
         if (rc) return rc;
         chosen = std::min<uint32_t>(h_ctr->choice, (uint32_t)cands.size() - 1);
@@ -724,7 +729,15 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     rec.fill(-1.f);
     CountCfg cfg = fixed_cfg;
     uint32_t chosen = 0;
-    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen))) return rc;
+    // the model's choice is read with the next counter read-back (the JUMP's
+    // or, when deletions are pending, right before the support pass)
+    bool unresolved = pick == KT_PICK_MODEL;
+    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen, true))) return rc;
+    auto resolve_choice = [&]() {
+        chosen = std::min<uint32_t>(h_ctr->choice, nc - 1);
+        cfg = cands[chosen];
+        unresolved = false;
+    };
     std::vector<double> cand_ms(nc, 0.0);
     double phase_ms = 0.0;
     do {
@@ -733,6 +746,10 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         float dummy = 0;
         // the pending count is ctr->ndel of the last mark: reset only afterwards
         if (pending || !(iter == 0 && reuse)) {
+            if (unresolved) {
+                if ((rc = sync_counters())) return rc;
+                resolve_choice();
+            }
             if (pick == KT_PICK_SWEEP) rc = sweep_pass(k, cand_ms, dummy);
             else rc = support_pass(cfg, k, dummy);
             if (rc) return rc;
@@ -745,6 +762,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             KT_HIP_TRY(dev::min_live_support(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E,
                                              ctr.as<DevCounters>(), stream));
             if ((rc = sync_counters())) return rc;
+            if (unresolved) resolve_choice();
             const uint32_t s = h_ctr->min_sup;
             if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
         }
@@ -752,6 +770,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
                                    ctr.as<DevCounters>(), stream));
         if ((rc = sync_counters())) return rc;
+        if (unresolved) resolve_choice();
         float count_ms = 0;
         if (counted && pick == KT_PICK_SWEEP) {
             float best = 0;

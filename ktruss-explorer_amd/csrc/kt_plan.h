@@ -63,7 +63,7 @@ struct Plan {
     std::vector<CountCfg> cands;
     uint32_t pick = KT_PICK_FIXED;
     DBuf model;          // cost-model parameters (KT_PICK_MODEL)
-    uint32_t model_C = 0;
+    uint32_t model_C = 0, model_n = 0;
     DBuf feat_acc, feat, ctr_scratch;
     float* h_feat = nullptr;
     hipEvent_t ev_sw[2 * KT_MAX_CANDIDATES] = {};
@@ -82,7 +82,8 @@ struct Plan {
 
     int resolve(const kt_count_config& in, CountCfg& out) const;
     int set_candidates(const kt_count_config* c, uint32_t n);
-    int set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C);
+    int set_pick(uint32_t mode, const float* params, uint32_t nparams, uint32_t C,
+                 const float* range);
 
     ~Plan();
     int create(const kt_graph* g, const kt_variant* v);
@@ -97,7 +98,9 @@ struct Plan {
     int ensure_delta();
     int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
-    int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen);
+    // defer: leave the model's choice on the device (read with the next
+    // counter read-back) instead of synchronising for it now
+    int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer = false);
     int compact_now();
     int sync_counters();
     int set_counter(size_t offset, uint32_t value);

@@ -43,6 +43,7 @@ EXPORTS = [
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
     "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
     "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
+    "kt_mlp_set_input_range", "kt_mlp_get_input_range",
     "kt_batch_create", "kt_batch_destroy", "kt_batch_info", "kt_batch_set_tuning",
     "kt_batch_get_tuning", "kt_batch_autotune", "kt_batch_tune_log", "kt_batch_decompose",
     "kt_batch_truss_numbers", "kt_batch_graph_info", "kt_batch_set_timing",
@@ -192,6 +193,8 @@ def lib():
         L.kt_mlp_train.argtypes = [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64,
                                    _f32p]
+        L.kt_mlp_set_input_range.argtypes = [ctypes.c_void_p, _f32p, _f32p]
+        L.kt_mlp_get_input_range.argtypes = [ctypes.c_void_p, _f32p, _f32p]
         _vp = ctypes.c_void_p
         L.kt_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(Variant),
                                       ctypes.POINTER(_vp)]
@@ -501,6 +504,19 @@ class Mlp:
     def set_params(self, p):
         p, pp = _f32(p)
         _check(lib().kt_mlp_set_params(self._h, pp, len(p)), "kt_mlp_set_params")
+
+    def set_input_range(self, lo, hi):
+        """Inference inputs are clamped to [lo, hi] per feature."""
+        lo, lp = _f32(lo)
+        hi, hp = _f32(hi)
+        _check(lib().kt_mlp_set_input_range(self._h, lp, hp), "kt_mlp_set_input_range")
+
+    def get_input_range(self):
+        lo = np.zeros(MLP_IN, np.float32)
+        hi = np.zeros(MLP_IN, np.float32)
+        _check(lib().kt_mlp_get_input_range(self._h, lo.ctypes.data_as(_f32p), hi.ctypes.data_as(_f32p)),
+               "kt_mlp_get_input_range")
+        return lo, hi
 
     def forward(self, x):
         x, xp = _f32(x)

@@ -143,11 +143,15 @@ def weights(costs):
 
 
 def train(feats, costs, epochs=300, batch=256, lr=2e-3, seed=1):
+    """Fit the cost model; its inference inputs are then clamped to the range
+    of the training features, so graphs beyond that range (larger, deeper)
+    are priced as the most extreme graphs it was trained on."""
     C = costs.shape[1]
     m = K.Mlp(C, seed=seed)
     t = targets(costs)
     w = weights(costs)
     loss = m.train(feats, t, None, w, epochs=epochs, batch=min(batch, len(feats)), lr=lr, seed=seed)
+    m.set_input_range(feats.min(0), feats.max(0))
     return m, loss
 
 
@@ -168,7 +172,8 @@ def picks_regret(model, feats, costs):
 def save_model(model, path=MODEL_PATH, info=None):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     cands = np.array([[K.COUNT_KERNELS[k], b, c] for k, b, c in CANDIDATES], np.uint32)
-    np.savez(path, params=model.get_params(), candidates=cands,
+    lo, hi = model.get_input_range()
+    np.savez(path, params=model.get_params(), candidates=cands, lo=lo, hi=hi,
              info=np.array(json.dumps(info or {})))
 
 
@@ -176,13 +181,17 @@ def load_model(path=MODEL_PATH):
     with np.load(path, allow_pickle=False) as z:
         params = z["params"]
         cands = z["candidates"]
+        lo = z["lo"] if "lo" in z.files else None
+        hi = z["hi"] if "hi" in z.files else None
     configs = [K.count_config(int(k), int(b), int(c)) for k, b, c in cands]
     m = K.Mlp(len(configs))
     m.set_params(params)
+    if lo is not None:
+        m.set_input_range(lo, hi)
     return m, configs
 
 
-def evaluate(graph, model, configs, sweep=False, singles=()):
+def evaluate(graph, model, configs, sweep=False, singles=(), default_ms=None):
     """The fixed default variant, the model's per-phase pick, each candidate
     in `singles` used alone for the whole run, and (sweep) the exhaustive
     per-phase best; wall-clock milliseconds of full decompositions."""
@@ -194,11 +203,21 @@ def evaluate(graph, model, configs, sweep=False, singles=()):
         return 1e3 * (time.perf_counter() - t0), st
 
     out = {"m": graph.m}
-    p = K.Plan(graph, K.default_variant())
-    out["default_ms"], st = timed(p)
-    tau_default = p.truss_numbers()
-    out["k_max"] = st["k_max"]
-    del p
+    if default_ms is None:
+        p = K.Plan(graph, K.default_variant())
+        out["default_ms"], st = timed(p)
+        tau_default = p.truss_numbers()
+        out["k_max"] = st["k_max"]
+        del p
+    else:  # a recorded time of the default variant; parity against the first single
+        out["default_ms"] = default_ms
+        out["default_recorded"] = True
+        cfg = configs[singles[0]]
+        q = K.Plan(graph, K.make_variant(count_kernel=cfg.count_kernel, block_size=cfg.block_size,
+                                         cap_lds=cfg.cap_lds, chunk=cfg.chunk))
+        q.decompose()
+        tau_default = q.truss_numbers()
+        del q
     p = K.Plan(graph, K.default_variant())
     p.set_candidates(configs)
     p.set_pick(K.PICK_MODEL, model)
@@ -256,6 +275,8 @@ def main():
     e.add_argument("--scale", type=int, default=20)
     e.add_argument("--sweep", action="store_true")
     e.add_argument("--singles", type=int, nargs="*", default=[])
+    e.add_argument("--default-ms", type=float, default=None,
+                   help="use this recorded default-variant time instead of running it")
     args = ap.parse_args()
     K.init(0)
     if args.cmd == "collect":
@@ -283,7 +304,7 @@ def main():
         model, configs = load_model(args.model)
         n, s, d = K.generate_rmat(args.scale, 16, seed=1)
         g = K.Graph(n, s, d, pairs=True)
-        res = evaluate(g, model, configs, args.sweep, args.singles)
+        res = evaluate(g, model, configs, args.sweep, args.singles, args.default_ms)
         res["scale"] = args.scale
         print(json.dumps(res), flush=True)
 

@@ -101,3 +101,20 @@ def test_candidate_validation():
     p.set_candidates([K.count_config("row", 256)])
     with pytest.raises(K.KtError):
         p.set_pick(K.PICK_MODEL, K.Mlp(3))  # one output per candidate
+
+
+def test_input_range_clamps_inference():
+    m = K.Mlp(3, seed=4)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-2, 2, size=(32, K.MLP_IN)).astype(np.float32)
+    lo = np.full(K.MLP_IN, -0.5, np.float32)
+    hi = np.full(K.MLP_IN, 0.75, np.float32)
+    y_clamped_input = m.forward(np.clip(x, lo, hi))
+    m.set_input_range(lo, hi)
+    got_lo, got_hi = m.get_input_range()
+    assert np.array_equal(got_lo, lo) and np.array_equal(got_hi, hi)
+    assert np.array_equal(m.forward(x), y_clamped_input)
+    # the reference network sees the clamped input too
+    import mlp_ref as R
+    ref = R.forward(m.get_params(), 3, np.clip(x, lo, hi))
+    assert np.allclose(m.forward(x), ref, rtol=1e-4, atol=1e-4)
