@@ -124,6 +124,7 @@ typedef struct kt_stats {
     double ms_delta;          /* summed device time of the DELTA update kernel     */
     double balg_delta_bytes;  /* DELTA algorithmic bytes: 4 (candidates + window
                                  entries read) + 32 per destroyed triangle          */
+    uint64_t delta_live;      /* DELTA candidates whose edge was still alive       */
 } kt_stats;
 
 typedef struct kt_graph kt_graph; /* validated host-side edge set            */

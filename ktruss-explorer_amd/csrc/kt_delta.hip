@@ -56,15 +56,18 @@ __device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, ui
 
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77), deferred: a deleted entry
 // is flagged KT_DYING and listed; it becomes KT_DEAD in k_finalize.
+// Items are list[i] for i < n (the live list), or the entries 0..n-1.
 __global__ void k_mark_dying(uint32_t* __restrict__ dst, const uint32_t* __restrict__ eid,
                              const uint32_t* __restrict__ sup, uint32_t* __restrict__ tau,
-                             uint32_t* __restrict__ dl, uint32_t E, uint32_t k, DevCounters* ctr) {
+                             uint32_t* __restrict__ dl, const uint32_t* __restrict__ list,
+                             uint32_t n, uint32_t k, DevCounters* ctr) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < E; base += gridDim.x * blockDim.x) {
-        const uint32_t e = base + threadIdx.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
         bool del = false;
-        uint32_t v = 0;
-        if (e < E) {
+        uint32_t v = 0, e = 0;
+        if (i < n) {
+            e = list ? list[i] : i;
             v = dst[e];
             del = !(v & (KT_DEAD | KT_DYING)) && sup[e] < k - 2;
         }
@@ -74,6 +77,29 @@ __global__ void k_mark_dying(uint32_t* __restrict__ dst, const uint32_t* __restr
             tau[eid[e]] = k - 1;
             dl[pos] = e;
         }
+    }
+}
+
+// Minimum support over the live list (the JUMP target); one atomic per
+// workgroup.
+__global__ void k_min_list(const uint32_t* __restrict__ list, uint32_t n,
+                           const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
+                           DevCounters* ctr) {
+    __shared__ uint32_t s_min[kBlock / kWave];
+    uint32_t m = 0xFFFFFFFFu;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t e = list ? list[i] : i;
+        if (!(dst[e] & (KT_DEAD | KT_DYING)) && sup[e] < m) m = sup[e];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off);
+        m = o < m ? o : m;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) s_min[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < kBlock / kWave; ++w) m = s_min[w] < m ? s_min[w] : m;
+        if (m != 0xFFFFFFFFu) atomicMin(&ctr->min_sup, m);
     }
 }
 
@@ -254,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         if (toff[i + half] <= t0) i += half;
         len -= half;
     }
-    unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0;
+    unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
     for (uint32_t t = t0; t < t1; ++t) {
         while (i + 1 < nd && toff[i + 1] <= t) ++i;
         const uint4 m = meta[i];
@@ -264,6 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         const uint32_t x = j < m.y ? snbr[j] : KT_DEAD;
         const bool cand = !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
+        acc_live += (uint32_t)__popcll(cm);
         if (!cm) continue;
         // candidates ascend with the lane: the window of the longer list is
         // [lower bound of xmin, lower bound of xmax + 1), two wave-uniform
@@ -274,7 +301,9 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         lower_bounds_2(snbr, m.z, m.w, xmin, xmax + 1, lo, hi);
         const uint32_t wl = hi - lo;
         if (wl == 0) continue;
-        acc_win += wl;
+        // entries read from the longer list: the staged window, or one probe
+        // per search step per live candidate when searched in place
+        acc_win += wl <= kWin ? wl : (uint64_t)__popcll(cm) * (32u - __builtin_clz(wl));
         uint32_t p;
         bool hit;
         if (wl <= kWin) {
@@ -312,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
     if (lane == 0) {
         atomicAdd(&ctr->delta_elems, acc_elems);
         if (acc_win) atomicAdd(&ctr->delta_win, acc_win);
+        if (acc_live) atomicAdd(&ctr->delta_live, acc_live);
         if (acc_tri) atomicAdd(&ctr->delta_tris, acc_tri);
     }
 }
@@ -334,15 +364,154 @@ uint32_t blocks_for(uint64_t items, uint32_t per_block) {
     return b ? (uint32_t)b : 1u;
 }
 
+// ---------------------------------------------------------------------------
+// Compaction of the DELTA state (stable, tiles of kTileC items, one 256-thread
+// workgroup per tile: count -> one-workgroup scan of the tile counts ->
+// scatter with in-tile ballot prefixes).
+//   live list   the mark pass walks a list of the entries still alive instead
+//               of every stored entry; items are list[i] (or i when the list
+//               is the identity) kept when the entry is neither dead nor dying
+//   adjacency   slots whose edge is dead leave the lists; the list starts are
+//               rebuilt from the slot prefix and every live entry's two slots
+//               (sa, sb) are found again from the kept slots
+// Entries never move, so supports, truss numbers and the oriented layout are
+// untouched.
+constexpr uint32_t kTileC = 2048;
+
+__device__ __forceinline__ bool item_kept(const uint32_t* list, uint32_t i, const uint32_t* dst,
+                                          const uint32_t* snbr, uint32_t* item) {
+    if (snbr) {  // adjacency slot i
+        *item = i;
+        return !(snbr[i] & KT_DEAD);
+    }
+    const uint32_t e = list ? list[i] : i;
+    *item = e;
+    return !(dst[e] & (KT_DEAD | KT_DYING));
+}
+
+__global__ __launch_bounds__(kBlock) void k_cmp_count(const uint32_t* __restrict__ list, uint32_t n,
+                                                      const uint32_t* __restrict__ dst,
+                                                      const uint32_t* __restrict__ snbr,
+                                                      uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t s_w[kBlock / kWave];
+    const uint32_t t = blockIdx.x;
+    uint32_t c = 0, item;
+    for (uint32_t i = t * kTileC + threadIdx.x; i < (t + 1) * kTileC && i < n; i += kBlock)
+        c += item_kept(list, i, dst, snbr, &item) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; ++w) sum += s_w[w];
+        tsum[t] = sum;
+    }
+}
+
+// Exclusive scan of nt tile counts in place by one workgroup; total at tsum[nt].
+__global__ __launch_bounds__(1024) void k_cmp_scan(uint32_t* tsum, uint32_t nt) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t per = (nt + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < b + per && i < nt; ++i) sum += tsum[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
+    for (uint32_t i = b; i < b + per && i < nt; ++i) {
+        const uint32_t c = tsum[i];
+        tsum[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) tsum[nt] = s_part[1023];
+}
+
+// Scatter of the kept items.  Live list: out[pos] = entry.  Adjacency:
+// out / out2 receive snbr / seid, and xs[i] = kept slots before slot i (for
+// every slot; xs[n] is set by the last tile).
+__global__ __launch_bounds__(kBlock) void k_cmp_scatter(const uint32_t* __restrict__ list, uint32_t n,
+                                                        const uint32_t* __restrict__ dst,
+                                                        const uint32_t* __restrict__ snbr,
+                                                        const uint32_t* __restrict__ seid,
+                                                        const uint32_t* __restrict__ tsum,
+                                                        uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ out2,
+                                                        uint32_t* __restrict__ xs) {
+    __shared__ uint32_t s_w[kBlock / kWave];
+    const uint32_t t = blockIdx.x;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    uint32_t run = tsum[t];
+    for (uint32_t r = 0; r < kTileC / kBlock; ++r) {
+        const uint32_t i = t * kTileC + r * kBlock + threadIdx.x;
+        uint32_t item = 0;
+        const bool keep = i < n && item_kept(list, i, dst, snbr, &item);
+        const uint64_t m = ballot(keep);
+        if (lane == 0) s_w[wid] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = run, round = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+            if (w < wid) before += s_w[w];
+            round += s_w[w];
+        }
+        const uint32_t pos = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (xs && i < n) xs[i] = pos;
+        if (keep) {
+            if (snbr) {
+                out[pos] = snbr[item];
+                out2[pos] = seid[item];
+            } else {
+                out[pos] = item;
+            }
+        }
+        run += round;
+        __syncthreads();
+    }
+    if (xs && t == gridDim.x - 1 && threadIdx.x == 0) xs[n] = tsum[gridDim.x];
+}
+
+__global__ void k_cmp_sptr(const uint32_t* __restrict__ sptr, const uint32_t* __restrict__ xs,
+                           uint32_t* __restrict__ sptr2, uint32_t n) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x <= n; x += gridDim.x * blockDim.x)
+        sptr2[x] = xs[sptr[x]];
+}
+
+// A kept slot j holds neighbour y of edge e = (u, v): it lies in u's list
+// when y == v.
+__global__ void k_cmp_slots(const uint32_t* __restrict__ snbr2, const uint32_t* __restrict__ seid2,
+                            uint32_t S2, const uint32_t* __restrict__ dst, uint32_t* __restrict__ sa,
+                            uint32_t* __restrict__ sb) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < S2; j += gridDim.x * blockDim.x) {
+        const uint32_t e = seid2[j];
+        if ((snbr2[j] & KT_VMASK) == (dst[e] & KT_IDMASK)) sa[e] = j;
+        else sb[e] = j;
+    }
+}
+
 }  // namespace
 
 namespace dev {
 
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
-                      uint32_t* dl, uint32_t E, uint32_t k, DevCounters* ctr, hipStream_t s) {
-    if (!E) return hipSuccess;
-    hipLaunchKernelGGL(k_mark_dying, dim3(blocks_for(E, kBlock)), dim3(kBlock), 0, s, dst, eid, sup,
-                       tau, dl, E, k, ctr);
+                      uint32_t* dl, const uint32_t* list, uint32_t n, uint32_t k, DevCounters* ctr,
+                      hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_mark_dying, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, dst, eid, sup,
+                       tau, dl, list, n, k, ctr);
+    return hipGetLastError();
+}
+
+hipError_t min_live_list(const uint32_t* list, uint32_t n, const uint32_t* dst, const uint32_t* sup,
+                         DevCounters* ctr, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const uint32_t b = blocks_for(n, kBlock);
+    hipLaunchKernelGGL(k_min_list, dim3(b < 2048 ? b : 2048), dim3(kBlock), 0, s, list, n, dst, sup,
+                       ctr);
     return hipGetLastError();
 }
 
@@ -392,6 +561,45 @@ hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t
     if (!nd_max) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nd_max, kBlock)), dim3(kBlock), 0, s, dl, nd_dev,
                        dst, snbr, sa, sb);
+    return hipGetLastError();
+}
+
+uint32_t compact_tiles_for(uint32_t n) { return (n + kTileC - 1) / kTileC; }
+
+hipError_t compact_live_list(const uint32_t* list, uint32_t n, const uint32_t* dst, uint32_t* out,
+                             uint32_t* tsum, hipStream_t s) {
+    const uint32_t nt = compact_tiles_for(n);
+    if (!nt) return hipSuccess;
+    hipLaunchKernelGGL(k_cmp_count, dim3(nt), dim3(kBlock), 0, s, list, n, dst, nullptr, tsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, s, tsum, nt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cmp_scatter, dim3(nt), dim3(kBlock), 0, s, list, n, dst, nullptr, nullptr,
+                       tsum, out, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t compact_adjacency(uint32_t nv, uint32_t S, const uint32_t* sptr, const uint32_t* snbr,
+                             const uint32_t* seid, uint32_t* sptr2, uint32_t* snbr2, uint32_t* seid2,
+                             uint32_t S2, const uint32_t* dst, uint32_t* sa, uint32_t* sb,
+                             uint32_t* tsum, uint32_t* xs, hipStream_t s) {
+    const uint32_t nt = compact_tiles_for(S);
+    if (!nt) return hipSuccess;
+    hipLaunchKernelGGL(k_cmp_count, dim3(nt), dim3(kBlock), 0, s, nullptr, S, nullptr, snbr, tsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, s, tsum, nt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cmp_scatter, dim3(nt), dim3(kBlock), 0, s, nullptr, S, nullptr, snbr, seid,
+                       tsum, snbr2, seid2, xs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cmp_sptr, dim3(blocks_for((uint64_t)nv + 1, kBlock)), dim3(kBlock), 0, s,
+                       sptr, xs, sptr2, nv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (S2)
+        hipLaunchKernelGGL(k_cmp_slots, dim3(blocks_for(S2, kBlock)), dim3(kBlock), 0, s, snbr2, seid2,
+                           S2, dst, sa, sb);
     return hipGetLastError();
 }
 

@@ -200,8 +200,13 @@ int Plan::reset() {
     KT_HIP_TRY(dev::iota_u32(eid[0].as<uint32_t>(), E0, stream));
     KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E0, stream));
     KT_HIP_TRY(dev::fill_u32(ctr.as<uint32_t>(), 0, sizeof(DevCounters) / 4, stream));
-    if (delta_ready)
-        KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, snbr.size, hipMemcpyDeviceToDevice, stream));
+    if (delta_ready) {
+        const int rc = restore_delta();
+        if (rc) return rc;
+    }
+    use_list = false;
+    lcur = 0;
+    L = E0;
     pending = 0;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     std::memset(h_ctr, 0, sizeof(DevCounters));
@@ -491,7 +496,7 @@ int Plan::decompose() {
     }
     if (use_delta && pending) {  // the last pass removed the last edges at k = 3
         KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, pending,
-                                         dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
+                                         dst[cur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
                                          sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
         pending = 0;
     }
@@ -510,6 +515,7 @@ void Plan::finish_stats() {
     stats.merge_elems = h_ctr->merge_elems;
     stats.recount_edges = h_ctr->recount_edges;
     stats.delta_elems = h_ctr->delta_elems;
+    stats.delta_live = h_ctr->delta_live;
     stats.delta_triangles = h_ctr->delta_tris;
     // DELTA's algorithmic bytes: candidates and longer-list windows read,
     // and per destroyed triangle two slot->entry reads, two flag reads and
@@ -594,23 +600,74 @@ int Plan::ensure_delta() {
     int rc = build_adjacency(n, E0, h_ptr.data(), h_src.data(), h_dst.data(), hs_ptr, hs_nbr,
                              hs_eid, hs_a, hs_b);
     if (rc) return rc;
-    const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4;
-    if ((rc = sptr.alloc(((size_t)n + 1) * 4)) || (rc = snbr0.alloc(2 * Eb)) ||
-        (rc = snbr.alloc(2 * Eb)) || (rc = seid.alloc(2 * Eb)) || (rc = sa.alloc(Eb)) ||
-        (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)) ||
-        (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))))
+    const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4, Pb = ((size_t)n + 1) * 4;
+    const uint32_t ct = dev::compact_tiles_for(2 * E0) + 2;
+    for (int b = 0; b < 2; ++b)
+        if ((rc = sptr_b[b].alloc(Pb)) || (rc = snbr_b[b].alloc(2 * Eb)) ||
+            (rc = seid_b[b].alloc(2 * Eb)) || (rc = ll[b].alloc(Eb)))
+            return rc;
+    if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
+        (rc = sa0.alloc(Eb)) || (rc = sb0.alloc(Eb)) || (rc = sa.alloc(Eb)) || (rc = sb.alloc(Eb)) ||
+        (rc = dl.alloc(Eb)) || (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))) ||
+        (rc = ctsum.alloc((size_t)ct * 4)) || (rc = cxs.alloc(2 * Eb + 4)))
         return rc;
     dev::delta_scratch_carve(dscratch.p, E0, dws);
-    KT_HIP_TRY(hipMemcpyAsync(sptr.p, hs_ptr.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream));
+    KT_HIP_TRY(hipMemcpyAsync(sptr0.p, hs_ptr.data(), Pb, hipMemcpyHostToDevice, stream));
     if (E0) {
         KT_HIP_TRY(hipMemcpyAsync(snbr0.p, hs_nbr.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, 2 * (size_t)E0 * 4, hipMemcpyDeviceToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(seid.p, hs_eid.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(sa.p, hs_a.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(sb.p, hs_b.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(seid0.p, hs_eid.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(sa0.p, hs_a.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(sb0.p, hs_b.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
     }
+    if ((rc = restore_delta())) return rc;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     delta_ready = true;
+    return KT_OK;
+}
+
+// The working adjacency from its pristine copy.
+int Plan::restore_delta() {
+    const size_t Sb = 2 * (size_t)E0 * 4, Eb = (size_t)E0 * 4, Pb = ((size_t)n + 1) * 4;
+    acur = 0;
+    S_cur = 2 * E0;
+    KT_HIP_TRY(hipMemcpyAsync(sptr_b[0].p, sptr0.p, Pb, hipMemcpyDeviceToDevice, stream));
+    if (E0) {
+        KT_HIP_TRY(hipMemcpyAsync(snbr_b[0].p, snbr0.p, Sb, hipMemcpyDeviceToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(seid_b[0].p, seid0.p, Sb, hipMemcpyDeviceToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(sa.p, sa0.p, Eb, hipMemcpyDeviceToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(sb.p, sb0.p, Eb, hipMemcpyDeviceToDevice, stream));
+    }
+    return KT_OK;
+}
+
+// With no deletion pending (every entry is live or dead), shrink what the
+// next passes walk: the mark's live list once fewer than half of its items
+// are alive, the adjacency lists once fewer than half of their slots are.
+// The new sizes are known on the host (live entries, two slots per live edge).
+int Plan::compact_delta_state() {
+    if (pending || live == 0) return KT_OK;
+    const uint32_t items = use_list ? L : E0;
+    if ((uint64_t)items > 2 * live && items > 1024) {
+        const int o = 1 - lcur;
+        KT_HIP_TRY(dev::compact_live_list(use_list ? ll[lcur].as<uint32_t>() : nullptr, items,
+                                          dst[cur].as<uint32_t>(), ll[o].as<uint32_t>(),
+                                          ctsum.as<uint32_t>(), stream));
+        lcur = o;
+        use_list = true;
+        L = (uint32_t)live;
+    }
+    const uint64_t keep = 2 * live;
+    if ((uint64_t)S_cur > 2 * keep && S_cur > 2048) {
+        const int o = 1 - acur;
+        KT_HIP_TRY(dev::compact_adjacency(n, S_cur, sptr_b[acur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
+                                          seid_b[acur].as<uint32_t>(), sptr_b[o].as<uint32_t>(),
+                                          snbr_b[o].as<uint32_t>(), seid_b[o].as<uint32_t>(),
+                                          (uint32_t)keep, dst[cur].as<uint32_t>(), sa.as<uint32_t>(),
+                                          sb.as<uint32_t>(), ctsum.as<uint32_t>(), cxs.as<uint32_t>(),
+                                          stream));
+        acur = o;
+        S_cur = (uint32_t)keep;
+    }
     return KT_OK;
 }
 
@@ -624,23 +681,23 @@ int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
     const uint32_t* nd_dev = &ctr.as<DevCounters>()->ndel;
     if (pending && cfg.kernel == KT_COUNT_DELTA) {
         KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
-                                      dst[cur].as<uint32_t>(), sptr.as<uint32_t>(), dws,
+                                      dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws,
                                       ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c0, stream));
-        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(), snbr.as<uint32_t>(),
-                                    seid.as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
+        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
+                                    seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
                                     ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c1, stream));
         last_pass_delta = true;
         KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
-                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                         snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                          stream));
         stats.delta_passes++;
         stats.delta_edges += pending;
     } else {
         if (pending)
             KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
-                                             snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                             snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                              stream));
         CountCfg rcfg = cfg;
         if (rcfg.kernel == KT_COUNT_DELTA) rcfg.kernel = KT_COUNT_ROW;
@@ -685,17 +742,17 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
             first = false;
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
             KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
-                                          dst[cur].as<uint32_t>(), sptr.as<uint32_t>(), dws,
+                                          dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws,
                                           ctr.as<DevCounters>(), stream));
             KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
-                                        snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                        snbr_b[acur].as<uint32_t>(), seid_b[acur].as<uint32_t>(),
                                         sup[cur].as<uint32_t>(), dws, ctr.as<DevCounters>(), stream));
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
         }
     }
     if (had_pending)
         KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
-                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                         snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                          stream));
     uint32_t last_recount = nc;
     for (uint32_t c = 0; c < nc; ++c)
@@ -757,18 +814,21 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             stats.count_passes++;
         }
         if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
+        if ((rc = compact_delta_state())) return rc;
+        const uint32_t* list = use_list ? ll[lcur].as<uint32_t>() : nullptr;
+        const uint32_t items = use_list ? L : E;
         if (iter == 0 && jump) {
             if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
-            KT_HIP_TRY(dev::min_live_support(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E,
-                                             ctr.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::min_live_list(list, items, dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
+                                          ctr.as<DevCounters>(), stream));
             if ((rc = sync_counters())) return rc;
             if (unresolved) resolve_choice();
             const uint32_t s = h_ctr->min_sup;
             if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
         }
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
-                                   sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
-                                   ctr.as<DevCounters>(), stream));
+                                   sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), list,
+                                   items, k, ctr.as<DevCounters>(), stream));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;

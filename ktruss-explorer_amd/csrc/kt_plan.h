@@ -73,7 +73,19 @@ struct Plan {
     // the oriented host layout kept here) and the list of entries deleted by
     // the last mark pass whose triangles are still counted in the supports
     std::vector<uint32_t> h_ptr, h_src, h_dst;
-    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, sup_snap, dscratch;
+    // adjacency lists: working pair (sptr/snbr/seid)[acur] of S_cur slots,
+    // compacted into the other of the pair; pristine copies for every run
+    DBuf sptr_b[2], snbr_b[2], seid_b[2], sa, sb;
+    DBuf sptr0, snbr0, seid0, sa0, sb0;
+    int acur = 0;
+    uint32_t S_cur = 0;
+    // live list walked by the mark pass (identity over [0, E0) until the
+    // first compaction)
+    DBuf ll[2];
+    int lcur = 0;
+    bool use_list = false;
+    uint32_t L = 0;
+    DBuf dl, sup_snap, dscratch, ctsum, cxs;
     dev::DeltaScratch dws{};
     bool delta_ready = false;
     bool last_pass_delta = false;  // the last support pass was a DELTA update
@@ -96,6 +108,8 @@ struct Plan {
     int run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out);
     int run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule);
     int ensure_delta();
+    int compact_delta_state();
+    int restore_delta();
     int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
     // defer: leave the model's choice on the device (read with the next

@@ -99,7 +99,8 @@ class Stats(ctypes.Structure):
                 ("count_launches", ctypes.c_uint64), ("balg_count_bytes", ctypes.c_double),
                 ("delta_passes", ctypes.c_uint64), ("delta_edges", ctypes.c_uint64),
                 ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64),
-                ("ms_delta", ctypes.c_double), ("balg_delta_bytes", ctypes.c_double)]
+                ("ms_delta", ctypes.c_double), ("balg_delta_bytes", ctypes.c_double),
+                ("delta_live", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

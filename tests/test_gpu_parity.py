@@ -127,7 +127,8 @@ def test_rmat_matches_golden_fixture(scale):
     assert h == gold["pairs_sha256"], "generator output changed"
     g = K.Graph(n, s, d, pairs=True)
     variants = [K.default_variant(), K.make_variant("directed-by-degree", "coocsr", "affected"),
-                K.make_variant("directed-by-degree", "tiled-coocsr", "all")]
+                K.make_variant("directed-by-degree", "tiled-coocsr", "all"),
+                K.make_variant(count_kernel="delta")]
     if scale <= 16:
         variants += [K.make_variant("undirected", "coocsr", "all"),
                      K.make_variant("directed-by-index", "coocsr", "affected", e_iters=4)]
