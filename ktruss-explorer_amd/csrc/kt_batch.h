@@ -54,10 +54,12 @@ hipError_t batch_control(const BatchDev& b, uint32_t mode, hipStream_t s);
 hipError_t batch_expand(const BatchDev& b, uint32_t blocks, hipStream_t s);
 hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, uint32_t blocks,
                       hipStream_t s);
-// dl != nullptr: DELTA marking (KT_DYING, deleted entries appended to dl, count *dn).
+// dl != nullptr: DELTA marking (KT_DYING on the entry and its two adjacency
+// slots snbr[sa[e]], snbr[sb[e]]; deleted entries appended to dl, count *dn).
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
-                      unsigned int* dn, uint32_t blocks, hipStream_t s);
+                      unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
+                      uint32_t blocks, hipStream_t s);
 hipError_t batch_min(const TaskList& tl, const uint32_t* dst, const uint32_t* sup, uint32_t* gmin,
                      uint32_t blocks, hipStream_t s);
 }  // namespace dev

@@ -72,7 +72,8 @@ template <bool DELTA>
 __global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
                          uint32_t* __restrict__ tau, const uint32_t* __restrict__ gk,
                          uint32_t* gndel, uint32_t* gmin, uint32_t* __restrict__ dl,
-                         unsigned int* dn) {
+                         unsigned int* dn, uint32_t* __restrict__ snbr,
+                         const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t n = *tl.ntasks;
     for (uint32_t t = gwave(); t < n; t += nwaves()) {
@@ -98,7 +99,12 @@ __global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t
             }
             if (DELTA) {
                 const uint32_t pos = wave_append(dn, del, lane);
-                if (del) dl[pos] = i;
+                if (del) {
+                    dl[pos] = i;
+                    const uint32_t a = sa[i], b = sb[i];
+                    snbr[a] |= KT_DYING;  // the deleted edge's adjacency slots
+                    snbr[b] |= KT_DYING;
+                }
             }
             ndel += (uint32_t)__popcll(ballot(del));
         }
@@ -309,13 +315,14 @@ hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, ui
 
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
-                      unsigned int* dn, uint32_t blocks, hipStream_t s) {
+                      unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
+                      uint32_t blocks, hipStream_t s) {
     if (dl)
         hipLaunchKernelGGL(k_b_mark<true>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
-                           gk, gndel, gmin, dl, dn);
+                           gk, gndel, gmin, dl, dn, snbr, sa, sb);
     else
         hipLaunchKernelGGL(k_b_mark<false>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
-                           gk, gndel, gmin, dl, dn);
+                           gk, gndel, gmin, dl, dn, snbr, sa, sb);
     return hipGetLastError();
 }
 

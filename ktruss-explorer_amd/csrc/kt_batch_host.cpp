@@ -337,12 +337,13 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
         // mark -> DELTA update of this step's deletions -> finalize -> control -> expand
         unsigned int* dn = &bd.bc->count.ndel;
         KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
-                                   bd.gndel, bd.gmin, dl.as<uint32_t>(), dn, stream_blocks(), stream));
+                                   bd.gndel, bd.gmin, dl.as<uint32_t>(), dn, snbr.as<uint32_t>(),
+                                   sa.as<uint32_t>(), sb.as<uint32_t>(), stream_blocks(), stream));
         KT_HIP_TRY(dev::delta_prepare(dl.as<uint32_t>(), dn, E, src.as<uint32_t>(), dst.as<uint32_t>(),
                                       sptr.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
-        KT_HIP_TRY(dev::delta_apply(dl.as<uint32_t>(), dn, E, dst.as<uint32_t>(), snbr.as<uint32_t>(),
-                                    seid.as<uint32_t>(), sup.as<uint32_t>(), dws, &bd.bc->count, stream));
+        KT_HIP_TRY(dev::delta_apply(dl.as<uint32_t>(), dn, E, snbr.as<uint32_t>(), seid.as<uint32_t>(),
+                                    sup.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c1) KT_HIP_TRY(hipEventRecord(c1, stream));
         KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), dn, E, dst.as<uint32_t>(),
                                          snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
@@ -351,7 +352,8 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
         return KT_OK;
     }
     KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
-                               bd.gndel, bd.gmin, nullptr, nullptr, stream_blocks(), stream));
+                               bd.gndel, bd.gmin, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_control(bd, 2, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_zero(dtl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));

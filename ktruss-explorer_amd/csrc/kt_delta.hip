@@ -19,6 +19,8 @@
 //               exactly once from every surviving edge;
 //   k_finalize  the deleted entries become KT_DEAD in the oriented layout and
 //               in both of their adjacency-list slots.
+// Between the mark and k_finalize a deleted entry and both of its adjacency
+// slots carry KT_DYING (bit 30; ids stay below 2^30).
 // Adjacency layout (symmetric CSR over the relabelled ids):
 //   sptr[n+1]   list starts
 //   snbr[2E]    neighbour id, KT_DEAD set once the edge is gone; a vertex's
@@ -56,18 +58,18 @@ __device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, ui
 
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77), deferred: a deleted entry
 // is flagged KT_DYING and listed; it becomes KT_DEAD in k_finalize.
-// Items are list[i] for i < n (the live list), or the entries 0..n-1.
+// The deleted entry's two adjacency slots get KT_DYING as well.
 __global__ void k_mark_dying(uint32_t* __restrict__ dst, const uint32_t* __restrict__ eid,
                              const uint32_t* __restrict__ sup, uint32_t* __restrict__ tau,
-                             uint32_t* __restrict__ dl, const uint32_t* __restrict__ list,
-                             uint32_t n, uint32_t k, DevCounters* ctr) {
+                             uint32_t* __restrict__ dl, uint32_t n, uint32_t k,
+                             uint32_t* __restrict__ snbr, const uint32_t* __restrict__ sa,
+                             const uint32_t* __restrict__ sb, DevCounters* ctr) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t e = base + threadIdx.x;
         bool del = false;
-        uint32_t v = 0, e = 0;
-        if (i < n) {
-            e = list ? list[i] : i;
+        uint32_t v = 0;
+        if (e < n) {
             v = dst[e];
             del = !(v & (KT_DEAD | KT_DYING)) && sup[e] < k - 2;
         }
@@ -76,21 +78,20 @@ __global__ void k_mark_dying(uint32_t* __restrict__ dst, const uint32_t* __restr
             dst[e] = v | KT_DYING;
             tau[eid[e]] = k - 1;
             dl[pos] = e;
+            const uint32_t a = sa[e], b = sb[e];
+            snbr[a] |= KT_DYING;
+            snbr[b] |= KT_DYING;
         }
     }
 }
 
-// Minimum support over the live list (the JUMP target); one atomic per
-// workgroup.
-__global__ void k_min_list(const uint32_t* __restrict__ list, uint32_t n,
-                           const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
-                           DevCounters* ctr) {
+// Minimum live support (the JUMP target); one atomic per workgroup.
+__global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
+                           const uint32_t* __restrict__ sup, DevCounters* ctr) {
     __shared__ uint32_t s_min[kBlock / kWave];
     uint32_t m = 0xFFFFFFFFu;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t e = list ? list[i] : i;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
         if (!(dst[e] & (KT_DEAD | KT_DYING)) && sup[e] < m) m = sup[e];
-    }
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t o = __shfl_xor(m, off);
         m = o < m ? o : m;
@@ -214,13 +215,25 @@ __global__ __launch_bounds__(kBlock) void k_dscan_local(const uint32_t* __restri
 // The update proper.  Task t = (deletion i, chunk c) holds the candidates
 // short[64c, 64c + 64) of deletion i; wave w takes the task range
 // [T w / W, T (w + 1) / W), finds its first deletion by a binary search over
-// the task offsets and walks forward.  A task whose candidates are all dead
-// is skipped.  Otherwise the candidates' values span [xmin, xmax]; the window
-// of the longer list holding that range is found by two wave-uniform binary
-// searches.  A window of up to kWin entries is staged in the wave's LDS slice
-// (one coalesced read) and every lane binary-searches its candidate there;
-// a longer window is searched in place.
-constexpr uint32_t kWin = 512;
+// the task offsets and walks forward, so the tasks of one deletion follow
+// each other.  A task whose
+// candidates are all dead is skipped.  Every
+// candidate is binary searched in LDS, never by dependent probes of HBM:
+//   longer list of at most kStage entries: staged once per deletion in the
+//     wave's LDS slice (one coalesced read);
+//   longer list of at most 64 kSample entries: the first id of every
+//     64-entry block is staged once per deletion (one strided read); two
+//     searches there bound the window of the list that the task's candidate
+//     range [xmin, xmax] can hit, and the window (at most kStage entries, or
+//     searched in place beyond) is staged for the task;
+//   longer still (hub lists): the window is bounded by two interleaved
+//     wave-uniform searches of the list itself.
+// Adjacency slots carry KT_DYING for the edges the current pass deleted, so
+// the ownership test reads no flags of the oriented layout: a triangle whose
+// other two edges survive (the common case) costs two slot->entry reads and
+// two atomic decrements.
+constexpr uint32_t kStage = 512;
+constexpr uint32_t kSample = 512;
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -228,17 +241,17 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// First index in [lo, lo + len) whose value (dead bit masked) is >= x; the
-// trip count depends on len only.
+// First index in [lo, lo + len) whose id (flags masked) is >= x; the trip
+// count depends on len only.
 __device__ __forceinline__ uint32_t lower_bound_u(const uint32_t* a, uint32_t lo, uint32_t len,
                                                   uint32_t x) {
     if (len == 0) return lo;
     while (len > 1) {
         const uint32_t half = len >> 1;
-        if ((a[lo + half] & KT_VMASK) < x) lo += half;
+        if ((a[lo + half] & KT_IDMASK) < x) lo += half;
         len -= half;
     }
-    return lo + ((a[lo] & KT_VMASK) < x ? 1u : 0u);
+    return lo + ((a[lo] & KT_IDMASK) < x ? 1u : 0u);
 }
 
 // Lower bounds of x0 and x1 in [lo, lo + len), interleaved (same trip count).
@@ -248,26 +261,52 @@ __device__ __forceinline__ void lower_bounds_2(const uint32_t* a, uint32_t lo, u
     uint32_t b0 = lo, b1 = lo;
     while (len > 1) {
         const uint32_t half = len >> 1;
-        const uint32_t v0 = a[b0 + half] & KT_VMASK, v1 = a[b1 + half] & KT_VMASK;
+        const uint32_t v0 = a[b0 + half] & KT_IDMASK, v1 = a[b1 + half] & KT_IDMASK;
         if (v0 < x0) b0 += half;
         if (v1 < x1) b1 += half;
         len -= half;
     }
-    r0 = b0 + ((a[b0] & KT_VMASK) < x0 ? 1u : 0u);
-    r1 = b1 + ((a[b1] & KT_VMASK) < x1 ? 1u : 0u);
+    r0 = b0 + ((a[b0] & KT_IDMASK) < x0 ? 1u : 0u);
+    r1 = b1 + ((a[b1] & KT_IDMASK) < x1 ? 1u : 0u);
+}
+
+// Number of samples < x in s[0, n) (n >= 1), uniform trip count.
+__device__ __forceinline__ uint32_t count_less_lds(const uint32_t* s, uint32_t n, uint32_t x) {
+    uint32_t b = 0;
+    while (n > 1) {
+        const uint32_t half = n >> 1;
+        if (s[b + half] < x) b += half;
+        n -= half;
+    }
+    return b + (s[b] < x ? 1u : 0u);
+}
+
+// Searches xv in buf[0, n) (staged list entries) and returns the slot offset
+// and the entry (KT_DEAD when past the end).
+__device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t n, uint32_t xv,
+                                                  uint32_t& y) {
+    uint32_t b = 0, m = n;
+    while (m > 1) {
+        const uint32_t half = m >> 1;
+        if ((buf[b + half] & KT_IDMASK) < xv) b += half;
+        m -= half;
+    }
+    const uint32_t q = b + ((buf[b] & KT_IDMASK) < xv ? 1u : 0u);
+    y = q < n ? buf[q] : KT_DEAD;
+    return q;
 }
 
 __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
                                                   const uint32_t* __restrict__ toff,
-                                                  const uint32_t* __restrict__ dst,
                                                   const uint32_t* __restrict__ snbr,
                                                   const uint32_t* __restrict__ seid,
                                                   uint32_t* sup, DevCounters* ctr) {
-    __shared__ uint32_t s_win[(kBlock / kWave) * kWin];
+    __shared__ uint32_t s_buf[(kBlock / kWave) * (kStage + kSample)];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    uint32_t* win = s_win + (threadIdx.x / kWave) * kWin;
+    uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample);
+    uint32_t* smp = buf + kStage;
     const uint32_t nd = *nd_dev;
     const uint32_t T = ctr->dtasks;
     const uint32_t W = nwaves(), w = gwave();
@@ -280,11 +319,14 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         if (toff[i + half] <= t0) i += half;
         len -= half;
     }
+    uint32_t full = 0xFFFFFFFFu;     // deletion whose whole longer list is in buf
+    uint32_t sampled = 0xFFFFFFFFu;  // deletion whose block samples are in smp
     unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
-    for (uint32_t t = t0; t < t1; ++t) {
+    for (uint32_t t = t0; t < t1;) {
         while (i + 1 < nd && toff[i + 1] <= t) ++i;
+        const uint32_t tc = t++;  // this task; the body may continue below
         const uint4 m = meta[i];
-        const uint32_t j0 = m.x + (t - toff[i]) * kWave;
+        const uint32_t j0 = m.x + (tc - toff[i]) * kWave;
         const uint32_t j = j0 + lane;
         acc_elems += min(kWave, m.y - j0);
         const uint32_t x = j < m.y ? snbr[j] : KT_DEAD;
@@ -292,45 +334,66 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         const uint64_t cm = ballot(cand);
         acc_live += (uint32_t)__popcll(cm);
         if (!cm) continue;
-        // candidates ascend with the lane: the window of the longer list is
-        // [lower bound of xmin, lower bound of xmax + 1), two wave-uniform
-        // searches run in the same loop so their loads overlap
-        const uint32_t xmin = __builtin_amdgcn_readlane(x, (uint32_t)__ffsll((long long)cm) - 1);
-        const uint32_t xmax = __builtin_amdgcn_readlane(x, 63u - (uint32_t)__builtin_clzll(cm));
-        uint32_t lo, hi;
-        lower_bounds_2(snbr, m.z, m.w, xmin, xmax + 1, lo, hi);
-        const uint32_t wl = hi - lo;
-        if (wl == 0) continue;
-        // entries read from the longer list: the staged window, or one probe
-        // per search step per live candidate when searched in place
-        acc_win += wl <= kWin ? wl : (uint64_t)__popcll(cm) * (32u - __builtin_clz(wl));
-        uint32_t p;
-        bool hit;
-        if (wl <= kWin) {
-            wave_lds_sync();
-            for (uint32_t q = lane; q < wl; q += kWave) win[q] = snbr[lo + q];
-            wave_lds_sync();
-            uint32_t b = 0, n = wl;
-            while (n > 1) {
-                const uint32_t half = n >> 1;
-                if ((win[b + half] & KT_VMASK) < x) b += half;
-                n -= half;
+        const uint32_t xv = x & KT_IDMASK;
+        uint32_t p, y;
+        if (m.w <= kStage) {
+            if (full != i) {
+                wave_lds_sync();
+                for (uint32_t q = lane; q < m.w; q += kWave) buf[q] = snbr[m.z + q];
+                wave_lds_sync();
+                full = i;
+                acc_win += m.w;
             }
-            const uint32_t q = b + ((win[b] & KT_VMASK) < x ? 1u : 0u);
-            hit = cand && q < wl && win[q] == x;
-            p = lo + q;
+            p = m.z + search_staged(buf, m.w, xv, y);
         } else {
-            p = lower_bound_u(snbr, lo, wl, x);
-            hit = cand && p < hi && snbr[p] == x;
+            // candidates ascend with the lane
+            const uint32_t xmin = __builtin_amdgcn_readlane(xv, (uint32_t)__ffsll((long long)cm) - 1);
+            const uint32_t xmax = __builtin_amdgcn_readlane(xv, 63u - (uint32_t)__builtin_clzll(cm));
+            uint32_t lo, hi;
+            const uint32_t nb = (m.w + kWave - 1) / kWave;
+            if (nb <= kSample) {
+                if (sampled != i) {
+                    wave_lds_sync();
+                    for (uint32_t q = lane; q < nb; q += kWave) smp[q] = snbr[m.z + q * kWave] & KT_IDMASK;
+                    wave_lds_sync();
+                    sampled = i;
+                    acc_win += nb;
+                }
+                // blocks whose first id is below xmin / xmax + 1: the bounds lie
+                // in the last such block (or at the list start)
+                const uint32_t ca = count_less_lds(smp, nb, xmin), cb = count_less_lds(smp, nb, xmax + 1);
+                lo = m.z + (ca ? (ca - 1) * kWave : 0u);
+                hi = m.z + min(m.w, cb * kWave);
+            } else {
+                lower_bounds_2(snbr, m.z, m.w, xmin, xmax + 1, lo, hi);
+            }
+            const uint32_t wl = hi - lo;
+            if (wl == 0) continue;
+            if (wl <= kStage) {
+                full = 0xFFFFFFFFu;  // buf now holds a window
+                acc_win += wl;
+                wave_lds_sync();
+                for (uint32_t q = lane; q < wl; q += kWave) buf[q] = snbr[lo + q];
+                wave_lds_sync();
+                p = lo + search_staged(buf, wl, xv, y);
+            } else {
+                acc_win += (uint64_t)__popcll(cm) * (32u - __builtin_clz(wl));
+                p = lower_bound_u(snbr, lo, wl, xv);
+                y = p < hi ? snbr[p] : KT_DEAD;
+            }
         }
+        const bool hit = cand && (y & (KT_DEAD | KT_IDMASK)) == xv;
         bool tri = false;
         if (hit) {
-            const uint32_t d = dl[i];
             const uint32_t e1 = seid[j], e2 = seid[p];
-            const bool y1 = (dst[e1] & KT_DYING) != 0;
-            const bool y2 = (dst[e2] & KT_DYING) != 0;
+            const bool y1 = (x & KT_DYING) != 0, y2 = (y & KT_DYING) != 0;
             // the deleted edge of lowest index owns the triangle
-            if (!((y1 && e1 < d) || (y2 && e2 < d))) {
+            bool own = true;
+            if (y1 || y2) {
+                const uint32_t d = dl[i];
+                own = !((y1 && e1 < d) || (y2 && e2 < d));
+            }
+            if (own) {
                 if (!y1) atomicSub(sup + e1, 1u);
                 if (!y2) atomicSub(sup + e2, 1u);
                 tri = true;
@@ -353,8 +416,9 @@ __global__ void k_finalize(const uint32_t* __restrict__ dl, const uint32_t* __re
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += gridDim.x * blockDim.x) {
         const uint32_t d = dl[t];
         dst[d] = (dst[d] & ~KT_DYING) | KT_DEAD;
-        snbr[sa[d]] |= KT_DEAD;
-        snbr[sb[d]] |= KT_DEAD;
+        const uint32_t a = sa[d], b = sb[d];
+        snbr[a] = (snbr[a] & ~KT_DYING) | KT_DEAD;
+        snbr[b] = (snbr[b] & ~KT_DYING) | KT_DEAD;
     }
 }
 
@@ -365,39 +429,22 @@ uint32_t blocks_for(uint64_t items, uint32_t per_block) {
 }
 
 // ---------------------------------------------------------------------------
-// Compaction of the DELTA state (stable, tiles of kTileC items, one 256-thread
-// workgroup per tile: count -> one-workgroup scan of the tile counts ->
-// scatter with in-tile ballot prefixes).
-//   live list   the mark pass walks a list of the entries still alive instead
-//               of every stored entry; items are list[i] (or i when the list
-//               is the identity) kept when the entry is neither dead nor dying
-//   adjacency   slots whose edge is dead leave the lists; the list starts are
-//               rebuilt from the slot prefix and every live entry's two slots
-//               (sa, sb) are found again from the kept slots
-// Entries never move, so supports, truss numbers and the oriented layout are
-// untouched.
+// Compaction of the adjacency lists (stable, tiles of kTileC slots, one
+// 256-thread workgroup per tile: count -> one-workgroup scan of the tile
+// counts -> scatter with in-tile ballot prefixes).  Slots whose edge is dead
+// leave the lists; the slot -> entry map is rewritten through the oriented
+// layout's own compaction map (the entries move too); the list starts are
+// rebuilt from the slot prefix and every live entry's two slots (sa, sb) are
+// found again from the kept slots.
 constexpr uint32_t kTileC = 2048;
 
-__device__ __forceinline__ bool item_kept(const uint32_t* list, uint32_t i, const uint32_t* dst,
-                                          const uint32_t* snbr, uint32_t* item) {
-    if (snbr) {  // adjacency slot i
-        *item = i;
-        return !(snbr[i] & KT_DEAD);
-    }
-    const uint32_t e = list ? list[i] : i;
-    *item = e;
-    return !(dst[e] & (KT_DEAD | KT_DYING));
-}
-
-__global__ __launch_bounds__(kBlock) void k_cmp_count(const uint32_t* __restrict__ list, uint32_t n,
-                                                      const uint32_t* __restrict__ dst,
-                                                      const uint32_t* __restrict__ snbr,
+__global__ __launch_bounds__(kBlock) void k_cmp_count(uint32_t n, const uint32_t* __restrict__ snbr,
                                                       uint32_t* __restrict__ tsum) {
     __shared__ uint32_t s_w[kBlock / kWave];
     const uint32_t t = blockIdx.x;
-    uint32_t c = 0, item;
+    uint32_t c = 0;
     for (uint32_t i = t * kTileC + threadIdx.x; i < (t + 1) * kTileC && i < n; i += kBlock)
-        c += item_kept(list, i, dst, snbr, &item) ? 1u : 0u;
+        c += (snbr[i] & KT_DEAD) ? 0u : 1u;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
     if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = c;
     __syncthreads();
@@ -432,16 +479,15 @@ __global__ __launch_bounds__(1024) void k_cmp_scan(uint32_t* tsum, uint32_t nt) 
     if (threadIdx.x == 1023) tsum[nt] = s_part[1023];
 }
 
-// Scatter of the kept items.  Live list: out[pos] = entry.  Adjacency:
-// out / out2 receive snbr / seid, and xs[i] = kept slots before slot i (for
-// every slot; xs[n] is set by the last tile).
-__global__ __launch_bounds__(kBlock) void k_cmp_scatter(const uint32_t* __restrict__ list, uint32_t n,
-                                                        const uint32_t* __restrict__ dst,
-                                                        const uint32_t* __restrict__ snbr,
+// Scatter of the kept slots: snbr2 / seid2 receive the slot's neighbour and
+// its entry's new position remap[seid], and xs[i] = kept slots before slot i
+// (for every slot; xs[n] is set by the last tile).
+__global__ __launch_bounds__(kBlock) void k_cmp_scatter(uint32_t n, const uint32_t* __restrict__ snbr,
                                                         const uint32_t* __restrict__ seid,
+                                                        const uint32_t* __restrict__ remap,
                                                         const uint32_t* __restrict__ tsum,
-                                                        uint32_t* __restrict__ out,
-                                                        uint32_t* __restrict__ out2,
+                                                        uint32_t* __restrict__ snbr2,
+                                                        uint32_t* __restrict__ seid2,
                                                         uint32_t* __restrict__ xs) {
     __shared__ uint32_t s_w[kBlock / kWave];
     const uint32_t t = blockIdx.x;
@@ -449,8 +495,8 @@ __global__ __launch_bounds__(kBlock) void k_cmp_scatter(const uint32_t* __restri
     uint32_t run = tsum[t];
     for (uint32_t r = 0; r < kTileC / kBlock; ++r) {
         const uint32_t i = t * kTileC + r * kBlock + threadIdx.x;
-        uint32_t item = 0;
-        const bool keep = i < n && item_kept(list, i, dst, snbr, &item);
+        const uint32_t y = i < n ? snbr[i] : KT_DEAD;
+        const bool keep = !(y & KT_DEAD);
         const uint64_t m = ballot(keep);
         if (lane == 0) s_w[wid] = (uint32_t)__popcll(m);
         __syncthreads();
@@ -460,19 +506,15 @@ __global__ __launch_bounds__(kBlock) void k_cmp_scatter(const uint32_t* __restri
             round += s_w[w];
         }
         const uint32_t pos = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (xs && i < n) xs[i] = pos;
+        if (i < n) xs[i] = pos;
         if (keep) {
-            if (snbr) {
-                out[pos] = snbr[item];
-                out2[pos] = seid[item];
-            } else {
-                out[pos] = item;
-            }
+            snbr2[pos] = y;
+            seid2[pos] = remap[seid[i]];
         }
         run += round;
         __syncthreads();
     }
-    if (xs && t == gridDim.x - 1 && threadIdx.x == 0) xs[n] = tsum[gridDim.x];
+    if (t == gridDim.x - 1 && threadIdx.x == 0) xs[n] = tsum[gridDim.x];
 }
 
 __global__ void k_cmp_sptr(const uint32_t* __restrict__ sptr, const uint32_t* __restrict__ xs,
@@ -488,7 +530,7 @@ __global__ void k_cmp_slots(const uint32_t* __restrict__ snbr2, const uint32_t* 
                             uint32_t* __restrict__ sb) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < S2; j += gridDim.x * blockDim.x) {
         const uint32_t e = seid2[j];
-        if ((snbr2[j] & KT_VMASK) == (dst[e] & KT_IDMASK)) sa[e] = j;
+        if ((snbr2[j] & KT_IDMASK) == (dst[e] & KT_IDMASK)) sa[e] = j;
         else sb[e] = j;
     }
 }
@@ -498,20 +540,19 @@ __global__ void k_cmp_slots(const uint32_t* __restrict__ snbr2, const uint32_t* 
 namespace dev {
 
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
-                      uint32_t* dl, const uint32_t* list, uint32_t n, uint32_t k, DevCounters* ctr,
-                      hipStream_t s) {
+                      uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
+                      const uint32_t* sb, DevCounters* ctr, hipStream_t s) {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_mark_dying, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, dst, eid, sup,
-                       tau, dl, list, n, k, ctr);
+                       tau, dl, n, k, snbr, sa, sb, ctr);
     return hipGetLastError();
 }
 
-hipError_t min_live_list(const uint32_t* list, uint32_t n, const uint32_t* dst, const uint32_t* sup,
-                         DevCounters* ctr, hipStream_t s) {
+hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
+                          hipStream_t s) {
     if (!n) return hipSuccess;
     const uint32_t b = blocks_for(n, kBlock);
-    hipLaunchKernelGGL(k_min_list, dim3(b < 2048 ? b : 2048), dim3(kBlock), 0, s, list, n, dst, sup,
-                       ctr);
+    hipLaunchKernelGGL(k_min_list, dim3(b < 2048 ? b : 2048), dim3(kBlock), 0, s, n, dst, sup, ctr);
     return hipGetLastError();
 }
 
@@ -546,12 +587,12 @@ hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd
 }
 
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
-                       const uint32_t* dst, const uint32_t* snbr, const uint32_t* seid,
-                       uint32_t* sup, const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
+                       const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
+                       const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
     if (!nd_max) return hipSuccess;
     const uint32_t db = nd_max < 64 ? 64 : (nd_max > 2048 ? 2048 : nd_max);
-    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, dst,
-                       snbr, seid, sup, ctr);
+    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, snbr,
+                       seid, sup, ctr);
     return hipGetLastError();
 }
 
@@ -566,33 +607,19 @@ hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t
 
 uint32_t compact_tiles_for(uint32_t n) { return (n + kTileC - 1) / kTileC; }
 
-hipError_t compact_live_list(const uint32_t* list, uint32_t n, const uint32_t* dst, uint32_t* out,
-                             uint32_t* tsum, hipStream_t s) {
-    const uint32_t nt = compact_tiles_for(n);
-    if (!nt) return hipSuccess;
-    hipLaunchKernelGGL(k_cmp_count, dim3(nt), dim3(kBlock), 0, s, list, n, dst, nullptr, tsum);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, s, tsum, nt);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cmp_scatter, dim3(nt), dim3(kBlock), 0, s, list, n, dst, nullptr, nullptr,
-                       tsum, out, nullptr, nullptr);
-    return hipGetLastError();
-}
-
 hipError_t compact_adjacency(uint32_t nv, uint32_t S, const uint32_t* sptr, const uint32_t* snbr,
-                             const uint32_t* seid, uint32_t* sptr2, uint32_t* snbr2, uint32_t* seid2,
-                             uint32_t S2, const uint32_t* dst, uint32_t* sa, uint32_t* sb,
-                             uint32_t* tsum, uint32_t* xs, hipStream_t s) {
+                             const uint32_t* seid, const uint32_t* remap, uint32_t* sptr2,
+                             uint32_t* snbr2, uint32_t* seid2, uint32_t S2, const uint32_t* dst,
+                             uint32_t* sa, uint32_t* sb, uint32_t* tsum, uint32_t* xs, hipStream_t s) {
     const uint32_t nt = compact_tiles_for(S);
     if (!nt) return hipSuccess;
-    hipLaunchKernelGGL(k_cmp_count, dim3(nt), dim3(kBlock), 0, s, nullptr, S, nullptr, snbr, tsum);
+    hipLaunchKernelGGL(k_cmp_count, dim3(nt), dim3(kBlock), 0, s, S, snbr, tsum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, s, tsum, nt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cmp_scatter, dim3(nt), dim3(kBlock), 0, s, nullptr, S, nullptr, snbr, seid,
-                       tsum, snbr2, seid2, xs);
+    hipLaunchKernelGGL(k_cmp_scatter, dim3(nt), dim3(kBlock), 0, s, S, snbr, seid, remap, tsum, snbr2,
+                       seid2, xs);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_cmp_sptr, dim3(blocks_for((uint64_t)nv + 1, kBlock)), dim3(kBlock), 0, s,
                        sptr, xs, sptr2, nv);

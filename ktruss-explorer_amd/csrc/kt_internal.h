@@ -121,12 +121,12 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s);
 // DELTA support maintenance (kt_delta.hip).
-// Mark pass over list[0, n) (list == nullptr: the entries 0..n-1).
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
-                      uint32_t* dl, const uint32_t* list, uint32_t n, uint32_t k, DevCounters* ctr,
-                      hipStream_t s);
-hipError_t min_live_list(const uint32_t* list, uint32_t n, const uint32_t* dst, const uint32_t* sup,
-                         DevCounters* ctr, hipStream_t s);
+                      uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
+                      const uint32_t* sb, DevCounters* ctr, hipStream_t s);
+// Minimum support of the live entries (those neither dead nor dying).
+hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
+                          hipStream_t s);
 // Scratch of a DELTA update for up to `cap` pending deletions.
 struct DeltaScratch {
     uint4* meta;        // per deletion: short list [x, y), long list [z, z + w)
@@ -143,23 +143,22 @@ hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
-                       const uint32_t* dst, const uint32_t* snbr, const uint32_t* seid,
-                       uint32_t* sup, const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+                       const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
+                       const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
 hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                             uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                             hipStream_t s);
-// Stable compactions of the DELTA state; tsum needs compact_tiles_for(n) + 1
-// words.  compact_live_list: out <- the live entries of list[0, n) (list ==
-// nullptr: the entries 0..n-1).  compact_adjacency: drops dead slots of the
-// S-slot adjacency into (sptr2, snbr2, seid2), S2 kept slots expected, and
-// rewrites sa / sb for the live entries; xs needs S + 1 words.
+// Stable compaction of the DELTA adjacency after a compaction of the oriented
+// layout: drops the dead slots of the S-slot lists into (sptr2, snbr2,
+// seid2), S2 kept slots expected, maps every kept slot's entry through remap
+// (old position -> new position of the oriented layout) and rewrites sa / sb
+// of the live entries from dst (the compacted layout).  tsum needs
+// compact_tiles_for(S) + 1 words, xs S + 1.
 uint32_t compact_tiles_for(uint32_t n);
-hipError_t compact_live_list(const uint32_t* list, uint32_t n, const uint32_t* dst, uint32_t* out,
-                             uint32_t* tsum, hipStream_t s);
 hipError_t compact_adjacency(uint32_t nv, uint32_t S, const uint32_t* sptr, const uint32_t* snbr,
-                             const uint32_t* seid, uint32_t* sptr2, uint32_t* snbr2, uint32_t* seid2,
-                             uint32_t S2, const uint32_t* dst, uint32_t* sa, uint32_t* sb,
-                             uint32_t* tsum, uint32_t* xs, hipStream_t s);
+                             const uint32_t* seid, const uint32_t* remap, uint32_t* sptr2,
+                             uint32_t* snbr2, uint32_t* seid2, uint32_t S2, const uint32_t* dst,
+                             uint32_t* sa, uint32_t* sb, uint32_t* tsum, uint32_t* xs, hipStream_t s);
 // Stable compaction of live entries into the *_out arrays and row-pointer
 // rebuild: tile_sums needs compact_tiles(E) + 1 entries, xs needs E + 1.
 uint32_t compact_tiles(uint32_t E);

@@ -204,9 +204,6 @@ int Plan::reset() {
         const int rc = restore_delta();
         if (rc) return rc;
     }
-    use_list = false;
-    lcur = 0;
-    L = E0;
     pending = 0;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     std::memset(h_ctr, 0, sizeof(DevCounters));
@@ -604,7 +601,7 @@ int Plan::ensure_delta() {
     const uint32_t ct = dev::compact_tiles_for(2 * E0) + 2;
     for (int b = 0; b < 2; ++b)
         if ((rc = sptr_b[b].alloc(Pb)) || (rc = snbr_b[b].alloc(2 * Eb)) ||
-            (rc = seid_b[b].alloc(2 * Eb)) || (rc = ll[b].alloc(Eb)))
+            (rc = seid_b[b].alloc(2 * Eb)))
             return rc;
     if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
         (rc = sa0.alloc(Eb)) || (rc = sb0.alloc(Eb)) || (rc = sa.alloc(Eb)) || (rc = sb.alloc(Eb)) ||
@@ -640,34 +637,27 @@ int Plan::restore_delta() {
     return KT_OK;
 }
 
-// With no deletion pending (every entry is live or dead), shrink what the
-// next passes walk: the mark's live list once fewer than half of its items
-// are alive, the adjacency lists once fewer than half of their slots are.
-// The new sizes are known on the host (live entries, two slots per live edge).
+// With no deletion pending (every entry is live or dead), drop the dead
+// entries once they are half of the stored ones: the oriented layout is
+// compacted (stable, rows stay sorted) and the adjacency lists with it, their
+// slot -> entry map following the entries to their new positions.  Live
+// entries and slots (two per live edge) are counted on the host, so the new
+// sizes need no read-back beyond the compaction's own check.
 int Plan::compact_delta_state() {
-    if (pending || live == 0) return KT_OK;
-    const uint32_t items = use_list ? L : E0;
-    if ((uint64_t)items > 2 * live && items > 1024) {
-        const int o = 1 - lcur;
-        KT_HIP_TRY(dev::compact_live_list(use_list ? ll[lcur].as<uint32_t>() : nullptr, items,
-                                          dst[cur].as<uint32_t>(), ll[o].as<uint32_t>(),
-                                          ctsum.as<uint32_t>(), stream));
-        lcur = o;
-        use_list = true;
-        L = (uint32_t)live;
-    }
-    const uint64_t keep = 2 * live;
-    if ((uint64_t)S_cur > 2 * keep && S_cur > 2048) {
-        const int o = 1 - acur;
-        KT_HIP_TRY(dev::compact_adjacency(n, S_cur, sptr_b[acur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
-                                          seid_b[acur].as<uint32_t>(), sptr_b[o].as<uint32_t>(),
-                                          snbr_b[o].as<uint32_t>(), seid_b[o].as<uint32_t>(),
-                                          (uint32_t)keep, dst[cur].as<uint32_t>(), sa.as<uint32_t>(),
-                                          sb.as<uint32_t>(), ctsum.as<uint32_t>(), cxs.as<uint32_t>(),
-                                          stream));
-        acur = o;
-        S_cur = (uint32_t)keep;
-    }
+    if (pending || live == 0 || (uint64_t)E <= 2 * live || E <= 1024) return KT_OK;
+    const uint32_t S = S_cur;
+    int rc = compact_now();  // oriented layout: xs[old position] = new position
+    if (rc) return rc;
+    const int o = 1 - acur;
+    const uint32_t keep = 2 * (uint32_t)live;
+    KT_HIP_TRY(dev::compact_adjacency(n, S, sptr_b[acur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
+                                      seid_b[acur].as<uint32_t>(), xs.as<uint32_t>(),
+                                      sptr_b[o].as<uint32_t>(), snbr_b[o].as<uint32_t>(),
+                                      seid_b[o].as<uint32_t>(), keep, dst[cur].as<uint32_t>(),
+                                      sa.as<uint32_t>(), sb.as<uint32_t>(), ctsum.as<uint32_t>(),
+                                      cxs.as<uint32_t>(), stream));
+    acur = o;
+    S_cur = keep;
     return KT_OK;
 }
 
@@ -684,7 +674,7 @@ int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
                                       dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws,
                                       ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c0, stream));
-        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
+        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
                                     ctr.as<DevCounters>(), stream));
         KT_HIP_TRY(hipEventRecord(ev_c1, stream));
@@ -744,9 +734,9 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
             KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
                                           dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws,
                                           ctr.as<DevCounters>(), stream));
-            KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
-                                        snbr_b[acur].as<uint32_t>(), seid_b[acur].as<uint32_t>(),
-                                        sup[cur].as<uint32_t>(), dws, ctr.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, snbr_b[acur].as<uint32_t>(),
+                                        seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
+                                        ctr.as<DevCounters>(), stream));
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
         }
     }
@@ -774,7 +764,8 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
 
 // One phase at k with DELTA available: the mark pass defers its deletions
 // (KT_DYING + list), and the support pass that opens the next iteration
-// either removes their triangles (DELTA) or recounts.  Entries never move.
+// either removes their triangles (DELTA) or recounts.  The dead entries are
+// compacted away once they are half of the stored ones.
 int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     const bool jump = (schedule & KT_SCHED_JUMP) != 0;
     uint32_t iter = 0;
@@ -815,20 +806,19 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         }
         if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
         if ((rc = compact_delta_state())) return rc;
-        const uint32_t* list = use_list ? ll[lcur].as<uint32_t>() : nullptr;
-        const uint32_t items = use_list ? L : E;
         if (iter == 0 && jump) {
             if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
-            KT_HIP_TRY(dev::min_live_list(list, items, dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
-                                          ctr.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::min_live_delta(E, dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
+                                           ctr.as<DevCounters>(), stream));
             if ((rc = sync_counters())) return rc;
             if (unresolved) resolve_choice();
             const uint32_t s = h_ctr->min_sup;
             if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
         }
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
-                                   sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), list,
-                                   items, k, ctr.as<DevCounters>(), stream));
+                                   sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
+                                   snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                   ctr.as<DevCounters>(), stream));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;

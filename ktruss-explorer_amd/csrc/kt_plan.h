@@ -79,12 +79,6 @@ struct Plan {
     DBuf sptr0, snbr0, seid0, sa0, sb0;
     int acur = 0;
     uint32_t S_cur = 0;
-    // live list walked by the mark pass (identity over [0, E0) until the
-    // first compaction)
-    DBuf ll[2];
-    int lcur = 0;
-    bool use_list = false;
-    uint32_t L = 0;
     DBuf dl, sup_snap, dscratch, ctsum, cxs;
     dev::DeltaScratch dws{};
     bool delta_ready = false;
