@@ -280,6 +280,7 @@ int Batch::reset() {
         KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, 2 * (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
     KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E, stream));
     KT_HIP_TRY(dev::fill_u32(bc.as<uint32_t>(), 0, sizeof(BatchCounters) / 4, stream));
+    if (delta) KT_HIP_TRY(dev::delta_stats_reset(dws, stream));
     std::memset(h_bc, 0, sizeof(BatchCounters));
     have_tau = false;
     stats = kt_batch_stats{};
@@ -421,6 +422,10 @@ int Batch::decompose() {
             return fail(KT_ERR_STATE, "batched peel exceeded " + std::to_string(step_bound) + " steps");
     }
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
+    if (delta) {
+        KT_HIP_TRY(dev::delta_stats_fold(dws, &bd.bc->count, stream));
+        KT_HIP_TRY(hipMemcpyAsync(h_bc, bc.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, stream));
+    }
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));

@@ -233,6 +233,8 @@ __global__ __launch_bounds__(kBlock) void k_dscan_local(const uint32_t* __restri
 // other two edges survive (the common case) costs two slot->entry reads and
 // two atomic decrements.
 constexpr uint32_t kStage = 512;
+constexpr uint32_t kDeltaGrid = 2048;  // most k_delta workgroups per launch
+constexpr size_t kStatBytes = (size_t)kDeltaGrid * 4 * 8;
 constexpr uint32_t kSample = 512;
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -240,6 +242,36 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Staging of up to N entries src[q * stride] (q < n) into a wave's LDS slice.
+// Every lane issues its N / 64 loads before the first one is waited for, so a
+// staging costs one memory round trip, not one per 64 entries.  The loads are
+// unconditional (indices past the end repeat the last entry): a load under a
+// lane condition makes the compiler wait for it inside the condition.
+template <uint32_t N>
+struct Staged {
+    uint32_t r[N / kWave];
+    __device__ __forceinline__ void load(const uint32_t* src, uint32_t n, uint32_t stride,
+                                         uint32_t lane) {
+        if (n == 0) return;
+        const uint32_t last = n - 1;
+#pragma unroll
+        for (uint32_t k = 0; k < N / kWave; ++k) {
+            const uint32_t q = lane + k * kWave;
+            r[k] = src[(size_t)(q < last ? q : last) * stride];
+        }
+    }
+    __device__ __forceinline__ void store(uint32_t* buf, uint32_t n, uint32_t mask,
+                                          uint32_t lane) const {
+        wave_lds_sync();
+#pragma unroll
+        for (uint32_t k = 0; k < N / kWave; ++k) {
+            const uint32_t q = lane + k * kWave;
+            if (q < n) buf[q] = r[k] & mask;
+        }
+        wave_lds_sync();
+    }
+};
 
 // First index in [lo, lo + len) whose id (flags masked) is >= x; the trip
 // count depends on len only.
@@ -302,7 +334,8 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
                                                   const uint32_t* __restrict__ toff,
                                                   const uint32_t* __restrict__ snbr,
                                                   const uint32_t* __restrict__ seid,
-                                                  uint32_t* sup, DevCounters* ctr) {
+                                                  uint32_t* sup, DevCounters* ctr,
+                                                  unsigned long long* __restrict__ bstat) {
     __shared__ uint32_t s_buf[(kBlock / kWave) * (kStage + kSample)];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample);
@@ -310,8 +343,9 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
     const uint32_t nd = *nd_dev;
     const uint32_t T = ctr->dtasks;
     const uint32_t W = nwaves(), w = gwave();
-    const uint32_t t0 = (uint32_t)((uint64_t)T * w / W), t1 = (uint32_t)((uint64_t)T * (w + 1) / W);
-    if (t0 >= t1 || nd == 0) return;
+    const uint32_t t0 = (uint32_t)((uint64_t)T * w / W);
+    const uint32_t t1 = nd ? (uint32_t)((uint64_t)T * (w + 1) / W) : t0;  // no early return:
+                                                                          // the block reduces below
     // last deletion whose first task is <= t0
     uint32_t i = 0, len = nd;
     while (len > 1) {
@@ -329,21 +363,24 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         const uint32_t j0 = m.x + (tc - toff[i]) * kWave;
         const uint32_t j = j0 + lane;
         acc_elems += min(kWave, m.y - j0);
+        // a longer list that fits the slice is staged with the candidates'
+        // load in flight
+        const bool fresh = m.w <= kStage && full != i;
+        Staged<kStage> st;
+        if (fresh) st.load(snbr + m.z, m.w, 1u, lane);
         const uint32_t x = j < m.y ? snbr[j] : KT_DEAD;
         const bool cand = !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
         acc_live += (uint32_t)__popcll(cm);
+        if (fresh) {
+            st.store(buf, m.w, 0xFFFFFFFFu, lane);
+            full = i;
+            acc_win += m.w;
+        }
         if (!cm) continue;
         const uint32_t xv = x & KT_IDMASK;
         uint32_t p, y;
         if (m.w <= kStage) {
-            if (full != i) {
-                wave_lds_sync();
-                for (uint32_t q = lane; q < m.w; q += kWave) buf[q] = snbr[m.z + q];
-                wave_lds_sync();
-                full = i;
-                acc_win += m.w;
-            }
             p = m.z + search_staged(buf, m.w, xv, y);
         } else {
             // candidates ascend with the lane
@@ -353,9 +390,9 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
             const uint32_t nb = (m.w + kWave - 1) / kWave;
             if (nb <= kSample) {
                 if (sampled != i) {
-                    wave_lds_sync();
-                    for (uint32_t q = lane; q < nb; q += kWave) smp[q] = snbr[m.z + q * kWave] & KT_IDMASK;
-                    wave_lds_sync();
+                    Staged<kSample> ss;
+                    ss.load(snbr + m.z, nb, kWave, lane);
+                    ss.store(smp, nb, KT_IDMASK, lane);
                     sampled = i;
                     acc_win += nb;
                 }
@@ -372,9 +409,9 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
             if (wl <= kStage) {
                 full = 0xFFFFFFFFu;  // buf now holds a window
                 acc_win += wl;
-                wave_lds_sync();
-                for (uint32_t q = lane; q < wl; q += kWave) buf[q] = snbr[lo + q];
-                wave_lds_sync();
+                Staged<kStage> sw;
+                sw.load(snbr + lo, wl, 1u, lane);
+                sw.store(buf, wl, 0xFFFFFFFFu, lane);
                 p = lo + search_staged(buf, wl, xv, y);
             } else {
                 acc_win += (uint64_t)__popcll(cm) * (32u - __builtin_clz(wl));
@@ -401,11 +438,47 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         }
         acc_tri += (uint32_t)__popcll(ballot(tri));
     }
+    // statistics: one slot per workgroup, updated in place (a launch has at
+    // most kDeltaGrid workgroups and launches on one stream do not overlap);
+    // counters every wave bumps with an atomic serialize on their one address
+    // and hold back the whole launch
+    __shared__ unsigned long long s_stat[kBlock / kWave][4];
     if (lane == 0) {
-        atomicAdd(&ctr->delta_elems, acc_elems);
-        if (acc_win) atomicAdd(&ctr->delta_win, acc_win);
-        if (acc_live) atomicAdd(&ctr->delta_live, acc_live);
-        if (acc_tri) atomicAdd(&ctr->delta_tris, acc_tri);
+        unsigned long long* q = s_stat[threadIdx.x / kWave];
+        q[0] = acc_elems;
+        q[1] = acc_win;
+        q[2] = acc_live;
+        q[3] = acc_tri;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long v = 0;
+        for (uint32_t k = 0; k < kBlock / kWave; ++k) v += s_stat[k][threadIdx.x];
+        if (v) bstat[blockIdx.x * 4 + threadIdx.x] += v;
+    }
+}
+
+// Adds the per-workgroup statistics to the counters and clears the slots.
+__global__ __launch_bounds__(256) void k_delta_fold(unsigned long long* __restrict__ bstat,
+                                                    DevCounters* ctr) {
+    __shared__ unsigned long long s[256];
+    const uint32_t c = threadIdx.x & 3;
+    unsigned long long v = 0;
+    for (uint32_t b = threadIdx.x >> 2; b < kDeltaGrid; b += 64) {
+        v += bstat[b * 4 + c];
+        bstat[b * 4 + c] = 0;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 128; off >= 4; off >>= 1) {  // offsets keep the stat index t & 3
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctr->delta_elems += s[0];
+        ctr->delta_win += s[1];
+        ctr->delta_live += s[2];
+        ctr->delta_tris += s[3];
     }
 }
 
@@ -558,12 +631,13 @@ hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, 
 
 size_t delta_scratch_bytes(uint32_t cap) {
     const size_t c = cap ? cap : 1;
-    return 16 * c + 4 * c + 4 * (c / kTileD + 2) + 256;
+    return kStatBytes + 16 * c + 4 * c + 4 * (c / kTileD + 2) + 256;
 }
 
 void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
     const size_t c = cap ? cap : 1;
-    char* p = (char*)base;
+    s.bstat = (unsigned long long*)base;
+    char* p = (char*)base + kStatBytes;
     s.meta = (uint4*)p;
     s.ntask = (uint32_t*)(p + 16 * c);
     s.tsum = (uint32_t*)(p + 20 * c);
@@ -590,9 +664,18 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
     if (!nd_max) return hipSuccess;
-    const uint32_t db = nd_max < 64 ? 64 : (nd_max > 2048 ? 2048 : nd_max);
+    const uint32_t db = nd_max < 64 ? 64 : (nd_max > kDeltaGrid ? kDeltaGrid : nd_max);
     hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, snbr,
-                       seid, sup, ctr);
+                       seid, sup, ctr, ws.bstat);
+    return hipGetLastError();
+}
+
+hipError_t delta_stats_reset(const DeltaScratch& ws, hipStream_t s) {
+    return fill_u32((uint32_t*)ws.bstat, 0, kStatBytes / 4, s);
+}
+
+hipError_t delta_stats_fold(const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_delta_fold, dim3(1), dim3(256), 0, s, ws.bstat, ctr);
     return hipGetLastError();
 }
 

@@ -129,6 +129,7 @@ hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, 
                           hipStream_t s);
 // Scratch of a DELTA update for up to `cap` pending deletions.
 struct DeltaScratch {
+    unsigned long long* bstat;  // per k_delta workgroup: elems, win, live, tris
     uint4* meta;        // per deletion: short list [x, y), long list [z, z + w)
     uint32_t* ntask;    // per deletion: 64-candidate tasks, then their exclusive scan
     uint32_t* tsum;     // per 2048-deletion tile, plus one
@@ -142,6 +143,11 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+// The update's statistics are summed per workgroup slot; delta_stats_fold
+// adds them to ctr->delta_* and clears the slots, delta_stats_reset clears
+// them (neither may be captured in a graph that replays before stats are read).
+hipError_t delta_stats_reset(const DeltaScratch& ws, hipStream_t s);
+hipError_t delta_stats_fold(const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);

@@ -176,7 +176,8 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     return KT_OK;
 }
 
-int Plan::sync_counters() {
+int Plan::sync_counters(bool fold) {
+    if (fold && delta_ready) KT_HIP_TRY(dev::delta_stats_fold(dws, ctr.as<DevCounters>(), stream));
     KT_HIP_TRY(hipMemcpyAsync(h_ctr, ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
     return KT_OK;
@@ -501,7 +502,7 @@ int Plan::decompose() {
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
-    if ((rc = sync_counters())) return rc;
+    if ((rc = sync_counters(true))) return rc;
     stats.ms_total = ms;
     finish_stats();
     have_tau = true;
@@ -540,7 +541,7 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
-    if ((rc = sync_counters())) return rc;
+    if ((rc = sync_counters(true))) return rc;
     stats.ms_total = ms;
     finish_stats();
     std::vector<uint32_t> s(E), d(E);
@@ -634,6 +635,7 @@ int Plan::restore_delta() {
         KT_HIP_TRY(hipMemcpyAsync(sa.p, sa0.p, Eb, hipMemcpyDeviceToDevice, stream));
         KT_HIP_TRY(hipMemcpyAsync(sb.p, sb0.p, Eb, hipMemcpyDeviceToDevice, stream));
     }
+    KT_HIP_TRY(dev::delta_stats_reset(dws, stream));
     return KT_OK;
 }
 

@@ -110,7 +110,9 @@ struct Plan {
     // counter read-back) instead of synchronising for it now
     int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer = false);
     int compact_now();
-    int sync_counters();
+    // Reads the device counters; fold: add the DELTA update's per-workgroup
+    // statistics first (once a run's last update is done).
+    int sync_counters(bool fold = false);
     int set_counter(size_t offset, uint32_t value);
     CountArgs count_args(uint32_t k, bool use_wl, uint32_t nitems, const CountCfg& cfg,
                          DevCounters* counters);
