@@ -45,44 +45,79 @@ __device__ __forceinline__ uint32_t gwave() {
 }
 __device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / kWave; }
 
-// Wave-aggregated append to a device counter; returns this lane's slot.
-__device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, uint32_t lane) {
-    const uint64_t m = ballot(pred);
-    if (!m) return 0;
-    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(ctr, (unsigned int)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77), deferred: a deleted entry
 // is flagged KT_DYING and listed; it becomes KT_DEAD in k_finalize.
 // The deleted entry's two adjacency slots get KT_DYING as well.
-__global__ void k_mark_dying(uint32_t* __restrict__ dst, const uint32_t* __restrict__ eid,
-                             const uint32_t* __restrict__ sup, uint32_t* __restrict__ tau,
-                             uint32_t* __restrict__ dl, uint32_t n, uint32_t k,
-                             uint32_t* __restrict__ snbr, const uint32_t* __restrict__ sa,
-                             const uint32_t* __restrict__ sb, DevCounters* ctr) {
+// A workgroup walks tiles of kMarkTile entries (every lane loads all of its
+// entries and supports before the first wait) and lists its deletions in
+// LDS; the global list is reserved with one atomic per workgroup when the
+// LDS list is full and at the end.  A reservation per wave would put one
+// returning atomic per wave on a single address, served one at a time.
+constexpr uint32_t kMarkItems = 8;
+constexpr uint32_t kMarkTile = kBlock * kMarkItems;
+constexpr uint32_t kMarkBuf = 2 * kMarkTile;
+constexpr uint32_t kMarkGrid = 1024;
+
+__device__ __forceinline__ void mark_flush(const uint32_t* s_out, uint32_t* s_cnt, uint32_t* s_base,
+                                           uint32_t* dl, unsigned int* ndel) {
+    const uint32_t c = *s_cnt;  // read by every thread after a barrier: uniform
+    if (!c) return;
+    if (threadIdx.x == 0) *s_base = atomicAdd(ndel, c);
+    __syncthreads();
+    const uint32_t base = *s_base;
+    for (uint32_t i = threadIdx.x; i < c; i += kBlock) dl[base + i] = s_out[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s_cnt = 0;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ dst,
+                                                       const uint32_t* __restrict__ eid,
+                                                       const uint32_t* __restrict__ sup,
+                                                       uint32_t* __restrict__ tau,
+                                                       uint32_t* __restrict__ dl, uint32_t n, uint32_t k,
+                                                       uint32_t* __restrict__ snbr,
+                                                       const uint32_t* __restrict__ sa,
+                                                       const uint32_t* __restrict__ sb, DevCounters* ctr) {
+    __shared__ uint32_t s_out[kMarkBuf];
+    __shared__ uint32_t s_cnt, s_base;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t e = base + threadIdx.x;
-        bool del = false;
-        uint32_t v = 0;
-        if (e < n) {
-            v = dst[e];
-            del = !(v & (KT_DEAD | KT_DYING)) && sup[e] < k - 2;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t ntile = (n + kMarkTile - 1) / kMarkTile;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint32_t e0 = tile * kMarkTile + threadIdx.x;
+        uint32_t v[kMarkItems], sp[kMarkItems];
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkItems; ++j) {
+            const uint32_t e = e0 + j * kBlock;
+            v[j] = e < n ? dst[e] : KT_DEAD;
+            sp[j] = e < n ? sup[e] : 0u;
         }
-        const uint32_t pos = wave_append(&ctr->ndel, del, lane);
-        if (del) {
-            dst[e] = v | KT_DYING;
-            tau[eid[e]] = k - 1;
-            dl[pos] = e;
-            const uint32_t a = sa[e], b = sb[e];
-            snbr[a] |= KT_DYING;
-            snbr[b] |= KT_DYING;
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkItems; ++j) {
+            const uint32_t e = e0 + j * kBlock;
+            const bool del = !(v[j] & (KT_DEAD | KT_DYING)) && sp[j] < k - 2;
+            const uint64_t m = ballot(del);
+            if (!m) continue;
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(&s_cnt, (uint32_t)__popcll(m));  // LDS
+            b = __shfl(b, 0);
+            if (del) {
+                s_out[b + (uint32_t)__popcll(m & below)] = e;
+                dst[e] = v[j] | KT_DYING;
+                tau[eid[e]] = k - 1;
+                const uint32_t a = sa[e], c = sb[e];
+                snbr[a] |= KT_DYING;
+                snbr[c] |= KT_DYING;
+            }
         }
+        __syncthreads();
+        if (s_cnt > kMarkBuf - kMarkTile) mark_flush(s_out, &s_cnt, &s_base, dl, &ctr->ndel);
     }
+    __syncthreads();
+    mark_flush(s_out, &s_cnt, &s_base, dl, &ctr->ndel);
 }
 
 // Minimum live support (the JUMP target); one atomic per workgroup.
@@ -616,8 +651,9 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, u
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, DevCounters* ctr, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_mark_dying, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, dst, eid, sup,
-                       tau, dl, n, k, snbr, sa, sb, ctr);
+    const uint32_t nt = (n + kMarkTile - 1) / kMarkTile;
+    hipLaunchKernelGGL(k_mark_dying, dim3(nt < kMarkGrid ? nt : kMarkGrid), dim3(kBlock), 0, s, dst,
+                       eid, sup, tau, dl, n, k, snbr, sa, sb, ctr);
     return hipGetLastError();
 }
 
