@@ -52,29 +52,36 @@ __global__ void k_b_zero(TaskList tl, const uint32_t* __restrict__ dst, uint32_t
     }
 }
 
-// Wave-aggregated append to a device counter; returns this lane's slot.
-__device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, uint32_t lane) {
-    const uint64_t m = ballot(pred);
-    if (!m) return 0;
-    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(ctr, (unsigned int)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77) with the graph's own k; the
 // minimum support of the entries that stay alive is folded per graph, which
 // is the JUMP target whenever the graph's phase ends with this step.  DELTA:
-// a deleted entry is flagged KT_DYING and appended to dl (count *dn) for the
-// support update that follows.
+// a deleted entry is flagged KT_DYING and listed in dl (count *dn) for the
+// support update that follows.  Each wave lists its deletions in its own LDS
+// slice and the workgroup reserves room in dl with one atomic at the end (a
+// wave whose slice fills reserves for it alone): a reservation per 64-entry
+// chunk would put one returning atomic per chunk on a single address.
+constexpr uint32_t kBMarkCap = 1024;  // LDS list entries per wave
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool DELTA>
-__global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
-                         uint32_t* __restrict__ tau, const uint32_t* __restrict__ gk,
-                         uint32_t* gndel, uint32_t* gmin, uint32_t* __restrict__ dl,
-                         unsigned int* dn, uint32_t* __restrict__ snbr,
-                         const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
+__global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restrict__ dst,
+                                                const uint32_t* __restrict__ sup,
+                                                uint32_t* __restrict__ tau, const uint32_t* __restrict__ gk,
+                                                uint32_t* gndel, uint32_t* gmin, uint32_t* __restrict__ dl,
+                                                unsigned int* dn, uint32_t* __restrict__ snbr,
+                                                const uint32_t* __restrict__ sa,
+                                                const uint32_t* __restrict__ sb) {
+    __shared__ uint32_t s_list[DELTA ? (256 / kWave) * kBMarkCap : 1];
+    __shared__ uint32_t s_n[256 / kWave], s_base;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t* my = s_list + (DELTA ? wid * kBMarkCap : 0);
+    uint32_t cnt = 0;  // wave-uniform length of the wave's LDS list
     const uint32_t n = *tl.ntasks;
     for (uint32_t t = gwave(); t < n; t += nwaves()) {
         const uint4 d = tl.desc[t];
@@ -97,22 +104,48 @@ __global__ void k_b_mark(TaskList tl, uint32_t* __restrict__ dst, const uint32_t
                     }
                 }
             }
-            if (DELTA) {
-                const uint32_t pos = wave_append(dn, del, lane);
+            const uint64_t m = ballot(del);
+            if (DELTA && m) {
+                if (cnt + kWave > kBMarkCap) {  // the slice is full: reserve for it alone
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(dn, cnt);
+                    b = __shfl(b, 0);
+                    wave_sync_lds();
+                    for (uint32_t q = lane; q < cnt; q += kWave) dl[b + q] = my[q];
+                    wave_sync_lds();
+                    cnt = 0;
+                }
                 if (del) {
-                    dl[pos] = i;
+                    my[cnt + (uint32_t)__popcll(m & below)] = i;
                     const uint32_t a = sa[i], b = sb[i];
                     snbr[a] |= KT_DYING;  // the deleted edge's adjacency slots
                     snbr[b] |= KT_DYING;
                 }
+                cnt += (uint32_t)__popcll(m);
             }
-            ndel += (uint32_t)__popcll(ballot(del));
+            ndel += (uint32_t)__popcll(m);
         }
         smin = wave_min(smin);
         if (lane == 0) {
             if (ndel) atomicAdd(gndel + d.z, ndel);
             if (smin != 0xFFFFFFFFu) atomicMin(gmin + d.z, smin);
         }
+    }
+    if (DELTA) {
+        if (lane == 0) s_n[wid] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t w = 0; w < 256 / kWave; ++w) {
+                const uint32_t c = s_n[w];
+                s_n[w] = tot;
+                tot += c;
+            }
+            s_base = tot ? atomicAdd(dn, tot) : 0u;
+        }
+        __syncthreads();
+        const uint32_t b = s_base + s_n[wid];
+        for (uint32_t q = lane; q < cnt; q += kWave) dl[b + q] = my[q];
     }
 }
 

@@ -171,7 +171,8 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
         (rc = d_gbeg.alloc(((size_t)G + 1) * 4)) ||
         (rc = gstate.alloc((size_t)kStateArrays * G * 4)) ||
         (rc = lists.alloc(4 * ((size_t)G + 1) * 4)) || (rc = ddesc.alloc(maxtasks * 16)) ||
-        (rc = adesc.alloc(maxtasks * 16)) || (rc = bc.alloc(sizeof(BatchCounters))))
+        (rc = adesc.alloc(maxtasks * 16)) || (rc = bc.alloc(sizeof(BatchCounters))) ||
+        (rc = stripes.alloc(kStripeBytes)))
         return rc;
     KT_HIP_TRY(hipMemcpyAsync(ptr.p, hptr.data(), ((size_t)N + 1) * 4, hipMemcpyHostToDevice, stream));
     if (E) {
@@ -280,6 +281,7 @@ int Batch::reset() {
         KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, 2 * (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
     KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E, stream));
     KT_HIP_TRY(dev::fill_u32(bc.as<uint32_t>(), 0, sizeof(BatchCounters) / 4, stream));
+    KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     if (delta) KT_HIP_TRY(dev::delta_stats_reset(dws, stream));
     std::memset(h_bc, 0, sizeof(BatchCounters));
     have_tau = false;
@@ -310,6 +312,7 @@ int Batch::prestep(hipEvent_t c0, hipEvent_t c1) {
     a.cap_lds = cap;
     a.ytiles = 1;
     a.ctr = &bd.bc->count;
+    a.stripes = stripes.as<unsigned long long>();
     KT_HIP_TRY(dev::batch_control(bd, 0, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_zero(dtl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
@@ -334,6 +337,7 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
     a.cap_lds = cap;
     a.ytiles = 1;
     a.ctr = &bd.bc->count;
+    a.stripes = stripes.as<unsigned long long>();
     if (delta) {
         // mark -> DELTA update of this step's deletions -> finalize -> control -> expand
         unsigned int* dn = &bd.bc->count.ndel;
@@ -422,10 +426,9 @@ int Batch::decompose() {
             return fail(KT_ERR_STATE, "batched peel exceeded " + std::to_string(step_bound) + " steps");
     }
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
-    if (delta) {
-        KT_HIP_TRY(dev::delta_stats_fold(dws, &bd.bc->count, stream));
-        KT_HIP_TRY(hipMemcpyAsync(h_bc, bc.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, stream));
-    }
+    KT_HIP_TRY(dev::stripes_fold(stripes.as<unsigned long long>(), &bd.bc->count, stream));
+    if (delta) KT_HIP_TRY(dev::delta_stats_fold(dws, &bd.bc->count, stream));
+    KT_HIP_TRY(hipMemcpyAsync(h_bc, bc.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, stream));
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));

@@ -20,6 +20,7 @@ struct Batch {
 
     DBuf ptr, src, dst0, dst, sup, tau;
     DBuf d_gbeg, gstate, lists, ddesc, adesc, bc;
+    DBuf stripes;  // count statistics per stripe line (folded into bc->count at the end)
     bool delta = false;                               // DELTA support maintenance
     DBuf sptr, snbr0, snbr, seid, sa, sb, dl, dscratch;
     dev::DeltaScratch dws{};

@@ -296,6 +296,18 @@ __device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r,
     return c < cap ? c : cap;
 }
 
+
+// Adds a wave's statistics to its stripe line (one line per wave index modulo
+// kStripes, so the atomics of a launch are spread over kStripes addresses).
+__device__ __forceinline__ void stripe_stats(const CountArgs& a, unsigned long long merge,
+                                             unsigned long long rec) {
+    const uint32_t w = ((blockIdx.y * gridDim.x + blockIdx.x) * (blockDim.x / kWave) +
+                        threadIdx.x / kWave) % kStripes;
+    unsigned long long* q = a.stripes + (size_t)w * kStripeWords;
+    atomicAdd(q, merge);
+    atomicAdd(q + 1, rec);
+}
+
 template <bool DIR, bool TILED>
 __global__ __launch_bounds__(1024) void k_count(CountArgs a) {
     extern __shared__ uint32_t lds[];
@@ -379,10 +391,7 @@ __global__ __launch_bounds__(1024) void k_count(CountArgs a) {
         }
     }
     flush_row<DIR>(r, s_cnt, a.sup, lane);
-    if (lane == 0 && acc_rec) {
-        atomicAdd(&a.ctr->recount_edges, acc_rec);
-        atomicAdd(&a.ctr->merge_elems, acc_merge);
-    }
+    if (lane == 0 && acc_rec) stripe_stats(a, acc_merge, acc_rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -676,10 +685,7 @@ __global__ __launch_bounds__(1024) void k_count_flat(CountArgs a) {
         acc_rec += __shfl_xor(acc_rec, off);
         acc_merge += __shfl_xor(acc_merge, off);
     }
-    if (lane == 0 && acc_rec) {
-        atomicAdd(&a.ctr->recount_edges, acc_rec);
-        atomicAdd(&a.ctr->merge_elems, acc_merge);
-    }
+    if (lane == 0 && acc_rec) stripe_stats(a, acc_merge, acc_rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -849,10 +855,7 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a, TaskList tl) {
         acc_rec += __shfl_xor(acc_rec, off);
         acc_merge += __shfl_xor(acc_merge, off);
     }
-    if (lane == 0 && acc_rec) {
-        atomicAdd(&a.ctr->recount_edges, acc_rec);
-        atomicAdd(&a.ctr->merge_elems, acc_merge);
-    }
+    if (lane == 0 && acc_rec) stripe_stats(a, acc_merge, acc_rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -868,6 +871,34 @@ uint32_t grid_for(uint64_t count) {
     if (b > 65536) b = 65536;
     if (b == 0) b = 1;
     return (uint32_t)b;
+}
+
+// Adds the stripe lines to the counters and clears them (one workgroup).
+__global__ __launch_bounds__(256) void k_stripes_fold(unsigned long long* __restrict__ st,
+                                                      DevCounters* ctr) {
+    __shared__ unsigned long long s_m[256], s_r[256];
+    unsigned long long m = 0, r = 0;
+    for (uint32_t l = threadIdx.x; l < kStripes; l += 256) {
+        unsigned long long* q = st + (size_t)l * kStripeWords;
+        m += q[0];
+        r += q[1];
+        q[0] = 0;
+        q[1] = 0;
+    }
+    s_m[threadIdx.x] = m;
+    s_r[threadIdx.x] = r;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s_m[threadIdx.x] += s_m[threadIdx.x + off];
+            s_r[threadIdx.x] += s_r[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctr->merge_elems += s_m[0];
+        ctr->recount_edges += s_r[0];
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t value, uint64_t count) {
@@ -893,48 +924,68 @@ __global__ void k_zero_listed(const uint32_t* __restrict__ wl, uint32_t n, uint3
     for (uint32_t i = gtid(); i < n; i += gsize()) sup[wl[i]] = 0;
 }
 
-// Wave-aggregated increment of a device counter; returns the wave's base.
-__device__ __forceinline__ uint32_t wave_append(unsigned int* ctr, bool pred, uint32_t lane) {
-    const uint64_t m = ballot(pred);
-    if (!m) return 0;
-    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(ctr, (unsigned int)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+// Grids of the list-building passes below: a grid-stride loop over at most
+// kListGrid workgroups, each of which reserves room in the global list (or
+// adds to the global count) once per flush of its LDS list, not once per
+// wave: atomics of every wave on one address are served one at a time.
+constexpr uint32_t kListGrid = 1024;
+constexpr uint32_t kListCap = 2048;  // LDS list entries per workgroup
+
+uint32_t list_grid(uint32_t items) {
+    const uint32_t b = (items + kBlock - 1) / kBlock;
+    return b < kListGrid ? (b ? b : 1u) : kListGrid;
+}
+
+// Copies the workgroup's LDS list to the global list at a reserved offset
+// and empties it.  Every thread calls it after a barrier.
+__device__ __forceinline__ void list_flush(const uint32_t* s_out, uint32_t* s_cnt, uint32_t* s_base,
+                                           uint32_t* out, unsigned int* total) {
+    const uint32_t c = *s_cnt;
+    if (!c) return;
+    if (threadIdx.x == 0) *s_base = atomicAdd(total, c);
+    __syncthreads();
+    const uint32_t base = *s_base;
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) out[base + i] = s_out[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s_cnt = 0;
+    __syncthreads();
 }
 
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77 and its three twins): an
-// entry whose support is below k-2 dies; its truss number is k-1.
-__global__ void k_mark_deleted(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
-                               const uint32_t* __restrict__ eid, const uint32_t* __restrict__ sup,
-                               const uint8_t* __restrict__ state, uint8_t* __restrict__ nflag,
-                               uint32_t* __restrict__ tau, const uint32_t* __restrict__ wl,
-                               uint32_t nitems, uint32_t k, int affected, DevCounters* ctr) {
+// entry whose support is below k-2 dies; its truss number is k-1.  The
+// deletions are counted per workgroup in LDS and added once.
+__global__ __launch_bounds__(kBlock) void k_mark_deleted(uint32_t* __restrict__ dst,
+                                                         const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ eid,
+                                                         const uint32_t* __restrict__ sup,
+                                                         const uint8_t* __restrict__ state,
+                                                         uint8_t* __restrict__ nflag,
+                                                         uint32_t* __restrict__ tau,
+                                                         const uint32_t* __restrict__ wl,
+                                                         uint32_t nitems, uint32_t k, int affected,
+                                                         DevCounters* ctr) {
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t stride = gsize();
-    // uniform trip count so every lane reaches the ballots
-    for (uint32_t base = blockIdx.x * blockDim.x; base < nitems; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        bool del = false;
-        if (i < nitems) {
-            const uint32_t e = wl ? wl[i] : i;
-            const uint32_t v = dst[e];
-            if (!(v & KT_DEAD) && (!affected || state[e] == KT_DIRECTLY_AFFECTED) &&
-                sup[e] < k - 2) {
-                del = true;
-                dst[e] = v | KT_DEAD;
-                tau[eid[e]] = k - 1;
-                if (affected) {
-                    nflag[src[e]] = KT_DIRECTLY_AFFECTED;
-                    nflag[v] = KT_DIRECTLY_AFFECTED;
-                }
+    uint32_t mine = 0;
+    for (uint32_t i = gtid(); i < nitems; i += gsize()) {
+        const uint32_t e = wl ? wl[i] : i;
+        const uint32_t v = dst[e];
+        if (!(v & KT_DEAD) && (!affected || state[e] == KT_DIRECTLY_AFFECTED) && sup[e] < k - 2) {
+            ++mine;
+            dst[e] = v | KT_DEAD;
+            tau[eid[e]] = k - 1;
+            if (affected) {
+                nflag[src[e]] = KT_DIRECTLY_AFFECTED;
+                nflag[v] = KT_DIRECTLY_AFFECTED;
             }
         }
-        const uint64_t m = ballot(del);
-        if (m && lane == (uint32_t)__ffsll((long long)m) - 1)
-            atomicAdd(&ctr->ndel, (unsigned int)__popcll(m));
     }
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if (lane == 0 && mine) atomicAdd(&s_cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_cnt) atomicAdd(&ctr->ndel, s_cnt);
 }
 
 // mark_directly_affected_edges_kernel (kernel_ddc.cu:79-98, kernel_duc.cu:79-93).
@@ -963,12 +1014,22 @@ __global__ void k_mark_directly(const uint32_t* __restrict__ src, const uint32_t
 
 // mark_indirectly_affected_edges_kernel (kernel_ddc.cu:100-114) fused with the
 // worklist build: every DIRECTLY / INDIRECTLY live entry is appended and its
-// support zeroed before the next count.
-__global__ void k_build_worklist(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-                                 uint8_t* __restrict__ state, const uint8_t* __restrict__ nflag,
-                                 uint32_t* __restrict__ sup, uint32_t* __restrict__ wl, uint32_t E,
-                                 int directed, DevCounters* ctr) {
+// support zeroed before the next count.  The workgroup lists its entries in
+// LDS and reserves room in the worklist once per flush.
+__global__ __launch_bounds__(kBlock) void k_build_worklist(const uint32_t* __restrict__ src,
+                                                           const uint32_t* __restrict__ dst,
+                                                           uint8_t* __restrict__ state,
+                                                           const uint8_t* __restrict__ nflag,
+                                                           uint32_t* __restrict__ sup,
+                                                           uint32_t* __restrict__ wl, uint32_t E,
+                                                           int directed, DevCounters* ctr) {
+    __shared__ uint32_t s_out[kListCap];
+    __shared__ uint32_t s_cnt, s_base;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t below = (1ull << lane) - 1ull;
+    // uniform trip count: every thread reaches the barriers
     for (uint32_t base = blockIdx.x * blockDim.x; base < E; base += gsize()) {
         const uint32_t e = base + threadIdx.x;
         bool take = false;
@@ -986,12 +1047,21 @@ __global__ void k_build_worklist(const uint32_t* __restrict__ src, const uint32_
                 take = st != KT_NOT_AFFECTED;
             }
         }
-        const uint32_t pos = wave_append(&ctr->wl_size, take, lane);
-        if (take) {
-            wl[pos] = e;
-            sup[e] = 0;
+        const uint64_t m = ballot(take);
+        if (m) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(&s_cnt, (uint32_t)__popcll(m));  // LDS
+            b = __shfl(b, 0);
+            if (take) {
+                s_out[b + (uint32_t)__popcll(m & below)] = e;
+                sup[e] = 0;
+            }
         }
+        __syncthreads();
+        if (s_cnt > kListCap - kBlock) list_flush(s_out, &s_cnt, &s_base, wl, &ctr->wl_size);
     }
+    __syncthreads();
+    list_flush(s_out, &s_cnt, &s_base, wl, &ctr->wl_size);
 }
 
 // Minimum live support.  Grid-stride over at most 2048 workgroups, wave
@@ -1117,6 +1187,11 @@ __global__ void k_set_tail(uint32_t* xs, uint32_t E, const uint32_t* tile_sums, 
 
 namespace dev {
 
+hipError_t stripes_fold(unsigned long long* stripes, DevCounters* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_stripes_fold, dim3(1), dim3(256), 0, s, stripes, ctr);
+    return hipGetLastError();
+}
+
 hipError_t fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s) {
     if (!count) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count)), dim3(kBlock), 0, s, p, value, count);
@@ -1187,7 +1262,7 @@ hipError_t mark_deleted(uint32_t* dst, const uint32_t* src, const uint32_t* eid,
                         uint32_t* tau, const uint32_t* wl, uint32_t nitems, uint32_t k,
                         bool affected, DevCounters* ctr, hipStream_t s) {
     if (!nitems) return hipSuccess;
-    hipLaunchKernelGGL(k_mark_deleted, dim3(grid_for(nitems)), dim3(kBlock), 0, s, dst, src, eid,
+    hipLaunchKernelGGL(k_mark_deleted, dim3(list_grid(nitems)), dim3(kBlock), 0, s, dst, src, eid,
                        sup, state, nflag, tau, wl, nitems, k, affected ? 1 : 0, ctr);
     return hipGetLastError();
 }
@@ -1204,7 +1279,7 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
                           const uint8_t* nflag, uint32_t* sup, uint32_t* wl, uint32_t E,
                           bool directed, DevCounters* ctr, hipStream_t s) {
     if (!E) return hipSuccess;
-    hipLaunchKernelGGL(k_build_worklist, dim3(grid_for(E)), dim3(kBlock), 0, s, src, dst, state,
+    hipLaunchKernelGGL(k_build_worklist, dim3(list_grid(E)), dim3(kBlock), 0, s, src, dst, state,
                        nflag, sup, wl, E, directed ? 1 : 0, ctr);
     return hipGetLastError();
 }

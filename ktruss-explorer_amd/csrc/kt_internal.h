@@ -69,6 +69,14 @@ struct DevCounters {
 };
 
 // Arguments of the support-count kernels.
+// Statistics every wave of a count launch adds to go to one of kStripes
+// 128-byte lines (by wave index) instead of one address: tens of thousands of
+// atomics on one address are served one at a time and hold back the launch.
+// dev::stripes_fold adds the lines to the counters and clears them.
+constexpr uint32_t kStripes = 512;
+constexpr uint32_t kStripeWords = 16;  // 64-bit words per line: merge_elems, recount_edges
+constexpr size_t kStripeBytes = (size_t)kStripes * kStripeWords * 8;
+
 struct CountArgs {
     const uint32_t* ptr;
     const uint32_t* src;
@@ -84,6 +92,7 @@ struct CountArgs {
     uint32_t query_factor;// query the shorter list when it is this many times shorter
     uint32_t prefilter;   // undirected COO+CSR stored-length pre-filter (huc:20)
     DevCounters* ctr;
+    unsigned long long* stripes;  // kStripes lines of the count statistics
 };
 
 // Device-resident task list of a batched decomposition: task t covers the
@@ -98,6 +107,7 @@ struct TaskList {
 // Kernel launchers (kt_device.hip).  All asynchronous on `stream`.
 namespace dev {
 hipError_t fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
+hipError_t stripes_fold(unsigned long long* stripes, DevCounters* ctr, hipStream_t s);
 hipError_t fill_u8(uint8_t* p, uint8_t value, uint64_t count, hipStream_t s);
 hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s);
 hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t algo,

@@ -157,7 +157,8 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = wl.alloc(Eb)) || (rc = tau.alloc(Eb)) ||
         (rc = tiles.alloc(((size_t)dev::compact_tiles(E0) + 2) * sizeof(uint32_t))) ||
         (rc = xs.alloc(Eb + sizeof(uint32_t))) || (rc = ctr.alloc(sizeof(DevCounters))) ||
-        (rc = ctr_scratch.alloc(sizeof(DevCounters))) ||
+        (rc = ctr_scratch.alloc(sizeof(DevCounters))) || (rc = stripes.alloc(kStripeBytes)) ||
+        (rc = stripes_x.alloc(kStripeBytes)) ||
         (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))) ||
         (rc = feat.alloc(KT_MLP_IN * sizeof(float))))
         return rc;
@@ -166,6 +167,8 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         KT_HIP_TRY(hipMemcpyAsync(src0.p, L.src.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
         KT_HIP_TRY(hipMemcpyAsync(dst0.p, L.dst.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
     }
+    KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
+    KT_HIP_TRY(dev::fill_u32(stripes_x.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
     if (directed && !tiled) {  // kept for the adjacency lists DELTA builds on first use
         h_ptr = std::move(L.ptr);
@@ -177,6 +180,10 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
 }
 
 int Plan::sync_counters(bool fold) {
+    if (stripes_dirty) {
+        KT_HIP_TRY(dev::stripes_fold(stripes.as<unsigned long long>(), ctr.as<DevCounters>(), stream));
+        stripes_dirty = false;
+    }
     if (fold && delta_ready) KT_HIP_TRY(dev::delta_stats_fold(dws, ctr.as<DevCounters>(), stream));
     KT_HIP_TRY(hipMemcpyAsync(h_ctr, ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
@@ -201,6 +208,8 @@ int Plan::reset() {
     KT_HIP_TRY(dev::iota_u32(eid[0].as<uint32_t>(), E0, stream));
     KT_HIP_TRY(dev::fill_u32(tau.as<uint32_t>(), 0, E0, stream));
     KT_HIP_TRY(dev::fill_u32(ctr.as<uint32_t>(), 0, sizeof(DevCounters) / 4, stream));
+    KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
+    stripes_dirty = false;
     if (delta_ready) {
         const int rc = restore_delta();
         if (rc) return rc;
@@ -307,6 +316,12 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
     a.query_factor = query_factor;
     a.prefilter = (!directed && !tiled) ? 1u : 0u;  // kernel_duc.cu:27; kernel_dut.cu has none
     a.ctr = counters;
+    if (counters == ctr.as<DevCounters>()) {
+        a.stripes = stripes.as<unsigned long long>();
+        stripes_dirty = true;
+    } else {
+        a.stripes = stripes_x.as<unsigned long long>();
+    }
     return a;
 }
 

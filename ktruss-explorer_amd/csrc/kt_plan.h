@@ -65,6 +65,8 @@ struct Plan {
     DBuf model;          // cost-model parameters (KT_PICK_MODEL)
     uint32_t model_C = 0, model_n = 0;
     DBuf feat_acc, feat, ctr_scratch;
+    DBuf stripes, stripes_x;  // count statistics of ctr / of ctr_scratch (never read)
+    bool stripes_dirty = false;  // a count added to `stripes` since the last fold
     float* h_feat = nullptr;
     hipEvent_t ev_sw[2 * KT_MAX_CANDIDATES] = {};
     std::vector<std::array<float, KT_PHASE_REC>> phase_log;
