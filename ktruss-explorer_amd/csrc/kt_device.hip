@@ -23,6 +23,9 @@
 // rows), keeping a row's data and its support counters in one XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "kt_internal.h"
 
 namespace kt {
@@ -843,12 +846,14 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a, TaskList tl) {
             const uint4 d = tl.desc[t];
             run(d.x, d.y);
         }
-    } else {
-        const uint64_t first = (uint64_t)task0 * a.chunk;
+    } else {  // persistent too: tasks of `chunk` items, wave_id + W, ...
         const uint32_t nitems = a.nitems;
-        if (first >= nitems) return;
-        const uint32_t i0 = (uint32_t)first;
-        run(i0, nitems - i0 < a.chunk ? nitems : i0 + a.chunk);
+        const uint32_t ntasks = (uint32_t)(((uint64_t)nitems + a.chunk - 1) / a.chunk);
+        const uint32_t W = gridDim.x * nw;
+        for (uint32_t t = task0; t < ntasks; t += W) {
+            const uint32_t i0 = t * a.chunk;
+            run(i0, nitems - i0 < a.chunk ? nitems : i0 + a.chunk);
+        }
     }
     flush();
     for (int off = 32; off > 0; off >>= 1) {
@@ -1210,6 +1215,31 @@ hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Workgroups of `kernel` (block threads, lds dynamic LDS bytes) resident on
+// the whole device at once; cached per (kernel, block, lds).
+uint32_t resident_blocks(const void* kernel, uint32_t block, size_t lds) {
+    struct Key {
+        const void* k;
+        uint32_t b;
+        size_t l;
+        uint32_t v;
+    };
+    static std::vector<Key> cache;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> g(mu);
+    for (const Key& c : cache)
+        if (c.k == kernel && c.b == block && c.l == lds) return c.v;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)block, lds) != hipSuccess || per < 1)
+        per = 1;
+    const uint32_t v = (uint32_t)per * (uint32_t)(ncu > 0 ? ncu : 256);
+    cache.push_back({kernel, block, lds, v});
+    return v;
+}
+
 hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_t algo,
                           uint32_t block, hipStream_t s) {
     if (!a.nitems) return hipSuccess;
@@ -1224,8 +1254,13 @@ hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_
         return hipGetLastError();
     }
     if (directed && !tiled) {
+        // persistent: as many workgroups as stay resident at once (a
+        // workgroup keeps its LDS until its slowest wave ends, so a grid of
+        // one wave per task leaves CUs waiting behind hub rows)
         const size_t lds = (size_t)nw * (a.cap_lds + a.cap_lds / 2) * sizeof(uint32_t);
-        hipLaunchKernelGGL(k_count_dir<false>, grid, dim3(block), lds, s, a, TaskList{});
+        const uint32_t res = resident_blocks((const void*)k_count_dir<false>, block, lds);
+        const uint32_t pb = blocks < res ? blocks : res;
+        hipLaunchKernelGGL(k_count_dir<false>, dim3(pb ? pb : 1), dim3(block), lds, s, a, TaskList{});
         return hipGetLastError();
     }
     const size_t lds = (size_t)nw * (directed ? 2 : 1) * a.cap_lds * sizeof(uint32_t);
