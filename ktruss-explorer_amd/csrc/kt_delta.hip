@@ -48,11 +48,15 @@ __device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / k
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77), deferred: a deleted entry
 // is flagged KT_DYING and listed; it becomes KT_DEAD in k_finalize.
 // The deleted entry's two adjacency slots get KT_DYING as well.
+// The scan reads supports only: a deleted entry's support becomes
+// KT_SUP_GONE (above every threshold), and an entry's flags are read only when
+// its support is below k - 2.
 // A workgroup walks tiles of kMarkTile entries (every lane loads all of its
-// entries and supports before the first wait) and lists its deletions in
+// supports before the first wait) and lists its deletions in
 // LDS; the global list is reserved with one atomic per workgroup when the
 // LDS list is full and at the end.  A reservation per wave would put one
 // returning atomic per wave on a single address, served one at a time.
+constexpr uint32_t KT_SUP_GONE = 0xFFFFFFFFu;  // support of an entry the mark deleted
 constexpr uint32_t kMarkItems = 8;
 constexpr uint32_t kMarkTile = kBlock * kMarkItems;
 constexpr uint32_t kMarkBuf = 2 * kMarkTile;
@@ -73,7 +77,7 @@ __device__ __forceinline__ void mark_flush(const uint32_t* s_out, uint32_t* s_cn
 
 __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ dst,
                                                        const uint32_t* __restrict__ eid,
-                                                       const uint32_t* __restrict__ sup,
+                                                       uint32_t* __restrict__ sup,
                                                        uint32_t* __restrict__ tau,
                                                        uint32_t* __restrict__ dl, uint32_t n, uint32_t k,
                                                        uint32_t* __restrict__ snbr,
@@ -88,17 +92,25 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
     const uint32_t ntile = (n + kMarkTile - 1) / kMarkTile;
     for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const uint32_t e0 = tile * kMarkTile + threadIdx.x;
-        uint32_t v[kMarkItems], sp[kMarkItems];
+        uint32_t sp[kMarkItems];
 #pragma unroll
         for (uint32_t j = 0; j < kMarkItems; ++j) {
             const uint32_t e = e0 + j * kBlock;
-            v[j] = e < n ? dst[e] : KT_DEAD;
-            sp[j] = e < n ? sup[e] : 0u;
+            const uint32_t x = sup[e < n ? e : n - 1];  // unconditional: the loads stay in flight together
+            sp[j] = e < n ? x : KT_SUP_GONE;
+        }
+        // flags are read only for items where some lane is below k - 2 (all
+        // such loads in flight together; lanes past the end read entry n - 1)
+        uint32_t v[kMarkItems];
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkItems; ++j) {
+            const uint32_t e = e0 + j * kBlock;
+            v[j] = ballot(sp[j] < k - 2) ? dst[e < n ? e : n - 1] : 0u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kMarkItems; ++j) {
             const uint32_t e = e0 + j * kBlock;
-            const bool del = !(v[j] & (KT_DEAD | KT_DYING)) && sp[j] < k - 2;
+            const bool del = sp[j] < k - 2 && !(v[j] & (KT_DEAD | KT_DYING));
             const uint64_t m = ballot(del);
             if (!m) continue;
             uint32_t b = 0;
@@ -106,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
             b = __shfl(b, 0);
             if (del) {
                 s_out[b + (uint32_t)__popcll(m & below)] = e;
+                sup[e] = KT_SUP_GONE;
                 dst[e] = v[j] | KT_DYING;
                 tau[eid[e]] = k - 1;
                 const uint32_t a = sa[e], c = sb[e];
@@ -125,8 +138,10 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
                            const uint32_t* __restrict__ sup, DevCounters* ctr) {
     __shared__ uint32_t s_min[kBlock / kWave];
     uint32_t m = 0xFFFFFFFFu;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-        if (!(dst[e] & (KT_DEAD | KT_DYING)) && sup[e] < m) m = sup[e];
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t x = sup[e];
+        if (x < m && !(dst[e] & (KT_DEAD | KT_DYING))) m = x;  // flags read for a new minimum only
+    }
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t o = __shfl_xor(m, off);
         m = o < m ? o : m;
@@ -647,7 +662,7 @@ __global__ void k_cmp_slots(const uint32_t* __restrict__ snbr2, const uint32_t* 
 
 namespace dev {
 
-hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
+hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, DevCounters* ctr, hipStream_t s) {
     if (!n) return hipSuccess;

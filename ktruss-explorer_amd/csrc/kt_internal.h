@@ -131,7 +131,7 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s);
 // DELTA support maintenance (kt_delta.hip).
-hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, const uint32_t* sup, uint32_t* tau,
+hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, DevCounters* ctr, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying).
