@@ -734,8 +734,9 @@ hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t
                             uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                             hipStream_t s) {
     if (!nd_max) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nd_max, kBlock)), dim3(kBlock), 0, s, dl, nd_dev,
-                       dst, snbr, sa, sb);
+    const uint32_t fb = blocks_for(nd_max, kBlock);  // grid-stride: nd_max may be a loose bound
+    hipLaunchKernelGGL(k_finalize, dim3(fb < 1024 ? fb : 1024), dim3(kBlock), 0, s, dl, nd_dev, dst,
+                       snbr, sa, sb);
     return hipGetLastError();
 }
 

@@ -83,6 +83,10 @@ int ensure_device() {
 Plan::~Plan() {
     if (h_ctr) (void)hipHostFree(h_ctr);
     if (h_feat) (void)hipHostFree(h_feat);
+    if (h_ra) (void)hipHostFree(h_ra);
+    for (auto& slot : ev_ra)
+        for (auto& ev : slot)
+            if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : ev_sw)
         if (ev) (void)hipEventDestroy(ev);
     for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1})
@@ -145,6 +149,9 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     KT_HIP_TRY(hipHostMalloc((void**)&h_feat, KT_MLP_IN * sizeof(float), hipHostMallocDefault));
     std::memset(h_feat, 0, KT_MLP_IN * sizeof(float));
     for (auto& ev : ev_sw) KT_HIP_TRY(hipEventCreate(&ev));
+    for (auto& slot : ev_ra)
+        for (auto& ev : slot) KT_HIP_TRY(hipEventCreate(&ev));
+    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, 2 * sizeof(uint32_t), hipHostMallocDefault));
 
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
     const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
@@ -660,8 +667,10 @@ int Plan::restore_delta() {
 // slot -> entry map following the entries to their new positions.  Live
 // entries and slots (two per live edge) are counted on the host, so the new
 // sizes need no read-back beyond the compaction's own check.
+bool Plan::compaction_due() const { return live > 0 && (uint64_t)E > 2 * live && E > 1024; }
+
 int Plan::compact_delta_state() {
-    if (pending || live == 0 || (uint64_t)E <= 2 * live || E <= 1024) return KT_OK;
+    if (pending || !compaction_due()) return KT_OK;
     const uint32_t S = S_cur;
     int rc = compact_now();  // oriented layout: xs[old position] = new position
     if (rc) return rc;
@@ -779,6 +788,100 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
     return KT_OK;
 }
 
+// Bookkeeping of one finished peel iteration (its deletions are read back).
+int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms) {
+    live -= ndel;
+    stats.iterations++;
+    trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)ndel, recount, (uint64_t)(count_ms * 1e6)});
+    if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
+    return KT_OK;
+}
+
+// The remaining iterations of a DELTA phase at k (k > 3), queued one ahead of
+// their read-backs: iteration j's update (of iteration j - 1's deletions,
+// their number read on the device, grids sized by an upper bound) and mark
+// are queued before the deletion count of iteration j - 1 comes back through
+// pinned memory, so the device does not wait for the host between
+// iterations.  The iteration queued after the one that deletes nothing only
+// finishes the last update (no support changes, so its mark deletes nothing);
+// it is not an iteration of the peel and is not recorded.  Returns with
+// `ended` when the phase is over (nothing pending), else with the last
+// iteration's deletions pending and `iter` the next index, for a compaction.
+// On entry `pending` holds the deletions of iteration iter - 1 (> 0).
+int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended) {
+    uint32_t* dl_p = dl.as<uint32_t>();
+    DevCounters* c = ctr.as<DevCounters>();
+    const uint32_t* nd_dev = &c->ndel;
+    uint32_t bound = pending;    // deletions the next update consumes, at most
+    uint32_t last_nd = pending;  // deletions of the last recorded iteration
+    int open = -1;               // slot of the queued iteration whose count is unread
+    uint32_t open_iter = 0;
+    ended = false;
+    auto queue = [&](int s) -> int {
+        KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, bound, src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                      sptr_b[acur].as<uint32_t>(), dws, c, stream));
+        KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
+        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, bound, snbr_b[acur].as<uint32_t>(),
+                                    seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream));
+        KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
+        KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, bound, dst[cur].as<uint32_t>(),
+                                         snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                         stream));
+        KT_HIP_TRY(dev::fill_u32(&c->ndel, 0, 1, stream));
+        KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
+                                   tau.as<uint32_t>(), dl_p, E, k, snbr_b[acur].as<uint32_t>(),
+                                   sa.as<uint32_t>(), sb.as<uint32_t>(), c, stream));
+        KT_HIP_TRY(hipMemcpyAsync(h_ra + s, &c->ndel, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
+        return KT_OK;
+    };
+    // reads the open iteration back and records it
+    auto finish = [&](uint32_t& nd) -> int {
+        KT_HIP_TRY(hipEventSynchronize(ev_ra[open][2]));
+        nd = h_ra[open];
+        float ms = 0;
+        KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[open][0], ev_ra[open][1]));
+        stats.ms_delta += ms;
+        phase_ms += ms;
+        stats.count_passes++;
+        stats.delta_passes++;
+        stats.delta_edges += last_nd;  // its update consumed the previous iteration's deletions
+        last_pass_delta = true;
+        last_nd = nd;
+        const int rc = record_iteration(k, open_iter, nd, 0, ms);
+        open = -1;
+        return rc;
+    };
+    for (;;) {
+        const int s = (int)(iter & 1u);
+        int rc = queue(s);  // iteration `iter`
+        if (rc) return rc;
+        if (open >= 0) {  // read iteration iter - 1 while iteration `iter` runs
+            const uint32_t prev_iter = open_iter;
+            uint32_t nd = 0;
+            if ((rc = finish(nd))) return rc;
+            if (nd == 0) {  // the phase ended there; iteration `iter` is void
+                KT_HIP_TRY(hipEventSynchronize(ev_ra[s][2]));
+                pending = h_ra[s];
+                iter = prev_iter + 1;
+                ended = true;
+                return pending ? fail(KT_ERR_STATE, "void DELTA iteration deleted entries") : KT_OK;
+            }
+        }
+        open = s;
+        open_iter = iter++;
+        bound = (uint32_t)std::min<uint64_t>(live, E);  // the open iteration deletes live entries only
+        if (compaction_due()) {  // read the open iteration back and leave for the compaction
+            uint32_t nd = 0;
+            if ((rc = finish(nd))) return rc;
+            pending = nd;
+            iter = open_iter + 1;
+            ended = nd == 0;
+            return KT_OK;
+        }
+    }
+}
+
 // One phase at k with DELTA available: the mark pass defers its deletions
 // (KT_DYING + list), and the support pass that opens the next iteration
 // either removes their triangles (DELTA) or recounts.  The dead entries are
@@ -860,17 +963,21 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         phase_ms += count_ms;
         const uint32_t ndel = h_ctr->ndel;
         pending = ndel;
-        live -= ndel;
-        stats.iterations++;
-        trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)ndel, h_ctr->recount_edges - rec_before,
-                         (uint64_t)(count_ms * 1e6)});
-        if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
+        if ((rc = record_iteration(k, iter, ndel, h_ctr->recount_edges - rec_before, count_ms))) return rc;
         changed = ndel > 0;
         if (changed && k == 3) {  // kernel_ddc.cu:183-186
             broke = true;
             break;
         }
         ++iter;
+        // with DELTA settled for the phase, the next iterations are queued
+        // ahead of their read-backs until the phase ends or a compaction is due
+        if (changed && pick != KT_PICK_SWEEP && !unresolved && cfg.kernel == KT_COUNT_DELTA &&
+            !compaction_due()) {
+            bool ended = false;
+            if ((rc = run_ahead_delta(k, iter, phase_ms, ended))) return rc;
+            if (ended) changed = false;
+        }
     } while (changed);
     if (pick == KT_PICK_SWEEP && nc) {
         chosen = 0;

@@ -53,6 +53,10 @@ struct Plan {
     DevCounters* h_ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
+    // run-ahead DELTA iterations: two slots of (update start, update end,
+    // count read-back) events and the pinned deletion counts
+    hipEvent_t ev_ra[2][3] = {};
+    uint32_t* h_ra = nullptr;
 
     std::vector<std::array<uint64_t, 5>> trace;
     kt_stats stats{};
@@ -105,8 +109,11 @@ struct Plan {
     int run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule);
     int ensure_delta();
     int compact_delta_state();
+    bool compaction_due() const;  // dead entries are more than half of the stored ones
     int restore_delta();
     int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
+    int run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended);
+    int record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms);
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
     // defer: leave the model's choice on the device (read with the next
     // counter read-back) instead of synchronising for it now
