@@ -4,10 +4,11 @@
 // its support sup[e] is the exact number of triangles through it).  After a
 // mark pass the deleted entries carry KT_DYING and sit in the list dl; the
 // supports are brought to those of the surviving graph by
-//   k_dprep, k_dscan_tiles, k_dscan_local
-//               per deleted entry d = (u, v): the shorter and the longer of
+//   k_dprep     per deleted entry d = (u, v): the shorter and the longer of
 //               the endpoints' full adjacency lists, and the tasks of 64
-//               neighbours of the shorter list, numbered by an exclusive scan;
+//               neighbours of the shorter list, numbered by an exclusive scan
+//               (per tile of deletions, then over the tiles by the last
+//               workgroup to finish);
 //   k_delta     waves take equal ranges of tasks; each neighbour w of a task
 //               is binary-searched (uniform trip count) in the longer list.
 //               Work is split by list length, so a deletion between two hubs
@@ -56,6 +57,16 @@ __device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / k
 // LDS; the global list is reserved with one atomic per workgroup when the
 // LDS list is full and at the end.  A reservation per wave would put one
 // returning atomic per wave on a single address, served one at a time.
+// A deleted entry whose update is done: KT_DEAD in the oriented layout and
+// in both of its adjacency slots.
+__device__ __forceinline__ void finalize_one(uint32_t d, uint32_t* dst, uint32_t* snbr,
+                                             const uint32_t* sa, const uint32_t* sb) {
+    dst[d] = (dst[d] & ~KT_DYING) | KT_DEAD;
+    const uint32_t a = sa[d], b = sb[d];
+    snbr[a] = (snbr[a] & ~KT_DYING) | KT_DEAD;
+    snbr[b] = (snbr[b] & ~KT_DYING) | KT_DEAD;
+}
+
 constexpr uint32_t KT_SUP_GONE = 0xFFFFFFFFu;  // support of an entry the mark deleted
 constexpr uint32_t kMarkItems = 8;
 constexpr uint32_t kMarkTile = kBlock * kMarkItems;
@@ -82,10 +93,17 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                                                        uint32_t* __restrict__ dl, uint32_t n, uint32_t k,
                                                        uint32_t* __restrict__ snbr,
                                                        const uint32_t* __restrict__ sa,
-                                                       const uint32_t* __restrict__ sb, DevCounters* ctr) {
+                                                       const uint32_t* __restrict__ sb, unsigned int* ndel,
+                                                       const uint32_t* __restrict__ fin_dl,
+                                                       const uint32_t* __restrict__ fin_nd) {
     __shared__ uint32_t s_out[kMarkBuf];
     __shared__ uint32_t s_cnt, s_base;
     if (threadIdx.x == 0) s_cnt = 0;
+    if (fin_dl) {  // the previous deletions, whose update is done, become KT_DEAD
+        const uint32_t nf = *fin_nd;
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x)
+            finalize_one(fin_dl[t], dst, snbr, sa, sb);
+    }
     __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t below = (1ull << lane) - 1ull;
@@ -127,10 +145,10 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
             }
         }
         __syncthreads();
-        if (s_cnt > kMarkBuf - kMarkTile) mark_flush(s_out, &s_cnt, &s_base, dl, &ctr->ndel);
+        if (s_cnt > kMarkBuf - kMarkTile) mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
     }
     __syncthreads();
-    mark_flush(s_out, &s_cnt, &s_base, dl, &ctr->ndel);
+    mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
 }
 
 // Minimum live support (the JUMP target); one atomic per workgroup.
@@ -154,112 +172,113 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
     }
 }
 
-// Per pending deletion i (d = dl[i] = (u, v)): the shorter and the longer of
-// the two adjacency lists and the number of 64-candidate tasks over the
-// shorter one (at least 1: the list holds the edge itself); per tile of
-// kTileD deletions the task total.
+// Task list of the pending deletions, one kernel.  Per deletion i (d = dl[i]
+// = (u, v)): the shorter and the longer of the two adjacency lists and the
+// number of 64-candidate tasks over the shorter one (at least 1: the list
+// holds the edge itself).  Deletions form tiles of kTileD; a workgroup scans
+// its tile (lofs[i] = first task of deletion i counted from the tile's
+// first) and stores the tile total; the last workgroup to finish (a ticket
+// taken after a device-scope fence) scans the tile totals in place, so the
+// first task of deletion i is tsum[i / kTileD] + lofs[i] and the task total
+// is tsum[ntiles] = ctr->dtasks.
 constexpr uint32_t kTileD = 2048;
+constexpr uint32_t kPerD = kTileD / kBlock;  // deletions per thread of a tile
 
 __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint32_t* __restrict__ src,
                                                   const uint32_t* __restrict__ dst,
                                                   const uint32_t* __restrict__ sptr,
-                                                  uint4* __restrict__ meta,
-                                                  uint32_t* __restrict__ ntask,
-                                                  uint32_t* __restrict__ tsum) {
+                                                  uint4* __restrict__ meta, uint32_t* __restrict__ lofs,
+                                                  uint32_t* tsum, DevCounters* ctr, unsigned int* zero_cnt) {
     __shared__ uint32_t s_w[kBlock / kWave];
-    const uint32_t nd = *nd_dev;
-    const uint32_t ntiles = (nd + kTileD - 1) / kTileD;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t end = min(nd, (t + 1) * kTileD);
-        uint32_t c = 0;
-        for (uint32_t i = t * kTileD + threadIdx.x; i < end; i += kBlock) {
-            const uint32_t d = dl[i];
-            const uint32_t u = src[d], v = dst[d] & KT_IDMASK;
-            const uint32_t ub = sptr[u], ue = sptr[u + 1], vb = sptr[v], ve = sptr[v + 1];
-            const bool us = ue - ub <= ve - vb;
-            const uint4 m = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
-            meta[i] = m;
-            const uint32_t nt = (m.y - m.x + kWave - 1) / kWave;
-            ntask[i] = nt;
-            c += nt;
-        }
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-        if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = c;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t sum = 0;
-            for (uint32_t w = 0; w < kBlock / kWave; ++w) sum += s_w[w];
-            tsum[t] = sum;
-        }
-        __syncthreads();
-    }
-}
-
-// Exclusive scan of the tile totals in place (one workgroup); the grand
-// total goes to tsum[ntiles] and ctr->dtasks.
-__global__ __launch_bounds__(1024) void k_dscan_tiles(const uint32_t* __restrict__ nd_dev,
-                                                      uint32_t* tsum, DevCounters* ctr) {
-    __shared__ uint32_t s_part[1024];
-    const uint32_t nt = (*nd_dev + kTileD - 1) / kTileD;
-    const uint32_t per = (nt + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t i = b; i < b + per && i < nt; ++i) sum += tsum[i];
-    s_part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t x = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += x;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
-    for (uint32_t i = b; i < b + per && i < nt; ++i) {
-        const uint32_t c = tsum[i];
-        tsum[i] = run;
-        run += c;
-    }
-    if (threadIdx.x == 1023) {
-        tsum[nt] = s_part[1023];
-        ctr->dtasks = s_part[1023];
-    }
-}
-
-// ntask[i] <- first task of deletion i (exclusive scan inside the tile plus
-// the tile's offset).  Thread th owns 8 consecutive deletions of the tile.
-__global__ __launch_bounds__(kBlock) void k_dscan_local(const uint32_t* __restrict__ nd_dev,
-                                                        uint32_t* __restrict__ ntask,
-                                                        const uint32_t* __restrict__ tsum) {
-    __shared__ uint32_t s_w[kBlock / kWave];
-    constexpr uint32_t kPer = kTileD / kBlock;
+    __shared__ uint32_t s_last;
+    if (zero_cnt && blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     const uint32_t nd = *nd_dev;
     const uint32_t ntiles = (nd + kTileD - 1) / kTileD;
     const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t i0 = t * kTileD + threadIdx.x * kPer;
-        uint32_t vals[kPer];
-        uint32_t sum = 0;
-        for (uint32_t r = 0; r < kPer; ++r) {
-            vals[r] = i0 + r < nd ? ntask[i0 + r] : 0u;
-            sum += vals[r];
+    if (ntiles == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            tsum[0] = 0;
+            ctr->dtasks = 0;
         }
-        uint32_t inc = sum;
+        return;
+    }
+    if (blockIdx.x >= ntiles) return;  // whole workgroup
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t i0 = t * kTileD + threadIdx.x * kPerD;
+        uint32_t cnt[kPerD];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < kPerD; ++r) {
+            const uint32_t i = i0 + r;
+            cnt[r] = 0;
+            if (i < nd) {
+                const uint32_t d = dl[i];
+                const uint32_t u = src[d], v = dst[d] & KT_IDMASK;
+                const uint32_t ub = sptr[u], ue = sptr[u + 1], vb = sptr[v], ve = sptr[v + 1];
+                const bool us = ue - ub <= ve - vb;
+                const uint4 m = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
+                meta[i] = m;
+                cnt[r] = (m.y - m.x + kWave - 1) / kWave;
+            }
+            sum += cnt[r];
+        }
+        uint32_t inc = sum;  // inclusive scan over the workgroup
         for (int off = 1; off < (int)kWave; off <<= 1) {
             const uint32_t y = __shfl_up(inc, off);
             if ((int)lane >= off) inc += y;
         }
         if (lane == kWave - 1) s_w[wid] = inc;
         __syncthreads();
-        uint32_t run = tsum[t] + inc - sum;
+        uint32_t run = inc - sum;
         for (uint32_t w = 0; w < wid; ++w) run += s_w[w];
-        for (uint32_t r = 0; r < kPer; ++r) {
-            if (i0 + r < nd) ntask[i0 + r] = run;
-            run += vals[r];
+#pragma unroll
+        for (uint32_t r = 0; r < kPerD; ++r) {
+            if (i0 + r < nd) lofs[i0 + r] = run;
+            run += cnt[r];
         }
+        if (threadIdx.x == kBlock - 1) tsum[t] = run;  // the tile total
         __syncthreads();
     }
+    // the last workgroup to finish scans the tile totals
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t parts = ntiles < gridDim.x ? ntiles : gridDim.x;
+        s_last = atomicAdd(&ctr->dticket, 1u) == parts - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    const uint32_t per = (ntiles + kBlock - 1) / kBlock;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < b + per && i < ntiles; ++i) sum += tsum[i];
+    uint32_t inc = sum;
+    for (int off = 1; off < (int)kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += y;
+    }
+    if (lane == kWave - 1) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (uint32_t w = 0; w < wid; ++w) run += s_w[w];
+    for (uint32_t i = b; i < b + per && i < ntiles; ++i) {
+        const uint32_t c = tsum[i];
+        tsum[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == kBlock - 1) {
+        tsum[ntiles] = run;
+        ctr->dtasks = run;
+        ctr->dticket = 0;
+    }
+}
+
+// First task of deletion i.
+__device__ __forceinline__ uint32_t first_task(const uint32_t* lofs, const uint32_t* tsum, uint32_t i) {
+    return tsum[i / kTileD] + lofs[i];
 }
 
 // The update proper.  Task t = (deletion i, chunk c) holds the candidates
@@ -381,7 +400,8 @@ __device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t 
 __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
-                                                  const uint32_t* __restrict__ toff,
+                                                  const uint32_t* __restrict__ lofs,
+                                                  const uint32_t* __restrict__ tsum,
                                                   const uint32_t* __restrict__ snbr,
                                                   const uint32_t* __restrict__ seid,
                                                   uint32_t* sup, DevCounters* ctr,
@@ -400,17 +420,17 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
     uint32_t i = 0, len = nd;
     while (len > 1) {
         const uint32_t half = len >> 1;
-        if (toff[i + half] <= t0) i += half;
+        if (first_task(lofs, tsum, i + half) <= t0) i += half;
         len -= half;
     }
     uint32_t full = 0xFFFFFFFFu;     // deletion whose whole longer list is in buf
     uint32_t sampled = 0xFFFFFFFFu;  // deletion whose block samples are in smp
     unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
     for (uint32_t t = t0; t < t1;) {
-        while (i + 1 < nd && toff[i + 1] <= t) ++i;
+        while (i + 1 < nd && first_task(lofs, tsum, i + 1) <= t) ++i;
         const uint32_t tc = t++;  // this task; the body may continue below
         const uint4 m = meta[i];
-        const uint32_t j0 = m.x + (tc - toff[i]) * kWave;
+        const uint32_t j0 = m.x + (tc - first_task(lofs, tsum, i)) * kWave;
         const uint32_t j = j0 + lane;
         acc_elems += min(kWave, m.y - j0);
         // a longer list that fits the slice is staged with the candidates'
@@ -536,13 +556,8 @@ __global__ void k_finalize(const uint32_t* __restrict__ dl, const uint32_t* __re
                            uint32_t* __restrict__ dst, uint32_t* __restrict__ snbr,
                            const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb) {
     const uint32_t nd = *nd_dev;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += gridDim.x * blockDim.x) {
-        const uint32_t d = dl[t];
-        dst[d] = (dst[d] & ~KT_DYING) | KT_DEAD;
-        const uint32_t a = sa[d], b = sb[d];
-        snbr[a] = (snbr[a] & ~KT_DYING) | KT_DEAD;
-        snbr[b] = (snbr[b] & ~KT_DYING) | KT_DEAD;
-    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += gridDim.x * blockDim.x)
+        finalize_one(dl[t], dst, snbr, sa, sb);
 }
 
 uint32_t blocks_for(uint64_t items, uint32_t per_block) {
@@ -664,11 +679,16 @@ namespace dev {
 
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
-                      const uint32_t* sb, DevCounters* ctr, hipStream_t s) {
-    if (!n) return hipSuccess;
+                      const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
+                      const uint32_t* fin_nd, uint32_t fin_max, hipStream_t s) {
+    if (!fin_max) fin_dl = nullptr;
+    if (!n) {
+        if (fin_dl) return finalize_deleted(fin_dl, fin_nd, fin_max, dst, snbr, sa, sb, s);
+        return hipSuccess;
+    }
     const uint32_t nt = (n + kMarkTile - 1) / kMarkTile;
     hipLaunchKernelGGL(k_mark_dying, dim3(nt < kMarkGrid ? nt : kMarkGrid), dim3(kBlock), 0, s, dst,
-                       eid, sup, tau, dl, n, k, snbr, sa, sb, ctr);
+                       eid, sup, tau, dl, n, k, snbr, sa, sb, ndel, fin_dl, fin_nd);
     return hipGetLastError();
 }
 
@@ -697,17 +717,16 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
 
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
-                         const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
-    if (!nd_max) return hipSuccess;
+                         const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
+                         unsigned int* zero_cnt) {
+    if (!nd_max) {
+        if (zero_cnt) return fill_u32(zero_cnt, 0, 1, s);
+        return hipSuccess;
+    }
     const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
-    const uint32_t tb = tiles < 2048 ? tiles : 2048;
+    const uint32_t tb = tiles < 1024 ? tiles : 1024;
     hipLaunchKernelGGL(k_dprep, dim3(tb), dim3(kBlock), 0, s, dl, nd_dev, src, dst, sptr, ws.meta,
-                       ws.ntask, ws.tsum);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dscan_tiles, dim3(1), dim3(1024), 0, s, nd_dev, ws.tsum, ctr);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dscan_local, dim3(tb), dim3(kBlock), 0, s, nd_dev, ws.ntask, ws.tsum);
+                       ws.ntask, ws.tsum, ctr, zero_cnt);
     return hipGetLastError();
 }
 
@@ -716,7 +735,7 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
     if (!nd_max) return hipSuccess;
     const uint32_t db = nd_max < 64 ? 64 : (nd_max > kDeltaGrid ? kDeltaGrid : nd_max);
-    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, snbr,
+    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, ws.tsum, snbr,
                        seid, sup, ctr, ws.bstat);
     return hipGetLastError();
 }

@@ -61,7 +61,8 @@ struct DevCounters {
     unsigned int live;                // entries kept by the last compaction
     unsigned int choice;              // candidate picked by the cost model
     unsigned int dtasks;              // DELTA tasks of the pending deletions
-    unsigned int pad[2];
+    unsigned int dticket;             // DELTA task list: workgroups done (reset by the last)
+    unsigned int ndel2;               // second deletion count (DELTA iterations run ahead)
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
     unsigned long long delta_win;     // longer-list window entries read by DELTA updates
@@ -131,9 +132,13 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s);
 // DELTA support maintenance (kt_delta.hip).
+// The deletions are appended to dl, counted in *ndel (zero on entry).  fin_dl
+// / fin_nd (optional, at most fin_max entries): a previous deletion list
+// whose update is done, finalized (KT_DEAD) by the same kernel.
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
-                      const uint32_t* sb, DevCounters* ctr, hipStream_t s);
+                      const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
+                      const uint32_t* fin_nd, uint32_t fin_max, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying).
 hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
                           hipStream_t s);
@@ -141,18 +146,20 @@ hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, 
 struct DeltaScratch {
     unsigned long long* bstat;  // per k_delta workgroup: elems, win, live, tris
     uint4* meta;        // per deletion: short list [x, y), long list [z, z + w)
-    uint32_t* ntask;    // per deletion: 64-candidate tasks, then their exclusive scan
+    uint32_t* ntask;    // per deletion: first task counted from its tile's first
     uint32_t* tsum;     // per 2048-deletion tile, plus one
     uint32_t cap;
 };
 size_t delta_scratch_bytes(uint32_t cap);
 void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
 // nd_dev: the number of pending deletions, read on the device (dl[0, *nd_dev));
-// nd_max bounds it and sizes the grids.  delta_prepare lists the tasks (three
-// small kernels), delta_apply is the update proper (k_delta).
+// nd_max bounds it and sizes the grids.  delta_prepare lists the tasks (one
+// kernel), delta_apply is the update proper (k_delta).
+// zero_cnt (optional): a counter set to 0 by the same kernel (the next mark's).
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
-                         const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+                         const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
+                         unsigned int* zero_cnt = nullptr);
 // The update's statistics are summed per workgroup slot; delta_stats_fold
 // adds them to ctr->delta_* and clears the slots, delta_stats_reset clears
 // them (neither may be captured in a graph that replays before stats are read).

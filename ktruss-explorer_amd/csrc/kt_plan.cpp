@@ -11,6 +11,7 @@
 #include <array>
 #include <atomic>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <string>
 #include <vector>
@@ -151,7 +152,7 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     for (auto& ev : ev_sw) KT_HIP_TRY(hipEventCreate(&ev));
     for (auto& slot : ev_ra)
         for (auto& ev : slot) KT_HIP_TRY(hipEventCreate(&ev));
-    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, 2 * sizeof(uint32_t), hipHostMallocDefault));
+    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, kAheadSlots * sizeof(uint32_t), hipHostMallocDefault));
 
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
     const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
@@ -628,7 +629,8 @@ int Plan::ensure_delta() {
             return rc;
     if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
         (rc = sa0.alloc(Eb)) || (rc = sb0.alloc(Eb)) || (rc = sa.alloc(Eb)) || (rc = sb.alloc(Eb)) ||
-        (rc = dl.alloc(Eb)) || (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))) ||
+        (rc = dl.alloc(Eb)) || (rc = dl_alt.alloc(Eb)) ||
+        (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))) ||
         (rc = ctsum.alloc((size_t)ct * 4)) || (rc = cxs.alloc(2 * Eb + 4)))
         return rc;
     dev::delta_scratch_carve(dscratch.p, E0, dws);
@@ -809,38 +811,47 @@ int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t re
 // iteration's deletions pending and `iter` the next index, for a compaction.
 // On entry `pending` holds the deletions of iteration iter - 1 (> 0).
 int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended) {
-    uint32_t* dl_p = dl.as<uint32_t>();
     DevCounters* c = ctr.as<DevCounters>();
-    const uint32_t* nd_dev = &c->ndel;
+    // deletion lists and counts alternate: iteration r (counted from entry)
+    // updates from list r & 1 and marks into list (r + 1) & 1; the pending
+    // deletions on entry are in dl / ctr->ndel (list 0)
+    uint32_t* lists[2] = {dl.as<uint32_t>(), dl_alt.as<uint32_t>()};
+    unsigned int* counts[2] = {&c->ndel, &c->ndel2};
     uint32_t bound = pending;    // deletions the next update consumes, at most
     uint32_t last_nd = pending;  // deletions of the last recorded iteration
-    int open = -1;               // slot of the queued iteration whose count is unread
-    uint32_t open_iter = 0;
+    // queued iterations whose counts are unread, oldest first: (slot, iteration)
+    std::deque<std::pair<int, uint32_t>> open;
     ended = false;
+    uint32_t r = 0;  // iterations queued since entry
     auto queue = [&](int s) -> int {
-        KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, bound, src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                      sptr_b[acur].as<uint32_t>(), dws, c, stream));
+        const uint32_t in = r & 1u, out = in ^ 1u;
+        // task list of the pending deletions; clears the mark's count
+        KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], bound, src[cur].as<uint32_t>(),
+                                      dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws, c, stream,
+                                      counts[out]));
         KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
-        KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, bound, snbr_b[acur].as<uint32_t>(),
+        KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream));
         KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
-        KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, bound, dst[cur].as<uint32_t>(),
-                                         snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
-                                         stream));
-        KT_HIP_TRY(dev::fill_u32(&c->ndel, 0, 1, stream));
+        // the mark also finalizes the deletions just updated
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
-                                   tau.as<uint32_t>(), dl_p, E, k, snbr_b[acur].as<uint32_t>(),
-                                   sa.as<uint32_t>(), sb.as<uint32_t>(), c, stream));
-        KT_HIP_TRY(hipMemcpyAsync(h_ra + s, &c->ndel, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+                                   tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
+                                   sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
+                                   bound, stream));
+        KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[out], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
+        ++r;
         return KT_OK;
     };
-    // reads the open iteration back and records it
+    // reads the oldest queued iteration back and records it
     auto finish = [&](uint32_t& nd) -> int {
-        KT_HIP_TRY(hipEventSynchronize(ev_ra[open][2]));
-        nd = h_ra[open];
+        const int s = open.front().first;
+        const uint32_t it = open.front().second;
+        open.pop_front();
+        KT_HIP_TRY(hipEventSynchronize(ev_ra[s][2]));
+        nd = h_ra[s];
         float ms = 0;
-        KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[open][0], ev_ra[open][1]));
+        KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[s][0], ev_ra[s][1]));
         stats.ms_delta += ms;
         phase_ms += ms;
         stats.count_passes++;
@@ -848,36 +859,37 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         stats.delta_edges += last_nd;  // its update consumed the previous iteration's deletions
         last_pass_delta = true;
         last_nd = nd;
-        const int rc = record_iteration(k, open_iter, nd, 0, ms);
-        open = -1;
-        return rc;
+        iter = it + 1;
+        return record_iteration(k, it, nd, 0, ms);
     };
-    for (;;) {
-        const int s = (int)(iter & 1u);
-        int rc = queue(s);  // iteration `iter`
-        if (rc) return rc;
-        if (open >= 0) {  // read iteration iter - 1 while iteration `iter` runs
-            const uint32_t prev_iter = open_iter;
-            uint32_t nd = 0;
-            if ((rc = finish(nd))) return rc;
-            if (nd == 0) {  // the phase ended there; iteration `iter` is void
-                KT_HIP_TRY(hipEventSynchronize(ev_ra[s][2]));
-                pending = h_ra[s];
-                iter = prev_iter + 1;
-                ended = true;
-                return pending ? fail(KT_ERR_STATE, "void DELTA iteration deleted entries") : KT_OK;
-            }
+    // the phase ended at the iteration just read: the younger queued ones are void
+    auto drain_void = [&]() -> int {
+        for (const auto& o : open) {
+            KT_HIP_TRY(hipEventSynchronize(ev_ra[o.first][2]));
+            if (h_ra[o.first]) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
         }
-        open = s;
-        open_iter = iter++;
-        bound = (uint32_t)std::min<uint64_t>(live, E);  // the open iteration deletes live entries only
-        if (compaction_due()) {  // read the open iteration back and leave for the compaction
+        open.clear();
+        pending = 0;
+        ended = true;
+        return KT_OK;
+    };
+    uint32_t next = iter;  // the next iteration to queue
+    for (;;) {
+        const int s = (int)(next % kAheadSlots);
+        int rc = queue(s);
+        if (rc) return rc;
+        open.emplace_back(s, next++);
+        bound = (uint32_t)std::min<uint64_t>(live, E);  // a queued iteration deletes live entries only
+        const bool leave = compaction_due();
+        while (open.size() > (leave ? 0u : kAhead)) {
             uint32_t nd = 0;
             if ((rc = finish(nd))) return rc;
-            pending = nd;
-            iter = open_iter + 1;
-            ended = nd == 0;
-            return KT_OK;
+            if (nd == 0) return drain_void();
+        }
+        if (leave) {  // everything read back: the compaction follows in the caller
+            pending = last_nd;
+            if (r & 1u) std::swap(dl.p, dl_alt.p);  // the pending list becomes dl
+            return set_counter(offsetof(DevCounters, ndel), pending);
         }
     }
 }
@@ -938,7 +950,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
                                    snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
-                                   ctr.as<DevCounters>(), stream));
+                                   &ctr.as<DevCounters>()->ndel, nullptr, nullptr, 0, stream));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;

@@ -53,9 +53,11 @@ struct Plan {
     DevCounters* h_ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
-    // run-ahead DELTA iterations: two slots of (update start, update end,
-    // count read-back) events and the pinned deletion counts
-    hipEvent_t ev_ra[2][3] = {};
+    // run-ahead DELTA iterations: kAhead queued beyond the one being read
+    // back; per slot (update start, update end, count read-back) events and
+    // the pinned deletion count
+    static constexpr uint32_t kAhead = 1, kAheadSlots = kAhead + 2;
+    hipEvent_t ev_ra[kAheadSlots][3] = {};
     uint32_t* h_ra = nullptr;
 
     std::vector<std::array<uint64_t, 5>> trace;
@@ -85,7 +87,7 @@ struct Plan {
     DBuf sptr0, snbr0, seid0, sa0, sb0;
     int acur = 0;
     uint32_t S_cur = 0;
-    DBuf dl, sup_snap, dscratch, ctsum, cxs;
+    DBuf dl, dl_alt, sup_snap, dscratch, ctsum, cxs;  // dl_alt: second deletion list (run-ahead)
     dev::DeltaScratch dws{};
     bool delta_ready = false;
     bool last_pass_delta = false;  // the last support pass was a DELTA update
