@@ -190,11 +190,15 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
                                                   const uint32_t* __restrict__ dst,
                                                   const uint32_t* __restrict__ sptr,
                                                   uint4* __restrict__ meta, uint32_t* __restrict__ lofs,
-                                                  uint32_t* tsum, DevCounters* ctr, unsigned int* zero_cnt) {
+                                                  uint32_t* tsum, DevCounters* ctr, unsigned int* zero_cnt,
+                                                  uint32_t* host_cnt) {
     __shared__ uint32_t s_w[kBlock / kWave];
     __shared__ uint32_t s_last;
-    if (zero_cnt && blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     const uint32_t nd = *nd_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (zero_cnt) *zero_cnt = 0;
+        if (host_cnt) __hip_atomic_store(host_cnt, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     const uint32_t ntiles = (nd + kTileD - 1) / kTileD;
     const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     if (ntiles == 0) {
@@ -718,15 +722,12 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
-                         unsigned int* zero_cnt) {
-    if (!nd_max) {
-        if (zero_cnt) return fill_u32(zero_cnt, 0, 1, s);
-        return hipSuccess;
-    }
+                         unsigned int* zero_cnt, uint32_t* host_cnt) {
+    if (!nd_max && !zero_cnt && !host_cnt) return hipSuccess;
     const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
-    const uint32_t tb = tiles < 1024 ? tiles : 1024;
+    const uint32_t tb = tiles < 1024 ? (tiles ? tiles : 1u) : 1024u;
     hipLaunchKernelGGL(k_dprep, dim3(tb), dim3(kBlock), 0, s, dl, nd_dev, src, dst, sptr, ws.meta,
-                       ws.ntask, ws.tsum, ctr, zero_cnt);
+                       ws.ntask, ws.tsum, ctr, zero_cnt, host_cnt);
     return hipGetLastError();
 }
 

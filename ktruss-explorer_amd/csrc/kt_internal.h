@@ -155,11 +155,12 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
 // nd_dev: the number of pending deletions, read on the device (dl[0, *nd_dev));
 // nd_max bounds it and sizes the grids.  delta_prepare lists the tasks (one
 // kernel), delta_apply is the update proper (k_delta).
-// zero_cnt (optional): a counter set to 0 by the same kernel (the next mark's).
+// zero_cnt (optional): a counter set to 0 by the same kernel (the next mark's);
+// host_cnt (optional, device address of pinned host memory): receives *nd_dev.
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
-                         unsigned int* zero_cnt = nullptr);
+                         unsigned int* zero_cnt = nullptr, uint32_t* host_cnt = nullptr);
 // The update's statistics are summed per workgroup slot; delta_stats_fold
 // adds them to ctr->delta_* and clears the slots, delta_stats_reset clears
 // them (neither may be captured in a graph that replays before stats are read).

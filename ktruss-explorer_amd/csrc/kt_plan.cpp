@@ -152,7 +152,10 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     for (auto& ev : ev_sw) KT_HIP_TRY(hipEventCreate(&ev));
     for (auto& slot : ev_ra)
         for (auto& ev : slot) KT_HIP_TRY(hipEventCreate(&ev));
-    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, kAheadSlots * sizeof(uint32_t), hipHostMallocDefault));
+    // fine-grained: a kernel's system-scope store lands in host memory
+    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, kAheadSlots * sizeof(uint32_t),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+    KT_HIP_TRY(hipHostGetDevicePointer((void**)&d_ra, h_ra, 0));
 
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
     const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
@@ -823,33 +826,48 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     std::deque<std::pair<int, uint32_t>> open;
     ended = false;
     uint32_t r = 0;  // iterations queued since entry
+    int prev_slot = -1;
+    // iteration r: task list (which also clears the mark's count and writes
+    // the previous iteration's deletion count to pinned memory), update, mark
+    // (which also finalizes the deletions just updated)
     auto queue = [&](int s) -> int {
         const uint32_t in = r & 1u, out = in ^ 1u;
-        // task list of the pending deletions; clears the mark's count
         KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], bound, src[cur].as<uint32_t>(),
                                       dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws, c, stream,
-                                      counts[out]));
+                                      counts[out], prev_slot >= 0 ? d_ra + prev_slot : nullptr));
         KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
         KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream));
         KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
-        // the mark also finalizes the deletions just updated
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                    tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
                                    bound, stream));
-        KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[out], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
+        prev_slot = s;
         ++r;
+        return KT_OK;
+    };
+    // deletion count of the i-th queued iteration: written by its successor's
+    // task list, or, for the youngest, copied back now
+    auto read_count = [&](size_t i, uint32_t& nd) -> int {
+        const int s = open[i].first;
+        if (i + 1 < open.size()) {
+            KT_HIP_TRY(hipEventSynchronize(ev_ra[open[i + 1].first][0]));
+        } else {
+            KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[r & 1u], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
+            KT_HIP_TRY(hipEventSynchronize(ev_ra[s][2]));
+        }
+        nd = h_ra[s];
         return KT_OK;
     };
     // reads the oldest queued iteration back and records it
     auto finish = [&](uint32_t& nd) -> int {
         const int s = open.front().first;
         const uint32_t it = open.front().second;
+        int rc = read_count(0, nd);
+        if (rc) return rc;
         open.pop_front();
-        KT_HIP_TRY(hipEventSynchronize(ev_ra[s][2]));
-        nd = h_ra[s];
         float ms = 0;
         KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[s][0], ev_ra[s][1]));
         stats.ms_delta += ms;
@@ -864,9 +882,11 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     };
     // the phase ended at the iteration just read: the younger queued ones are void
     auto drain_void = [&]() -> int {
-        for (const auto& o : open) {
-            KT_HIP_TRY(hipEventSynchronize(ev_ra[o.first][2]));
-            if (h_ra[o.first]) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
+        for (size_t i = 0; i < open.size(); ++i) {
+            uint32_t nd = 0;
+            const int rc = read_count(i, nd);
+            if (rc) return rc;
+            if (nd) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
         }
         open.clear();
         pending = 0;

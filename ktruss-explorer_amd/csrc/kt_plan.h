@@ -59,6 +59,7 @@ struct Plan {
     static constexpr uint32_t kAhead = 1, kAheadSlots = kAhead + 2;
     hipEvent_t ev_ra[kAheadSlots][3] = {};
     uint32_t* h_ra = nullptr;
+    uint32_t* d_ra = nullptr;  // h_ra's device address
 
     std::vector<std::array<uint64_t, 5>> trace;
     kt_stats stats{};
