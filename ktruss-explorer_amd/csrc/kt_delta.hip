@@ -305,9 +305,11 @@ __device__ __forceinline__ uint32_t first_task(const uint32_t* lofs, const uint3
 // the ownership test reads no flags of the oriented layout: a triangle whose
 // other two edges survive (the common case) costs two slot->entry reads and
 // two atomic decrements.
+constexpr uint32_t kDBlock = 256;  // k_delta workgroup (512 and 1024 measured the same)
 constexpr uint32_t kStage = 512;
 constexpr uint32_t kDeltaGrid = 2048;  // most k_delta workgroups per launch
 constexpr size_t kStatBytes = (size_t)kDeltaGrid * 4 * 8;
+static_assert(8192 / (kDBlock / kWave) <= kDeltaGrid, "k_delta grids of 8192 waves need a slot per workgroup");
 constexpr uint32_t kSample = 512;
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -401,7 +403,7 @@ __device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t 
     return q;
 }
 
-__global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ dl,
+__global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
                                                   const uint32_t* __restrict__ lofs,
@@ -410,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
                                                   const uint32_t* __restrict__ seid,
                                                   uint32_t* sup, DevCounters* ctr,
                                                   unsigned long long* __restrict__ bstat) {
-    __shared__ uint32_t s_buf[(kBlock / kWave) * (kStage + kSample)];
+    __shared__ uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample)];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample);
     uint32_t* smp = buf + kStage;
@@ -439,9 +441,14 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
         acc_elems += min(kWave, m.y - j0);
         // a longer list that fits the slice is staged with the candidates'
         // load in flight
+        // a longer list that fits the slice is staged, or the first ids of its
+        // 64-entry blocks are, with the candidates' load in flight
+        const uint32_t nb = (m.w + kWave - 1) / kWave;
         const bool fresh = m.w <= kStage && full != i;
+        const bool fresh_smp = m.w > kStage && nb <= kSample && sampled != i;
         Staged<kStage> st;
         if (fresh) st.load(snbr + m.z, m.w, 1u, lane);
+        else if (fresh_smp) st.load(snbr + m.z, nb, kWave, lane);
         const uint32_t x = j < m.y ? snbr[j] : KT_DEAD;
         const bool cand = !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
@@ -450,6 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
             st.store(buf, m.w, 0xFFFFFFFFu, lane);
             full = i;
             acc_win += m.w;
+        } else if (fresh_smp) {
+            st.store(smp, nb, KT_IDMASK, lane);
+            sampled = i;
+            acc_win += nb;
         }
         if (!cm) continue;
         const uint32_t xv = x & KT_IDMASK;
@@ -461,15 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
             const uint32_t xmin = __builtin_amdgcn_readlane(xv, (uint32_t)__ffsll((long long)cm) - 1);
             const uint32_t xmax = __builtin_amdgcn_readlane(xv, 63u - (uint32_t)__builtin_clzll(cm));
             uint32_t lo, hi;
-            const uint32_t nb = (m.w + kWave - 1) / kWave;
             if (nb <= kSample) {
-                if (sampled != i) {
-                    Staged<kSample> ss;
-                    ss.load(snbr + m.z, nb, kWave, lane);
-                    ss.store(smp, nb, KT_IDMASK, lane);
-                    sampled = i;
-                    acc_win += nb;
-                }
                 // blocks whose first id is below xmin / xmax + 1: the bounds lie
                 // in the last such block (or at the list start)
                 const uint32_t ca = count_less_lds(smp, nb, xmin), cb = count_less_lds(smp, nb, xmax + 1);
@@ -516,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
     // most kDeltaGrid workgroups and launches on one stream do not overlap);
     // counters every wave bumps with an atomic serialize on their one address
     // and hold back the whole launch
-    __shared__ unsigned long long s_stat[kBlock / kWave][4];
+    __shared__ unsigned long long s_stat[kDBlock / kWave][4];
     if (lane == 0) {
         unsigned long long* q = s_stat[threadIdx.x / kWave];
         q[0] = acc_elems;
@@ -527,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const uint32_t* __restrict__ d
     __syncthreads();
     if (threadIdx.x < 4) {
         unsigned long long v = 0;
-        for (uint32_t k = 0; k < kBlock / kWave; ++k) v += s_stat[k][threadIdx.x];
+        for (uint32_t k = 0; k < kDBlock / kWave; ++k) v += s_stat[k][threadIdx.x];
         if (v) bstat[blockIdx.x * 4 + threadIdx.x] += v;
     }
 }
@@ -735,8 +738,12 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
     if (!nd_max) return hipSuccess;
-    const uint32_t db = nd_max < 64 ? 64 : (nd_max > kDeltaGrid ? kDeltaGrid : nd_max);
-    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, ws.tsum, snbr,
+    // four waves per deletion, 256 to 8192 waves
+    const uint32_t wpb = kDBlock / kWave;
+    const uint64_t want = (uint64_t)nd_max * 4;
+    const uint32_t w = want < 256 ? 256u : (want > 8192 ? 8192u : (uint32_t)want);
+    const uint32_t db = (w + wpb - 1) / wpb;
+    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, ws.tsum, snbr,
                        seid, sup, ctr, ws.bstat);
     return hipGetLastError();
 }
