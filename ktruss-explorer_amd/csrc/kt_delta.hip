@@ -728,7 +728,7 @@ hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd
                          unsigned int* zero_cnt, uint32_t* host_cnt) {
     if (!nd_max && !zero_cnt && !host_cnt) return hipSuccess;
     const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
-    const uint32_t tb = tiles < 1024 ? (tiles ? tiles : 1u) : 1024u;
+    const uint32_t tb = tiles < 256 ? (tiles ? tiles : 1u) : 256u;
     hipLaunchKernelGGL(k_dprep, dim3(tb), dim3(kBlock), 0, s, dl, nd_dev, src, dst, sptr, ws.meta,
                        ws.ntask, ws.tsum, ctr, zero_cnt, host_cnt);
     return hipGetLastError();
