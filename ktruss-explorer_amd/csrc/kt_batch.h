@@ -56,7 +56,7 @@ hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, ui
                       hipStream_t s);
 // dl != nullptr: DELTA marking (KT_DYING on the entry and its two adjacency
 // slots snbr[sa[e]], snbr[sb[e]]; deleted entries appended to dl, count *dn).
-hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
+hipError_t batch_mark(const TaskList& tl, uint32_t* dst, uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
                       unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                       uint32_t blocks, hipStream_t s);

@@ -61,6 +61,7 @@ __global__ void k_b_zero(TaskList tl, const uint32_t* __restrict__ dst, uint32_t
 // wave whose slice fills reserves for it alone): a reservation per 64-entry
 // chunk would put one returning atomic per chunk on a single address.
 constexpr uint32_t kBMarkCap = 1024;  // LDS list entries per wave
+constexpr uint32_t kMarkGroup = 8;    // 64-entry chunks whose loads are in flight together
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -70,7 +71,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 template <bool DELTA>
 __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restrict__ dst,
-                                                const uint32_t* __restrict__ sup,
+                                                uint32_t* __restrict__ sup,
                                                 uint32_t* __restrict__ tau, const uint32_t* __restrict__ gk,
                                                 uint32_t* gndel, uint32_t* gmin, uint32_t* __restrict__ dl,
                                                 unsigned int* dn, uint32_t* __restrict__ snbr,
@@ -87,23 +88,8 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
         const uint4 d = tl.desc[t];
         const uint32_t k = gk[d.z];
         uint32_t ndel = 0, smin = 0xFFFFFFFFu;
-        for (uint32_t base = d.x; base < d.y; base += kWave) {
-            const uint32_t i = base + lane;
-            bool del = false;
-            uint32_t v = 0;
-            if (i < d.y) {
-                v = dst[i];
-                if (!(v & KT_DEAD)) {
-                    const uint32_t s = sup[i];
-                    if (s < k - 2) {
-                        del = true;
-                        dst[i] = v | (DELTA ? KT_DYING : KT_DEAD);
-                        tau[i] = k - 1;
-                    } else if (s < smin) {
-                        smin = s;
-                    }
-                }
-            }
+        // the deletions of one 64-entry chunk: LDS list (DELTA), counts
+        auto take = [&](uint32_t i, bool del) {
             const uint64_t m = ballot(del);
             if (DELTA && m) {
                 if (cnt + kWave > kBMarkCap) {  // the slice is full: reserve for it alone
@@ -124,6 +110,57 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
                 cnt += (uint32_t)__popcll(m);
             }
             ndel += (uint32_t)__popcll(m);
+        };
+        if (DELTA) {
+            // supports only (a deleted entry's support is KT_SUP_GONE), eight
+            // chunks in flight; flags read where some lane is below k - 2
+            const uint32_t last = d.y - 1;
+            for (uint32_t base = d.x; base < d.y; base += kMarkGroup * kWave) {
+                uint32_t sp[kMarkGroup], v[kMarkGroup];
+#pragma unroll
+                for (uint32_t j = 0; j < kMarkGroup; ++j) {
+                    const uint32_t i = base + j * kWave + lane;
+                    const uint32_t x = sup[i < last ? i : last];
+                    sp[j] = i < d.y ? x : KT_SUP_GONE;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kMarkGroup; ++j) {
+                    const uint32_t i = base + j * kWave + lane;
+                    v[j] = ballot(sp[j] < k - 2) ? dst[i < last ? i : last] : 0u;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kMarkGroup; ++j) {
+                    const uint32_t i = base + j * kWave + lane;
+                    const bool del = sp[j] < k - 2 && !(v[j] & (KT_DEAD | KT_DYING));
+                    if (del) {
+                        dst[i] = v[j] | KT_DYING;
+                        sup[i] = KT_SUP_GONE;
+                        tau[i] = k - 1;
+                    } else if (sp[j] >= k - 2 && sp[j] < smin) {
+                        smin = sp[j];
+                    }
+                    take(i, del);
+                }
+            }
+        } else {
+            for (uint32_t base = d.x; base < d.y; base += kWave) {
+                const uint32_t i = base + lane;
+                bool del = false;
+                if (i < d.y) {
+                    const uint32_t v = dst[i];
+                    if (!(v & KT_DEAD)) {
+                        const uint32_t s = sup[i];
+                        if (s < k - 2) {
+                            del = true;
+                            dst[i] = v | KT_DEAD;
+                            tau[i] = k - 1;
+                        } else if (s < smin) {
+                            smin = s;
+                        }
+                    }
+                }
+                take(i, del);
+            }
         }
         smin = wave_min(smin);
         if (lane == 0) {
@@ -346,7 +383,7 @@ hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, ui
     return hipGetLastError();
 }
 
-hipError_t batch_mark(const TaskList& tl, uint32_t* dst, const uint32_t* sup, uint32_t* tau,
+hipError_t batch_mark(const TaskList& tl, uint32_t* dst, uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
                       unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                       uint32_t blocks, hipStream_t s) {

@@ -67,7 +67,6 @@ __device__ __forceinline__ void finalize_one(uint32_t d, uint32_t* dst, uint32_t
     snbr[b] = (snbr[b] & ~KT_DYING) | KT_DEAD;
 }
 
-constexpr uint32_t KT_SUP_GONE = 0xFFFFFFFFu;  // support of an entry the mark deleted
 constexpr uint32_t kMarkItems = 8;
 constexpr uint32_t kMarkTile = kBlock * kMarkItems;
 constexpr uint32_t kMarkBuf = 2 * kMarkTile;
@@ -95,7 +94,9 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                                                        const uint32_t* __restrict__ sa,
                                                        const uint32_t* __restrict__ sb, unsigned int* ndel,
                                                        const uint32_t* __restrict__ fin_dl,
-                                                       const uint32_t* __restrict__ fin_nd) {
+                                                       const uint32_t* __restrict__ fin_nd,
+                                                       const uint32_t* __restrict__ src,
+                                                       uint32_t* __restrict__ ldeg) {
     __shared__ uint32_t s_out[kMarkBuf];
     __shared__ uint32_t s_cnt, s_base;
     if (threadIdx.x == 0) s_cnt = 0;
@@ -142,6 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                 const uint32_t a = sa[e], c = sb[e];
                 snbr[a] |= KT_DYING;
                 snbr[c] |= KT_DYING;
+                if (ldeg) atomicSub(ldeg + src[e], 1u);
             }
         }
         __syncthreads();
@@ -680,6 +682,11 @@ __global__ void k_cmp_slots(const uint32_t* __restrict__ snbr2, const uint32_t* 
     }
 }
 
+__global__ void k_row_lengths(const uint32_t* __restrict__ ptr, uint32_t nv, uint32_t* __restrict__ out) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
+        out[v] = ptr[v + 1] - ptr[v];
+}
+
 }  // namespace
 
 namespace dev {
@@ -687,7 +694,8 @@ namespace dev {
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
-                      const uint32_t* fin_nd, uint32_t fin_max, hipStream_t s) {
+                      const uint32_t* fin_nd, uint32_t fin_max, const uint32_t* src, uint32_t* ldeg,
+                      hipStream_t s) {
     if (!fin_max) fin_dl = nullptr;
     if (!n) {
         if (fin_dl) return finalize_deleted(fin_dl, fin_nd, fin_max, dst, snbr, sa, sb, s);
@@ -695,7 +703,7 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
     }
     const uint32_t nt = (n + kMarkTile - 1) / kMarkTile;
     hipLaunchKernelGGL(k_mark_dying, dim3(nt < kMarkGrid ? nt : kMarkGrid), dim3(kBlock), 0, s, dst,
-                       eid, sup, tau, dl, n, k, snbr, sa, sb, ndel, fin_dl, fin_nd);
+                       eid, sup, tau, dl, n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg);
     return hipGetLastError();
 }
 
@@ -745,6 +753,13 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
     const uint32_t db = (w + wpb - 1) / wpb;
     hipLaunchKernelGGL(k_delta, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, ws.tsum, snbr,
                        seid, sup, ctr, ws.bstat);
+    return hipGetLastError();
+}
+
+hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s) {
+    if (!nv) return hipSuccess;
+    const uint32_t b = blocks_for(nv, kBlock);
+    hipLaunchKernelGGL(k_row_lengths, dim3(b < 2048 ? b : 2048), dim3(kBlock), 0, s, ptr, nv, out);
     return hipGetLastError();
 }
 

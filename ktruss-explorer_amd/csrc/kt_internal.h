@@ -52,6 +52,10 @@ int fail(int status, const std::string& msg);
     } while (0)
 
 // Counters the device updates and the host reads back after each iteration.
+// Support of an entry a DELTA mark deleted: above every threshold, so later
+// marks find it without reading its flags.
+constexpr uint32_t KT_SUP_GONE = 0xFFFFFFFFu;
+
 struct DevCounters {
     unsigned long long merge_elems;   // sum of L_u + L_v over recounted entries
     unsigned long long recount_edges; // recounted entries
@@ -134,11 +138,16 @@ hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E
 // DELTA support maintenance (kt_delta.hip).
 // The deletions are appended to dl, counted in *ndel (zero on entry).  fin_dl
 // / fin_nd (optional, at most fin_max entries): a previous deletion list
-// whose update is done, finalized (KT_DEAD) by the same kernel.
+// whose update is done, finalized (KT_DEAD) by the same kernel.  ldeg
+// (optional): live length of every oriented row, decremented at src[e] for
+// every deleted entry e.
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
-                      const uint32_t* fin_nd, uint32_t fin_max, hipStream_t s);
+                      const uint32_t* fin_nd, uint32_t fin_max, const uint32_t* src, uint32_t* ldeg,
+                      hipStream_t s);
+// out[v] = ptr[v + 1] - ptr[v] for v < nv.
+hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying).
 hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
                           hipStream_t s);

@@ -64,11 +64,22 @@ __device__ void block_sum_store(unsigned long long (&v)[N], bool is_max_3,
     }
 }
 
-__global__ __launch_bounds__(kFeatThreads) void k_feat_rows(const uint32_t* __restrict__ ptr,
-                                                           uint32_t n, unsigned long long* part) {
+// Entries are sampled in 64-entry chunks, one chunk every `stride` (every
+// entry up to kFeatSample entries; the histogram is normalised and the live
+// count rescaled), so the features cost a bounded, coalesced pass however
+// large the graph.  One kernel: the rows (live lengths when given, else the
+// stored ones) and the sampled entries (dst, sup).
+constexpr uint32_t kFeatSample = 1u << 22;
+
+__global__ __launch_bounds__(kFeatThreads) void k_feat_scan(const uint32_t* __restrict__ ptr,
+                                                           const uint32_t* __restrict__ rowlen, uint32_t n,
+                                                           const uint32_t* __restrict__ dst,
+                                                           const uint32_t* __restrict__ sup, uint32_t E,
+                                                           uint32_t stride, uint32_t k, int sup_valid,
+                                                           unsigned long long* part) {
     unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // rows, s1, s2, max, big, huge
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        const uint32_t len = ptr[r + 1] - ptr[r];
+        const uint32_t len = rowlen ? rowlen[r] : ptr[r + 1] - ptr[r];
         if (!len) continue;
         v[0] += 1;
         v[1] += len;
@@ -78,66 +89,26 @@ __global__ __launch_bounds__(kFeatThreads) void k_feat_rows(const uint32_t* __re
         if (len > 1024) v[5] += len;
     }
     block_sum_store<6>(v, true, part + (size_t)blockIdx.x * KT_FEAT_ACC);
-}
-
-// Entries are sampled with a fixed stride (every entry up to kFeatSample
-// entries; the histogram is normalised and the live count rescaled), so the
-// features cost a bounded pass however large the graph.
-constexpr uint32_t kFeatSample = 1u << 22;
-
-__global__ __launch_bounds__(kFeatThreads) void k_feat_support(const uint32_t* __restrict__ dst,
-                                                              const uint32_t* __restrict__ sup,
-                                                              uint32_t E, uint32_t stride, uint32_t k,
-                                                              int sup_valid,
-                                                              unsigned long long* part) {
-    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries (sampled)
+    __syncthreads();  // block_sum_store's LDS is reused below
+    unsigned long long w[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries (sampled)
     const float denom = (float)(k > 2 ? k - 2 : 1);
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x * stride;
-    for (uint64_t e = (uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * stride; e < E; e += step) {
-        if (dst[e] & (KT_DEAD | KT_DYING)) continue;
-        v[5] += 1;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t cstep = (uint64_t)gridDim.x * (blockDim.x / 64) * stride * 64;
+    for (uint64_t c = ((uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * stride * 64; c < E;
+         c += cstep) {
+        const uint64_t e = c + lane;
+        if (e >= E || (dst[e] & (KT_DEAD | KT_DYING))) continue;
+        w[5] += 1;
         if (sup_valid) {
             const float r = (float)sup[e] / denom;
             const int b = r < 1.f ? 0 : r < 2.f ? 1 : r < 4.f ? 2 : r < 8.f ? 3 : 4;
-            v[b] += 1;
+            w[b] += 1;
         }
     }
-    block_sum_store<6>(v, false, part + (size_t)blockIdx.x * KT_FEAT_ACC + 6);
+    block_sum_store<6>(w, false, part + (size_t)blockIdx.x * KT_FEAT_ACC + 6);
 }
 
 __device__ float log2p1(float x) { return log2f(1.f + x); }
-
-__global__ void k_feat_finish(const unsigned long long* part, uint32_t E0, uint32_t stride,
-                              uint32_t k, int sup_valid, float* feat) {
-    __shared__ unsigned long long acc[KT_FEAT_ACC];
-    if (threadIdx.x < KT_FEAT_ACC) {
-        const int j = threadIdx.x;
-        unsigned long long r = 0;
-        for (uint32_t b = 0; b < kFeatBlocks; ++b) {
-            const unsigned long long x = part[(size_t)b * KT_FEAT_ACC + j];
-            r = j == 3 ? (x > r ? x : r) : r + x;
-        }
-        acc[j] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const float rows = (float)acc[0], s1 = (float)acc[1], s2 = (float)acc[2];
-    const float mean = rows > 0 ? s1 / rows : 0.f;
-    feat[0] = log2p1(s1) / 32.f;
-    feat[1] = log2p1(rows) / 32.f;
-    feat[2] = mean / 256.f;
-    feat[3] = log2p1((float)acc[3]) / 16.f;
-    feat[4] = mean > 0 ? sqrtf(s2 / rows) / mean / 8.f : 0.f;
-    feat[5] = s1 > 0 ? (float)acc[4] / s1 : 0.f;
-    feat[6] = s1 > 0 ? (float)acc[5] / s1 : 0.f;
-    feat[7] = E0 ? fminf(1.f, (float)acc[11] * (float)stride / (float)E0) : 0.f;  // live fraction
-    feat[8] = log2f((float)k) / 10.f;
-    feat[9] = s1 > 0 ? s2 / s1 / 1024.f : 0.f;
-    float tot = 0;
-    for (int b = 0; b < 5; ++b) tot += (float)acc[6 + b];
-    for (int b = 0; b < 5; ++b) feat[10 + b] = (sup_valid && tot > 0) ? (float)acc[6 + b] / tot : 0.f;
-    feat[15] = sup_valid ? 1.f : 0.f;
-}
 
 // ---------------------------------------------------------------------------
 // MLP forward for one sample in one wave; activations land in LDS.
@@ -262,16 +233,54 @@ __global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float
     }
 }
 
-// Inference inside the peel loop: forward on the phase features, argmin of
-// the predicted costs written to *choice.
-__global__ __launch_bounds__(64) void k_mlp_pick(const float* __restrict__ P, int C,
-                                                 const float* __restrict__ R,
-                                                 const float* __restrict__ feat,
-                                                 unsigned int* choice) {
+// The phase features from the workgroup partials (one wave); with P, the
+// model's pick on them as well (one launch per phase).
+__global__ __launch_bounds__(64) void k_feat_finish(const unsigned long long* part, uint32_t E0,
+                                                    uint32_t stride, uint32_t k, int sup_valid,
+                                                    float* feat, const float* __restrict__ P, int C,
+                                                    const float* __restrict__ R, unsigned int* choice) {
+    __shared__ unsigned long long acc[KT_FEAT_ACC];
     __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
     const int lane = threadIdx.x;
-    if (lane < kIn) s_x[lane] = clamp_in(feat[lane], R, lane);
+    // every lane folds kFeatBlocks / 64 workgroups, then the wave reduces
+    for (int j = 0; j < KT_FEAT_ACC; ++j) {
+        unsigned long long r = 0;
+        for (uint32_t b = lane; b < kFeatBlocks; b += 64) {
+            const unsigned long long x = part[(size_t)b * KT_FEAT_ACC + j];
+            r = j == 3 ? (x > r ? x : r) : r + x;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(r, off);
+            r = j == 3 ? (o > r ? o : r) : r + o;
+        }
+        if (lane == 0) acc[j] = r;
+    }
     __syncthreads();
+    if (lane == 0) {
+        float f[kIn];
+        const float rows = (float)acc[0], s1 = (float)acc[1], s2 = (float)acc[2];
+        const float mean = rows > 0 ? s1 / rows : 0.f;
+        f[0] = log2p1(s1) / 32.f;
+        f[1] = log2p1(rows) / 32.f;
+        f[2] = mean / 256.f;
+        f[3] = log2p1((float)acc[3]) / 16.f;
+        f[4] = mean > 0 ? sqrtf(s2 / rows) / mean / 8.f : 0.f;
+        f[5] = s1 > 0 ? (float)acc[4] / s1 : 0.f;
+        f[6] = s1 > 0 ? (float)acc[5] / s1 : 0.f;
+        f[7] = E0 ? fminf(1.f, (float)acc[11] * (float)stride / (float)E0) : 0.f;  // live fraction
+        f[8] = log2f((float)k) / 10.f;
+        f[9] = s1 > 0 ? s2 / s1 / 1024.f : 0.f;
+        float tot = 0;
+        for (int b = 0; b < 5; ++b) tot += (float)acc[6 + b];
+        for (int b = 0; b < 5; ++b) f[10 + b] = (sup_valid && tot > 0) ? (float)acc[6 + b] / tot : 0.f;
+        f[15] = sup_valid ? 1.f : 0.f;
+        for (int i = 0; i < kIn; ++i) {
+            feat[i] = f[i];
+            s_x[i] = P ? clamp_in(f[i], R, i) : 0.f;
+        }
+    }
+    __syncthreads();
+    if (!P) return;  // uniform
     mlp_forward_wave(P, C, s_x, s_h1, s_h2, s_o, lane);
     if (lane == 0) {
         int best = 0;
@@ -287,18 +296,17 @@ namespace dev {
 
 uint32_t mlp_num_params(uint32_t C) { return (uint32_t)layout((int)C).total; }
 
-hipError_t phase_features(const uint32_t* ptr, const uint32_t* dst, const uint32_t* sup, uint32_t n,
-                          uint32_t E, uint32_t E0, uint32_t k, bool sup_valid,
-                          unsigned long long* acc, float* feat, hipStream_t s) {
-    hipLaunchKernelGGL(k_feat_rows, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, ptr, n, acc);
+hipError_t phase_features(const uint32_t* ptr, const uint32_t* rowlen, const uint32_t* dst,
+                          const uint32_t* sup, uint32_t n, uint32_t E, uint32_t E0, uint32_t k,
+                          bool sup_valid, unsigned long long* acc, float* feat, const float* P,
+                          uint32_t C, const float* R, unsigned int* choice, hipStream_t s) {
+    const uint32_t stride = E > kFeatSample ? (E + kFeatSample - 1) / kFeatSample : 1u;
+    hipLaunchKernelGGL(k_feat_scan, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, ptr, rowlen, n, dst, sup, E,
+                       stride, k, sup_valid ? 1 : 0, acc);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    const uint32_t stride = E > kFeatSample ? (E + kFeatSample - 1) / kFeatSample : 1u;
-    hipLaunchKernelGGL(k_feat_support, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, dst, sup, E, stride,
-                       k, sup_valid ? 1 : 0, acc);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
     hipLaunchKernelGGL(k_feat_finish, dim3(1), dim3(64), 0, s, acc, E0, stride, k, sup_valid ? 1 : 0,
-                       feat);
+                       feat, P, (int)C, R, choice);
     return hipGetLastError();
 }
 
@@ -329,12 +337,6 @@ hipError_t adam_step(float* P, const float* G, float* m1, float* m2, uint32_t to
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     hipLaunchKernelGGL(k_adam, dim3((total + 255) / 256), dim3(256), 0, s, P, G, m1, m2, total, lr,
                        b1, b2, eps, c1, c2);
-    return hipGetLastError();
-}
-
-hipError_t mlp_pick(const float* P, uint32_t C, const float* R, const float* feat, unsigned int* choice,
-                    hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_pick, dim3(1), dim3(64), 0, s, P, (int)C, R, feat, choice);
     return hipGetLastError();
 }
 

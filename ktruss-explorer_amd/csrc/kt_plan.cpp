@@ -83,7 +83,6 @@ int ensure_device() {
 
 Plan::~Plan() {
     if (h_ctr) (void)hipHostFree(h_ctr);
-    if (h_feat) (void)hipHostFree(h_feat);
     if (h_ra) (void)hipHostFree(h_ra);
     for (auto& slot : ev_ra)
         for (auto& ev : slot)
@@ -147,8 +146,6 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1}) KT_HIP_TRY(hipEventCreate(ev));
     KT_HIP_TRY(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocDefault));
     std::memset(h_ctr, 0, sizeof(DevCounters));
-    KT_HIP_TRY(hipHostMalloc((void**)&h_feat, KT_MLP_IN * sizeof(float), hipHostMallocDefault));
-    std::memset(h_feat, 0, KT_MLP_IN * sizeof(float));
     for (auto& ev : ev_sw) KT_HIP_TRY(hipEventCreate(&ev));
     for (auto& slot : ev_ra)
         for (auto& ev : slot) KT_HIP_TRY(hipEventCreate(&ev));
@@ -170,8 +167,7 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = xs.alloc(Eb + sizeof(uint32_t))) || (rc = ctr.alloc(sizeof(DevCounters))) ||
         (rc = ctr_scratch.alloc(sizeof(DevCounters))) || (rc = stripes.alloc(kStripeBytes)) ||
         (rc = stripes_x.alloc(kStripeBytes)) ||
-        (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))) ||
-        (rc = feat.alloc(KT_MLP_IN * sizeof(float))))
+        (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))))
         return rc;
     KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
     if (E0) {
@@ -230,6 +226,7 @@ int Plan::reset() {
     std::memset(h_ctr, 0, sizeof(DevCounters));
     trace.clear();
     phase_log.clear();
+    feat_on.clear();
     stats = kt_stats{};
     have_tau = false;
     return KT_OK;
@@ -338,23 +335,36 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
 
 // Phase features of the surviving graph (supports used only when `exact`)
 // and, in MODEL mode, the cost model's pick among the candidates.
-int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer) {
+int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer, bool live_rows) {
     cfg = fixed_cfg;
     chosen = 0;
-    if (pick == KT_PICK_FIXED) {  // no pick to make: the phase log gets no features
-        for (int f = 0; f < KT_MLP_IN; ++f) h_feat[f] = -1.f;
-        return KT_OK;
+    phase_has_feat = false;
+    if (pick == KT_PICK_FIXED) return KT_OK;  // no pick to make: the phase log gets no features
+    // the phase's features stay on the device (feat_log, one row per phase)
+    // until the run ends
+    const size_t ph = phase_log.size();
+    if (ph >= feat_cap) {
+        const uint32_t cap = std::max<uint32_t>(256, 2 * feat_cap);
+        DBuf grown;
+        int rc = grown.alloc((size_t)cap * KT_MLP_IN * sizeof(float));
+        if (rc) return rc;
+        if (feat_cap)
+            KT_HIP_TRY(hipMemcpyAsync(grown.p, feat_log.p, (size_t)feat_cap * KT_MLP_IN * sizeof(float),
+                                      hipMemcpyDeviceToDevice, stream));
+        KT_HIP_TRY(hipStreamSynchronize(stream));
+        std::swap(grown.p, feat_log.p);
+        std::swap(grown.size, feat_log.size);
+        feat_cap = cap;
     }
-    KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                   sup[cur].as<uint32_t>(), n, E, E0, k, exact,
-                                   feat_acc.as<unsigned long long>(), feat.as<float>(), stream));
-    cfg = fixed_cfg;
-    chosen = 0;
-    if (pick == KT_PICK_MODEL)
-        KT_HIP_TRY(dev::mlp_pick(model.as<float>(), model_C, model.as<float>() + model_n,
-                                 feat.as<float>(), &ctr.as<DevCounters>()->choice, stream));
-    KT_HIP_TRY(hipMemcpyAsync(h_feat, feat.p, KT_MLP_IN * sizeof(float), hipMemcpyDeviceToHost, stream));
-    if (pick == KT_PICK_MODEL && !defer) {
+    const bool by_model = pick == KT_PICK_MODEL;
+    KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), live_rows ? ldeg.as<uint32_t>() : nullptr,
+                                   dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), n, E, E0, k, exact,
+                                   feat_acc.as<unsigned long long>(), feat_log.as<float>() + ph * KT_MLP_IN,
+                                   by_model ? model.as<float>() : nullptr, model_C,
+                                   by_model ? model.as<float>() + model_n : nullptr,
+                                   &ctr.as<DevCounters>()->choice, stream));
+    phase_has_feat = true;
+    if (by_model && !defer) {
         const int rc = sync_counters();
         if (rc) return rc;
         chosen = std::min<uint32_t>(h_ctr->choice, (uint32_t)cands.size() - 1);
@@ -493,8 +503,8 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     rec[1] = (float)iter + (broke ? 1.f : 0.f);
     rec[2] = pick == KT_PICK_FIXED ? -1.f : (float)chosen;
     rec[3] = (float)phase_ms;
-    for (int f = 0; f < KT_MLP_IN; ++f) rec[4 + f] = h_feat[f];
-    phase_log.push_back(rec);
+    phase_log.push_back(rec);  // features: filled in by flush_phase_features
+    feat_on.push_back(phase_has_feat ? 1 : 0);
     stats.k_last = k;
     return KT_OK;
 }
@@ -529,9 +539,22 @@ int Plan::decompose() {
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
     if ((rc = sync_counters(true))) return rc;
+    if ((rc = flush_phase_features())) return rc;
     stats.ms_total = ms;
     finish_stats();
     have_tau = true;
+    return KT_OK;
+}
+
+// The phase features kept on the device go into the phase log.
+int Plan::flush_phase_features() {
+    const size_t np = phase_log.size();
+    if (!np || !feat_cap) return KT_OK;
+    std::vector<float> f(np * KT_MLP_IN);
+    KT_HIP_TRY(hipMemcpy(f.data(), feat_log.p, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < np; ++i)
+        if (feat_on[i])
+            for (int j = 0; j < KT_MLP_IN; ++j) phase_log[i][4 + j] = f[i * KT_MLP_IN + j];
     return KT_OK;
 }
 
@@ -568,6 +591,7 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
     if ((rc = sync_counters(true))) return rc;
+    if ((rc = flush_phase_features())) return rc;
     stats.ms_total = ms;
     finish_stats();
     std::vector<uint32_t> s(E), d(E);
@@ -632,7 +656,7 @@ int Plan::ensure_delta() {
             return rc;
     if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
         (rc = sa0.alloc(Eb)) || (rc = sb0.alloc(Eb)) || (rc = sa.alloc(Eb)) || (rc = sb.alloc(Eb)) ||
-        (rc = dl.alloc(Eb)) || (rc = dl_alt.alloc(Eb)) ||
+        (rc = dl.alloc(Eb)) || (rc = dl_alt.alloc(Eb)) || (rc = ldeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
         (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))) ||
         (rc = ctsum.alloc((size_t)ct * 4)) || (rc = cxs.alloc(2 * Eb + 4)))
         return rc;
@@ -663,6 +687,7 @@ int Plan::restore_delta() {
         KT_HIP_TRY(hipMemcpyAsync(sb.p, sb0.p, Eb, hipMemcpyDeviceToDevice, stream));
     }
     KT_HIP_TRY(dev::delta_stats_reset(dws, stream));
+    KT_HIP_TRY(dev::row_lengths(ptr0.as<uint32_t>(), n, ldeg.as<uint32_t>(), stream));
     return KT_OK;
 }
 
@@ -842,7 +867,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                    tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
-                                   bound, stream));
+                                   bound, src[cur].as<uint32_t>(), ldeg.as<uint32_t>(), stream));
         prev_slot = s;
         ++r;
         return KT_OK;
@@ -932,7 +957,10 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     // the model's choice is read with the next counter read-back (the JUMP's
     // or, when deletions are pending, right before the support pass)
     bool unresolved = pick == KT_PICK_MODEL;
-    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen, true))) return rc;
+    // a compaction that fell due in the last phase happens before the phase's
+    // features are taken (they describe the stored rows)
+    if ((rc = compact_delta_state())) return rc;
+    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen, true, true))) return rc;
     auto resolve_choice = [&]() {
         chosen = std::min<uint32_t>(h_ctr->choice, nc - 1);
         cfg = cands[chosen];
@@ -970,7 +998,8 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
                                    snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
-                                   &ctr.as<DevCounters>()->ndel, nullptr, nullptr, 0, stream));
+                                   &ctr.as<DevCounters>()->ndel, nullptr, nullptr, 0, src[cur].as<uint32_t>(),
+                                   ldeg.as<uint32_t>(), stream));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;
@@ -1022,8 +1051,8 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     rec[1] = (float)iter + (broke ? 1.f : 0.f);
     rec[2] = pick == KT_PICK_FIXED ? -1.f : (float)chosen;
     rec[3] = (float)phase_ms;
-    for (int f = 0; f < KT_MLP_IN; ++f) rec[4 + f] = h_feat[f];
-    phase_log.push_back(rec);
+    phase_log.push_back(rec);  // features: filled in by flush_phase_features
+    feat_on.push_back(phase_has_feat ? 1 : 0);
     stats.k_last = k;
     return KT_OK;
 }

@@ -71,10 +71,14 @@ struct Plan {
     uint32_t pick = KT_PICK_FIXED;
     DBuf model;          // cost-model parameters (KT_PICK_MODEL)
     uint32_t model_C = 0, model_n = 0;
-    DBuf feat_acc, feat, ctr_scratch;
+    DBuf feat_acc, ctr_scratch;
+    DBuf feat_log;            // phase features, KT_MLP_IN floats per phase of the run
+    uint32_t feat_cap = 0;    // phases feat_log holds
+    std::vector<uint8_t> feat_on;  // per logged phase: features were computed
+    bool phase_has_feat = false;
+    int flush_phase_features();
     DBuf stripes, stripes_x;  // count statistics of ctr / of ctr_scratch (never read)
     bool stripes_dirty = false;  // a count added to `stripes` since the last fold
-    float* h_feat = nullptr;
     hipEvent_t ev_sw[2 * KT_MAX_CANDIDATES] = {};
     std::vector<std::array<float, KT_PHASE_REC>> phase_log;
 
@@ -89,6 +93,7 @@ struct Plan {
     int acur = 0;
     uint32_t S_cur = 0;
     DBuf dl, dl_alt, sup_snap, dscratch, ctsum, cxs;  // dl_alt: second deletion list (run-ahead)
+    DBuf ldeg;  // live length of every oriented row (phase features of DELTA plans)
     dev::DeltaScratch dws{};
     bool delta_ready = false;
     bool last_pass_delta = false;  // the last support pass was a DELTA update
@@ -120,7 +125,9 @@ struct Plan {
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
     // defer: leave the model's choice on the device (read with the next
     // counter read-back) instead of synchronising for it now
-    int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer = false);
+    // live_rows: row features from the live row lengths (DELTA phases keep them)
+    int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer = false,
+                   bool live_rows = false);
     int compact_now();
     // Reads the device counters; fold: add the DELTA update's per-workgroup
     // statistics first (once a run's last update is done).
