@@ -211,6 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
         return;
     }
     if (blockIdx.x >= ntiles) return;  // whole workgroup
+    uint32_t tile_total = 0;  // the last tile's total (thread kBlock - 1)
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t i0 = t * kTileD + threadIdx.x * kPerD;
         uint32_t cnt[kPerD];
@@ -244,8 +245,19 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
             if (i0 + r < nd) lofs[i0 + r] = run;
             run += cnt[r];
         }
-        if (threadIdx.x == kBlock - 1) tsum[t] = run;  // the tile total
+        if (threadIdx.x == kBlock - 1) {
+            tsum[t] = run;  // the tile total
+            tile_total = run;
+        }
         __syncthreads();
+    }
+    if (ntiles == 1) {  // one workgroup did everything: no ticket
+        if (threadIdx.x == kBlock - 1) {
+            tsum[0] = 0;
+            tsum[1] = tile_total;
+            ctr->dtasks = tile_total;
+        }
+        return;
     }
     // the last workgroup to finish scans the tile totals
     __threadfence();
@@ -711,7 +723,8 @@ hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, 
                           hipStream_t s) {
     if (!n) return hipSuccess;
     const uint32_t b = blocks_for(n, kBlock);
-    hipLaunchKernelGGL(k_min_list, dim3(b < 2048 ? b : 2048), dim3(kBlock), 0, s, n, dst, sup, ctr);
+    // at most 256 workgroups: one atomic each on the one counter
+    hipLaunchKernelGGL(k_min_list, dim3(b < 256 ? b : 256), dim3(kBlock), 0, s, n, dst, sup, ctr);
     return hipGetLastError();
 }
 

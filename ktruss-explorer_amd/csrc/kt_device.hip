@@ -1069,9 +1069,9 @@ __global__ __launch_bounds__(kBlock) void k_build_worklist(const uint32_t* __res
     list_flush(s_out, &s_cnt, &s_base, wl, &ctr->wl_size);
 }
 
-// Minimum live support.  Grid-stride over at most 2048 workgroups, wave
+// Minimum live support.  Grid-stride over at most 256 workgroups, wave
 // butterflies, then one atomic per workgroup: the counter sees a few
-// thousand atomics instead of one per wave of the whole array.
+// hundred atomics instead of one per wave of the whole array.
 __global__ void k_min_live(const uint32_t* __restrict__ dst, const uint32_t* __restrict__ sup,
                            uint32_t E, DevCounters* ctr) {
     __shared__ uint32_t s_min[kBlock / kWave];
@@ -1322,7 +1322,7 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s) {
     if (!E) return hipSuccess;
-    const uint32_t blocks = grid_for(E) < 2048 ? grid_for(E) : 2048;
+    const uint32_t blocks = grid_for(E) < 256 ? grid_for(E) : 256;  // one atomic per workgroup
     hipLaunchKernelGGL(k_min_live, dim3(blocks), dim3(kBlock), 0, s, dst, sup, E, ctr);
     return hipGetLastError();
 }

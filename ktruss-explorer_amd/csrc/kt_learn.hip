@@ -93,16 +93,30 @@ __global__ __launch_bounds__(kFeatThreads) void k_feat_scan(const uint32_t* __re
     unsigned long long w[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries (sampled)
     const float denom = (float)(k > 2 ? k - 2 : 1);
     const uint32_t lane = threadIdx.x & 63;
+    // four sampled chunks per round, their loads in flight together (indices
+    // past the end read entry E - 1 and are not counted)
     const uint64_t cstep = (uint64_t)gridDim.x * (blockDim.x / 64) * stride * 64;
     for (uint64_t c = ((uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * stride * 64; c < E;
-         c += cstep) {
-        const uint64_t e = c + lane;
-        if (e >= E || (dst[e] & (KT_DEAD | KT_DYING))) continue;
-        w[5] += 1;
-        if (sup_valid) {
-            const float r = (float)sup[e] / denom;
-            const int b = r < 1.f ? 0 : r < 2.f ? 1 : r < 4.f ? 2 : r < 8.f ? 3 : 4;
-            w[b] += 1;
+         c += 4 * cstep) {
+        uint32_t dv[4], sv[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t e = c + q * cstep + lane;
+            in[q] = e < E;
+            const uint64_t ee = in[q] ? e : E - 1;
+            dv[q] = dst[ee];
+            sv[q] = sup[ee];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!in[q] || (dv[q] & (KT_DEAD | KT_DYING))) continue;
+            w[5] += 1;
+            if (sup_valid) {
+                const float r = (float)sv[q] / denom;
+                const int b = r < 1.f ? 0 : r < 2.f ? 1 : r < 4.f ? 2 : r < 8.f ? 3 : 4;
+                w[b] += 1;
+            }
         }
     }
     block_sum_store<6>(w, false, part + (size_t)blockIdx.x * KT_FEAT_ACC + 6);

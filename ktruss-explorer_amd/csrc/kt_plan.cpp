@@ -857,9 +857,12 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     // (which also finalizes the deletions just updated)
     auto queue = [&](int s) -> int {
         const uint32_t in = r & 1u, out = in ^ 1u;
-        KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], bound, src[cur].as<uint32_t>(),
-                                      dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws, c, stream,
-                                      counts[out], prev_slot >= 0 ? d_ra + prev_slot : nullptr));
+        // the task list's grid follows the last count read back (it strides over
+        // the true count); the update's follows the upper bound
+        KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], std::max<uint32_t>(last_nd, 1u),
+                                      src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
+                                      sptr_b[acur].as<uint32_t>(), dws, c, stream, counts[out],
+                                      prev_slot >= 0 ? d_ra + prev_slot : nullptr));
         KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
         KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream));
