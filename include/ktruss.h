@@ -204,6 +204,12 @@ int kt_plan_ktruss(kt_plan* p, uint32_t k, uint32_t* out_src, uint32_t* out_dst,
  * layout; truss numbers stay on the device.  stats->ms_total is the hipEvent
  * time from the resident oriented layout to the resident truss numbers. */
 int kt_plan_decompose(kt_plan* p, kt_stats* stats);
+/* Timing (on by default): the support passes are bracketed by hipEvents,
+ * which fill stats->ms_count / ms_delta and the time column of the trace.  Off:
+ * DELTA iterations queued ahead of the host record no events (each costs the
+ * device a few microseconds between dependent kernels); their ms figures and
+ * trace times stay 0.  Results are the same either way. */
+int kt_plan_set_timing(kt_plan* p, int on);
 /* Copy the last decomposition's truss numbers: one per input entry of the
  * graph the plan was made from (count = m_input). */
 int kt_plan_truss_numbers(kt_plan* p, uint32_t* tau, uint64_t count);

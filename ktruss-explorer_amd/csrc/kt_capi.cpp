@@ -198,6 +198,12 @@ int kt_plan_set_candidates(kt_plan* p, const kt_count_config* cands, uint32_t n)
     KT_GUARD(return p->impl.set_candidates(cands, n);)
 }
 
+int kt_plan_set_timing(kt_plan* p, int on) {
+    if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
+    p->impl.timing = on != 0;
+    return KT_OK;
+}
+
 int kt_plan_set_pick(kt_plan* p, uint32_t mode, const kt_mlp* model) {
     if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
     if (mode == KT_PICK_MODEL && !model) return kt::fail(KT_ERR_ARG, "model is NULL");

@@ -14,6 +14,7 @@
 #include <deque>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kt_host.h"
@@ -857,16 +858,17 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     // (which also finalizes the deletions just updated)
     auto queue = [&](int s) -> int {
         const uint32_t in = r & 1u, out = in ^ 1u;
+        __atomic_store_n(h_ra + s, kCountPending, __ATOMIC_RELEASE);  // the successor overwrites it
         // the task list's grid follows the last count read back (it strides over
         // the true count); the update's follows the upper bound
         KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], std::max<uint32_t>(last_nd, 1u),
                                       src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
                                       sptr_b[acur].as<uint32_t>(), dws, c, stream, counts[out],
                                       prev_slot >= 0 ? d_ra + prev_slot : nullptr));
-        KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
+        if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
         KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream));
-        KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
+        if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                    tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
@@ -880,7 +882,19 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     auto read_count = [&](size_t i, uint32_t& nd) -> int {
         const int s = open[i].first;
         if (i + 1 < open.size()) {
-            KT_HIP_TRY(hipEventSynchronize(ev_ra[open[i + 1].first][0]));
+            // the successor's task list stores the count into pinned memory:
+            // wait for it there (an event between the kernels would cost the
+            // device a gap); a stream that went idle without it is an error
+            for (uint32_t spin = 1;; ++spin) {
+                if (__atomic_load_n(h_ra + s, __ATOMIC_ACQUIRE) != kCountPending) break;
+                if (spin % 256 == 0) {
+                    const hipError_t q = hipStreamQuery(stream);
+                    if (q == hipSuccess && __atomic_load_n(h_ra + s, __ATOMIC_ACQUIRE) == kCountPending)
+                        return fail(KT_ERR_STATE, "deletion count of a queued DELTA iteration never arrived");
+                    if (q != hipSuccess && q != hipErrorNotReady) KT_HIP_TRY(q);
+                }
+                std::this_thread::yield();
+            }
         } else {
             KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[r & 1u], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
@@ -897,7 +911,10 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         if (rc) return rc;
         open.pop_front();
         float ms = 0;
-        KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[s][0], ev_ra[s][1]));
+        if (timing) {
+            KT_HIP_TRY(hipEventSynchronize(ev_ra[s][1]));
+            KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[s][0], ev_ra[s][1]));
+        }
         stats.ms_delta += ms;
         phase_ms += ms;
         stats.count_passes++;

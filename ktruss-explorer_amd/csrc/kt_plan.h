@@ -57,6 +57,7 @@ struct Plan {
     // back; per slot (update start, update end, count read-back) events and
     // the pinned deletion count
     static constexpr uint32_t kAhead = 1, kAheadSlots = kAhead + 2;
+    static constexpr uint32_t kCountPending = 0xFFFFFFFFu;  // h_ra slot not written yet
     hipEvent_t ev_ra[kAheadSlots][3] = {};
     uint32_t* h_ra = nullptr;
     uint32_t* d_ra = nullptr;  // h_ra's device address
@@ -76,6 +77,7 @@ struct Plan {
     uint32_t feat_cap = 0;    // phases feat_log holds
     std::vector<uint8_t> feat_on;  // per logged phase: features were computed
     bool phase_has_feat = false;
+    bool timing = true;  // hipEvents around the support passes of run-ahead iterations
     int flush_phase_features();
     DBuf stripes, stripes_x;  // count statistics of ctr / of ctr_scratch (never read)
     bool stripes_dirty = false;  // a count added to `stripes` since the last fold

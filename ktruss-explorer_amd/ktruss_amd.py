@@ -41,6 +41,7 @@ EXPORTS = [
     "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
     "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
+    "kt_plan_set_timing",
     "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
     "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
     "kt_mlp_set_input_range", "kt_mlp_get_input_range",
@@ -181,6 +182,7 @@ def lib():
         L.kt_plan_trace.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, _u64p]
         L.kt_plan_set_candidates.argtypes = [ctypes.c_void_p, ctypes.POINTER(CountConfig), ctypes.c_uint32]
         L.kt_plan_set_pick.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.kt_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.kt_plan_phase_log.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64, _u64p]
         L.kt_mlp_create.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
         L.kt_mlp_destroy.argtypes = [ctypes.c_void_p]
@@ -375,6 +377,11 @@ class Plan:
         arr = (CountConfig * max(len(configs), 1))(*configs)
         _check(lib().kt_plan_set_candidates(self._h, arr, len(configs)), "kt_plan_set_candidates")
         self.candidates = list(configs)
+
+    def set_timing(self, on=True):
+        """hipEvents around every support pass (ms statistics and trace times); off
+        saves the device the gaps the events cost, results unchanged."""
+        _check(lib().kt_plan_set_timing(self._h, 1 if on else 0), "kt_plan_set_timing")
 
     def set_pick(self, mode, model=None):
         _check(lib().kt_plan_set_pick(self._h, mode, model._h if model is not None else None),
