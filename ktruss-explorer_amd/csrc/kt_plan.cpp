@@ -872,7 +872,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                    tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
-                                   bound, src[cur].as<uint32_t>(), ldeg.as<uint32_t>(), stream));
+                                   bound, src[cur].as<uint32_t>(), live_rows_ptr(), stream));
         prev_slot = s;
         ++r;
         return KT_OK;
@@ -1019,7 +1019,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
                                    snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                    &ctr.as<DevCounters>()->ndel, nullptr, nullptr, 0, src[cur].as<uint32_t>(),
-                                   ldeg.as<uint32_t>(), stream));
+                                   live_rows_ptr(), stream));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;

@@ -96,6 +96,8 @@ struct Plan {
     uint32_t S_cur = 0;
     DBuf dl, dl_alt, sup_snap, dscratch, ctsum, cxs;  // dl_alt: second deletion list (run-ahead)
     DBuf ldeg;  // live length of every oriented row (phase features of DELTA plans)
+    // the rows' live lengths are kept only when phases are picked (features)
+    uint32_t* live_rows_ptr() const { return pick == KT_PICK_FIXED ? nullptr : ldeg.as<uint32_t>(); }
     dev::DeltaScratch dws{};
     bool delta_ready = false;
     bool last_pass_delta = false;  // the last support pass was a DELTA update

@@ -104,3 +104,23 @@ def test_delta_rejected_where_unsupported(small_graphs):
         with pytest.raises(K.KtError) as ei:
             K.Plan(g, v)
         assert ei.value.status == K.ERR_ARG
+
+
+def test_delta_timing_off_same_results():
+    """Without per-pass events (the host polls the deletion counts of the
+    iterations it runs ahead) the peel is the same."""
+    n, s, d = K.generate_rmat(13, 16, seed=7)
+    g = K.Graph(n, s, d, pairs=True)
+    p = K.Plan(g, delta_variant())
+    st_on = p.decompose()
+    tau_on, tr_on = p.truss_numbers(), [t[:3] for t in p.trace()]
+    p.set_timing(False)
+    for _ in range(2):
+        st_off = p.decompose()
+        assert np.array_equal(p.truss_numbers(), tau_on)
+        assert [t[:3] for t in p.trace()] == tr_on
+        assert st_off["delta_triangles"] == st_on["delta_triangles"]
+        assert st_off["delta_passes"] == st_on["delta_passes"]
+        assert st_off["ms_delta"] <= st_on["ms_delta"]
+    p.set_timing(True)
+    assert p.decompose()["ms_delta"] > 0
