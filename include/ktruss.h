@@ -295,7 +295,7 @@ typedef struct kt_batch_tuning {
     uint32_t cap_lds;          /* LDS slice per wave; 0 = automatic                */
     uint32_t blocks_per_cu;    /* persistent count workgroups per CU; 0 = 2        */
     uint32_t mark_chunk;       /* entries per mark task (x64); 0 = 1024            */
-    uint32_t steps_per_replay; /* peel steps per captured graph replay; 0 = 16     */
+    uint32_t steps_per_replay; /* peel steps per captured graph replay; 0 = 16; rounded up to even */
 } kt_batch_tuning;
 
 typedef struct kt_batch_stats {

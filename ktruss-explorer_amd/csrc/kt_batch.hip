@@ -76,8 +76,20 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
                                                 uint32_t* gndel, uint32_t* gmin, uint32_t* __restrict__ dl,
                                                 unsigned int* dn, uint32_t* __restrict__ snbr,
                                                 const uint32_t* __restrict__ sa,
-                                                const uint32_t* __restrict__ sb) {
+                                                const uint32_t* __restrict__ sb,
+                                                const uint32_t* __restrict__ fin_dl,
+                                                const unsigned int* __restrict__ fin_nd) {
     __shared__ uint32_t s_list[DELTA ? (256 / kWave) * kBMarkCap : 1];
+    if (DELTA && fin_dl) {  // the previous step's deletions: KT_DEAD, entry and both slots
+        const uint32_t nf = *fin_nd;
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
+            const uint32_t d = fin_dl[t];
+            dst[d] = (dst[d] & ~KT_DYING) | KT_DEAD;
+            const uint32_t a = sa[d], b = sb[d];
+            snbr[a] = (snbr[a] & ~KT_DYING) | KT_DEAD;
+            snbr[b] = (snbr[b] & ~KT_DYING) | KT_DEAD;
+        }
+    }
     __shared__ uint32_t s_n[256 / kWave], s_base;
     const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -331,7 +343,7 @@ __global__ __launch_bounds__(1024) void k_b_control(BatchDev b, uint32_t mode) {
     if ((threadIdx.x & (kWave - 1)) == 0 && live_counted)
         atomicAdd(&b.bc->live_in_passes, live_counted);
     if (threadIdx.x == 0) {
-        b.bc->count.ndel = 0;  // the DELTA list of the next mark pass starts empty
+        if (b.zero_cnt) *b.zero_cnt = 0;  // the DELTA list of the next mark pass starts empty
         b.bc->ndirty = mode == 1 ? 0u : tot_nd;
         b.bc->nactive = tot_na;
         b.bc->dtasks = mode == 1 ? 0u : tot_td;
@@ -386,13 +398,14 @@ hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, ui
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
                       unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
-                      uint32_t blocks, hipStream_t s) {
+                      const uint32_t* fin_dl, const unsigned int* fin_nd, uint32_t blocks,
+                      hipStream_t s) {
     if (dl)
         hipLaunchKernelGGL(k_b_mark<true>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
-                           gk, gndel, gmin, dl, dn, snbr, sa, sb);
+                           gk, gndel, gmin, dl, dn, snbr, sa, sb, fin_dl, fin_nd);
     else
         hipLaunchKernelGGL(k_b_mark<false>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
-                           gk, gndel, gmin, dl, dn, snbr, sa, sb);
+                           gk, gndel, gmin, dl, dn, snbr, sa, sb, nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -40,7 +40,8 @@ bool same_tuning(const kt_batch_tuning& a, const kt_batch_tuning& b) {
 Batch::~Batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (h_bc) (void)hipHostFree(h_bc);
-    for (auto* ev : {&ev_start, &ev_stop, &ev_t0, &ev_t1})
+    if (h_rb) (void)hipHostFree(h_rb);
+    for (auto* ev : {&ev_start, &ev_stop, &ev_t0, &ev_t1, &ev_rb[0], &ev_rb[1]})
         if (*ev) (void)hipEventDestroy(*ev);
     for (hipEvent_t ev : tev) (void)hipEventDestroy(ev);
     if (stream) (void)hipStreamDestroy(stream);
@@ -159,8 +160,10 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
     num_cu = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256;
 
     KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto* ev : {&ev_start, &ev_stop, &ev_t0, &ev_t1}) KT_HIP_TRY(hipEventCreate(ev));
+    for (auto* ev : {&ev_start, &ev_stop, &ev_t0, &ev_t1, &ev_rb[0], &ev_rb[1]})
+        KT_HIP_TRY(hipEventCreate(ev));
     KT_HIP_TRY(hipHostMalloc((void**)&h_bc, sizeof(BatchCounters), hipHostMallocDefault));
+    KT_HIP_TRY(hipHostMalloc((void**)&h_rb, 2 * sizeof(BatchCounters), hipHostMallocDefault));
     std::memset(h_bc, 0, sizeof(BatchCounters));
 
     const size_t Eb = (size_t)std::max<uint32_t>(E, 1) * 4;
@@ -184,7 +187,7 @@ int Batch::create(const kt_graph* const* gs, uint32_t count, const kt_variant* v
     if (delta) {
         if ((rc = sptr.alloc(((size_t)N + 1) * 4)) || (rc = snbr0.alloc(2 * Eb)) ||
             (rc = snbr.alloc(2 * Eb)) || (rc = seid.alloc(2 * Eb)) || (rc = sa.alloc(Eb)) ||
-            (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)) ||
+            (rc = sb.alloc(Eb)) || (rc = dl.alloc(Eb)) || (rc = dl2.alloc(Eb)) ||
             (rc = dscratch.alloc(dev::delta_scratch_bytes(E))))
             return rc;
         dev::delta_scratch_carve(dscratch.p, E, dws);
@@ -236,6 +239,9 @@ int Batch::resolve(const kt_batch_tuning& in, kt_batch_tuning& out, uint32_t& ca
     out.mark_chunk = in.mark_chunk ? in.mark_chunk : 1024;
     out.blocks_per_cu = in.blocks_per_cu ? in.blocks_per_cu : 2;
     out.steps_per_replay = in.steps_per_replay ? in.steps_per_replay : 16;
+    // DELTA steps alternate between two deletion lists: a captured replay
+    // holds an even number of steps so every replay starts on the first
+    out.steps_per_replay += out.steps_per_replay & 1u;
     if (out.block_size % 64 || out.block_size > 1024)
         return fail(KT_ERR_ARG, "block_size must be a multiple of 64 up to 1024");
     if (out.chunk % 64 || out.mark_chunk % 64) return fail(KT_ERR_ARG, "chunks must be multiples of 64");
@@ -276,6 +282,7 @@ int Batch::set_tuning(const kt_batch_tuning& t) { return apply(t); }
 uint32_t Batch::count_blocks() const { return num_cu * tune.blocks_per_cu; }
 
 int Batch::reset() {
+    parity = 0;  // the first step marks into the first deletion list
     if (E) KT_HIP_TRY(hipMemcpyAsync(dst.p, dst0.p, (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
     if (delta && E)
         KT_HIP_TRY(hipMemcpyAsync(snbr.p, snbr0.p, 2 * (size_t)E * 4, hipMemcpyDeviceToDevice, stream));
@@ -339,26 +346,31 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
     a.ctr = &bd.bc->count;
     a.stripes = stripes.as<unsigned long long>();
     if (delta) {
-        // mark -> DELTA update of this step's deletions -> finalize -> control -> expand
-        unsigned int* dn = &bd.bc->count.ndel;
+        // mark (which finalizes the previous step's deletions) -> DELTA update
+        // of this step's deletions -> control (clears the next mark's count) -> expand
+        const uint32_t p = parity;
+        parity ^= 1u;
+        uint32_t* lists[2] = {dl.as<uint32_t>(), dl2.as<uint32_t>()};
+        unsigned int* counts[2] = {&bd.bc->count.ndel, &bd.bc->count.ndel2};
         KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
-                                   bd.gndel, bd.gmin, dl.as<uint32_t>(), dn, snbr.as<uint32_t>(),
-                                   sa.as<uint32_t>(), sb.as<uint32_t>(), stream_blocks(), stream));
-        KT_HIP_TRY(dev::delta_prepare(dl.as<uint32_t>(), dn, E, src.as<uint32_t>(), dst.as<uint32_t>(),
+                                   bd.gndel, bd.gmin, lists[p], counts[p], snbr.as<uint32_t>(),
+                                   sa.as<uint32_t>(), sb.as<uint32_t>(), lists[p ^ 1u], counts[p ^ 1u],
+                                   stream_blocks(), stream));
+        KT_HIP_TRY(dev::delta_prepare(lists[p], counts[p], E, src.as<uint32_t>(), dst.as<uint32_t>(),
                                       sptr.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
-        KT_HIP_TRY(dev::delta_apply(dl.as<uint32_t>(), dn, E, snbr.as<uint32_t>(), seid.as<uint32_t>(),
+        KT_HIP_TRY(dev::delta_apply(lists[p], counts[p], E, snbr.as<uint32_t>(), seid.as<uint32_t>(),
                                     sup.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c1) KT_HIP_TRY(hipEventRecord(c1, stream));
-        KT_HIP_TRY(dev::finalize_deleted(dl.as<uint32_t>(), dn, E, dst.as<uint32_t>(),
-                                         snbr.as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
-        KT_HIP_TRY(dev::batch_control(bd, 2, stream));
+        BatchDev b2 = bd;
+        b2.zero_cnt = counts[p ^ 1u];
+        KT_HIP_TRY(dev::batch_control(b2, 2, stream));
         KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
         return KT_OK;
     }
     KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
-                               bd.gndel, bd.gmin, nullptr, nullptr, nullptr, nullptr, nullptr,
-                               stream_blocks(), stream));
+                               bd.gndel, bd.gmin, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_control(bd, 2, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_zero(dtl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
@@ -376,7 +388,9 @@ int Batch::capture(uint32_t steps) {
     }
     KT_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int rc = KT_OK;
+    parity = 0;  // every replay starts on the first deletion list (steps is even)
     for (uint32_t s = 0; s < steps && !rc; ++s) rc = launch_step();
+    parity = 0;
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(stream, &graph);
     if (rc) {
@@ -409,21 +423,41 @@ int Batch::decompose() {
     if (timed && (rc = event_pair(nev++, c0, c1))) return rc;
     if ((rc = prestep(c0, c1))) return rc;
     uint64_t replays = 0;
-    while (true) {
-        if (timed) {
+    if (timed) {
+        while (true) {
             for (uint32_t s = 0; s < tune.steps_per_replay; ++s) {
                 if ((rc = event_pair(nev++, c0, c1))) return rc;
                 if ((rc = launch_step(c0, c1))) return rc;
             }
-        } else {
-            KT_HIP_TRY(hipGraphLaunch(exec, stream));
+            ++replays;
+            KT_HIP_TRY(hipMemcpyAsync(h_bc, bc.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, stream));
+            KT_HIP_TRY(hipStreamSynchronize(stream));
+            if (h_bc->nactive == 0) break;
+            if (h_bc->steps > step_bound)
+                return fail(KT_ERR_STATE, "batched peel exceeded " + std::to_string(step_bound) + " steps");
         }
-        ++replays;
-        KT_HIP_TRY(hipMemcpyAsync(h_bc, bc.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, stream));
-        KT_HIP_TRY(hipStreamSynchronize(stream));
-        if (h_bc->nactive == 0) break;
-        if (h_bc->steps > step_bound)
-            return fail(KT_ERR_STATE, "batched peel exceeded " + std::to_string(step_bound) + " steps");
+    } else {
+        // replays run one ahead of their counter read-backs, so the device
+        // does not wait for the host between replays; the replay queued after
+        // the last graph finished finds empty task lists and changes nothing
+        int open = -1;  // read-back slot of the previous replay
+        while (true) {
+            KT_HIP_TRY(hipGraphLaunch(exec, stream));
+            const int slot = (int)(replays & 1u);
+            ++replays;
+            KT_HIP_TRY(hipMemcpyAsync(h_rb + slot, bc.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, stream));
+            KT_HIP_TRY(hipEventRecord(ev_rb[slot], stream));
+            if (open >= 0) {
+                KT_HIP_TRY(hipEventSynchronize(ev_rb[open]));
+                if (h_rb[open].nactive == 0) {
+                    --replays;  // the one just queued is void
+                    break;
+                }
+                if (h_rb[open].steps > step_bound)
+                    return fail(KT_ERR_STATE, "batched peel exceeded " + std::to_string(step_bound) + " steps");
+            }
+            open = slot;
+        }
     }
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
     KT_HIP_TRY(dev::stripes_fold(stripes.as<unsigned long long>(), &bd.bc->count, stream));
@@ -433,8 +467,7 @@ int Batch::decompose() {
     float ms = 0;
     KT_HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_stop));
 
-    h_tau.resize(E);
-    if (E) KT_HIP_TRY(hipMemcpy(h_tau.data(), tau.p, (size_t)E * 4, hipMemcpyDeviceToHost));
+    tau_on_host = false;  // the truss numbers stay on the device until read
     std::vector<uint32_t> st((size_t)kStateArrays * G);
     KT_HIP_TRY(hipMemcpy(st.data(), gstate.p, st.size() * 4, hipMemcpyDeviceToHost));
     h_ginfo.assign((size_t)4 * G, 0);
@@ -565,6 +598,11 @@ int Batch::truss_numbers(uint32_t g, uint32_t* out, uint64_t count) {
     if (g >= G) return fail(KT_ERR_ARG, "graph index out of range");
     if (count != graphs[g]->m_input)
         return fail(KT_ERR_CAPACITY, "need " + std::to_string(graphs[g]->m_input) + " entries");
+    if (!tau_on_host) {  // the first read after a run downloads them all once
+        h_tau.resize(E);
+        if (E) KT_HIP_TRY(hipMemcpy(h_tau.data(), tau.p, (size_t)E * 4, hipMemcpyDeviceToHost));
+        tau_on_host = true;
+    }
     const uint32_t* uid = graphs[g]->uid.data();
     const uint32_t* pos = pos_of_edge[g].data();
     const uint32_t* t = h_tau.data() + gbeg[g];

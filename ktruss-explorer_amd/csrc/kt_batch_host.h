@@ -22,9 +22,12 @@ struct Batch {
     DBuf d_gbeg, gstate, lists, ddesc, adesc, bc;
     DBuf stripes;  // count statistics per stripe line (folded into bc->count at the end)
     bool delta = false;                               // DELTA support maintenance
-    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, dscratch;
+    DBuf sptr, snbr0, snbr, seid, sa, sb, dl, dl2, dscratch;
+    uint32_t parity = 0;  // DELTA steps alternate between (dl, ndel) and (dl2, ndel2)
     dev::DeltaScratch dws{};
     BatchCounters* h_bc = nullptr;
+    BatchCounters* h_rb = nullptr;        // two read-back slots: replays run one ahead
+    hipEvent_t ev_rb[2] = {nullptr, nullptr};
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -38,7 +41,8 @@ struct Batch {
     BatchDev bd{};
     kt_batch_stats stats{};
     bool have_tau = false;
-    std::vector<uint32_t> h_tau, h_ginfo;             // downloaded after a run
+    std::vector<uint32_t> h_tau, h_ginfo;             // downloaded after a run (h_tau on first read)
+    bool tau_on_host = false;
     std::vector<kt_batch_tuning> tune_log_cfg;
     std::vector<float> tune_log_ms;
 
