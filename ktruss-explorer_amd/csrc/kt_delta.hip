@@ -436,12 +436,20 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
     const uint32_t t0 = (uint32_t)((uint64_t)T * w / W);
     const uint32_t t1 = nd ? (uint32_t)((uint64_t)T * (w + 1) / W) : t0;  // no early return:
                                                                           // the block reduces below
-    // last deletion whose first task is <= t0
-    uint32_t i = 0, len = nd;
-    while (len > 1) {
-        const uint32_t half = len >> 1;
-        if (first_task(lofs, tsum, i + half) <= t0) i += half;
-        len -= half;
+    // last deletion whose first task is <= t0: a 64-ary search, the lanes
+    // probing 64 evenly spaced deletions per step (three dependent loads for
+    // 2^18 deletions instead of eighteen); first_task(0) = 0 <= t0
+    uint32_t i = 0;
+    if (t0 < t1) {
+        uint32_t len = nd;
+        while (len > 1) {
+            const uint32_t step = (len + kWave - 1) / kWave, end = i + len;
+            const uint32_t idx = i + lane * step;
+            const bool ok = idx < end && first_task(lofs, tsum, idx) <= t0;
+            const uint64_t m = ballot(ok);  // lane 0 always holds
+            i += (63u - (uint32_t)__builtin_clzll(m)) * step;
+            len = min(step, end - i);
+        }
     }
     uint32_t full = 0xFFFFFFFFu;     // deletion whose whole longer list is in buf
     uint32_t sampled = 0xFFFFFFFFu;  // deletion whose block samples are in smp
