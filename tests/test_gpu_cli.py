@@ -1,0 +1,67 @@
+"""The ./ktruss command line end to end on the GPU: the reference's flags,
+its TSV input (graph.cu:37-79), the k-truss it writes (-o) and the
+verification against a compare file (-c, verify.cu:12-40; the default
+compare file is the reference's, so runs without one pass -c none), checked
+against the CPU oracle."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import ktruss_amd as K
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_cli(*args):
+    return subprocess.run([K.CLI_PATH, *args], capture_output=True, text=True, timeout=120)
+
+
+def undirected(src, dst):
+    return {(min(a, b), max(a, b)) for a, b in zip(np.asarray(src).tolist(), np.asarray(dst).tolist())}
+
+
+@pytest.mark.parametrize("d", ["directed-by-degree", "undirected"])
+@pytest.mark.parametrize("k", [3, 4, 6])
+def test_cli_ktruss_matches_oracle(small_graphs, tmp_path, d, k):
+    for name, (n, s, dd) in small_graphs.items():
+        graph = tmp_path / f"{name}.tsv"
+        K.write_tsv(str(graph), s, dd)
+        out = tmp_path / f"{name}_k{k}.tsv"
+        r = run_cli("-g", str(graph), "-k", str(k), "-p", "gpu", "-d", d, "-o", str(out), "-c", "none",
+                    "-v", "0")
+        assert r.returncode == 0, (name, r.stderr)
+        _, gs, gd = K.read_tsv(str(out)) if out.stat().st_size else (0, [], [])
+        os_, od_, _, _ = O.ktruss(n, s, dd, k, O.make_variant(d))
+        assert undirected(gs, gd) == undirected(os_, od_), (name, k, d)
+
+
+def test_cli_compare_file_and_decompose(small_graphs, tmp_path):
+    n, s, d = small_graphs["plc300"]
+    graph = tmp_path / "g.tsv"
+    K.write_tsv(str(graph), s, d)
+    tau_o, _, _ = O.decompose(n, s, d)
+    k = int(np.median(tau_o)) + 1
+    # the oracle's k-truss as the compare file: verification passes
+    keep = tau_o >= k
+    ref = tmp_path / "ref.tsv"
+    K.write_tsv(str(ref), s[keep], d[keep])
+    r = run_cli("-g", str(graph), "-k", str(k), "-p", "gpu", "-c", str(ref), "-v", "0")
+    assert r.returncode == 0, r.stdout + r.stderr
+    # a compare file missing an edge (both directions) fails the verification
+    ks, kd = s[keep], d[keep]
+    a, b = ks[0], kd[0]
+    drop = ((ks == a) & (kd == b)) | ((ks == b) & (kd == a))
+    bad = tmp_path / "bad.tsv"
+    K.write_tsv(str(bad), ks[~drop], kd[~drop])
+    r = run_cli("-g", str(graph), "-k", str(k), "-p", "gpu", "-c", str(bad), "-v", "0")
+    assert r.returncode != 0
+    # -D: one line per input entry, "dst src tau" 1-based
+    out = tmp_path / "tau.tsv"
+    r = run_cli("-g", str(graph), "-D", "-p", "gpu", "-o", str(out), "-c", "none", "-v", "0")
+    assert r.returncode == 0, r.stderr
+    rows = np.loadtxt(str(out), dtype=np.int64, ndmin=2)
+    got = {(int(a) - 1, int(b) - 1): int(t) for a, b, t in rows}
+    exp = {(int(b), int(a)): int(t) for a, b, t in zip(s, d, tau_o)}
+    assert got == exp
