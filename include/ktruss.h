@@ -67,7 +67,7 @@ enum { KT_SCHED_JUMP = 1u, KT_SCHED_REUSE = 2u };
  *         then those a full recount would give, so the next mark deletes the
  *         same edges.  A pass with nothing to update (the first count) uses
  *         ROW.  Plans that use DELTA compact their entries (and the
- *         adjacency lists with them) once half of the stored entries are dead. */
+ *         adjacency lists with them) once a third of the stored entries are dead. */
 enum { KT_COUNT_AUTO = 0u, KT_COUNT_ROW = 1u, KT_COUNT_SEARCH = 2u, KT_COUNT_MERGE = 3u,
        KT_COUNT_DELTA = 4u };
 

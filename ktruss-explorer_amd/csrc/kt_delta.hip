@@ -158,9 +158,21 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
                            const uint32_t* __restrict__ sup, DevCounters* ctr) {
     __shared__ uint32_t s_min[kBlock / kWave];
     uint32_t m = 0xFFFFFFFFu;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const uint32_t x = sup[e];
-        if (x < m && !(dst[e] & (KT_DEAD | KT_DYING))) m = x;  // flags read for a new minimum only
+    // eight supports per thread in flight (indices past the end read entry
+    // n - 1, which is then not counted); flags read for a new minimum only
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x; e0 < n; e0 += 8 * stride) {
+        uint32_t x[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t e = e0 + j * stride;
+            x[j] = sup[e < n ? e : n - 1];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t e = e0 + j * stride;
+            if (e < n && x[j] < m && !(dst[e] & (KT_DEAD | KT_DYING))) m = x[j];
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t o = __shfl_xor(m, off);

@@ -693,12 +693,12 @@ int Plan::restore_delta() {
 }
 
 // With no deletion pending (every entry is live or dead), drop the dead
-// entries once they are half of the stored ones: the oriented layout is
+// entries once they are a third of the stored ones: the oriented layout is
 // compacted (stable, rows stay sorted) and the adjacency lists with it, their
 // slot -> entry map following the entries to their new positions.  Live
 // entries and slots (two per live edge) are counted on the host, so the new
 // sizes need no read-back beyond the compaction's own check.
-bool Plan::compaction_due() const { return live > 0 && (uint64_t)E > 2 * live && E > 1024; }
+bool Plan::compaction_due() const { return live > 0 && 2 * (uint64_t)E > 3 * live && E > 1024; }
 
 int Plan::compact_delta_state() {
     if (pending || !compaction_due()) return KT_OK;
@@ -962,7 +962,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
 // One phase at k with DELTA available: the mark pass defers its deletions
 // (KT_DYING + list), and the support pass that opens the next iteration
 // either removes their triangles (DELTA) or recounts.  The dead entries are
-// compacted away once they are half of the stored ones.
+// compacted away once they are a third of the stored ones.
 int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     const bool jump = (schedule & KT_SCHED_JUMP) != 0;
     uint32_t iter = 0;

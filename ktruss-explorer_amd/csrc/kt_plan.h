@@ -121,7 +121,7 @@ struct Plan {
     int run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule);
     int ensure_delta();
     int compact_delta_state();
-    bool compaction_due() const;  // dead entries are more than half of the stored ones
+    bool compaction_due() const;  // dead entries are more than a third of the stored ones
     int restore_delta();
     int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
     int run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended);
