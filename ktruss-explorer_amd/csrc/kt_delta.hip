@@ -186,6 +186,11 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
     }
 }
 
+__global__ void k_take_count(unsigned int* from, unsigned int* acc) {
+    *acc |= *from;
+    *from = 0;
+}
+
 // Task list of the pending deletions, one kernel.  Per deletion i (d = dl[i]
 // = (u, v)): the shorter and the longer of the two adjacency lists and the
 // number of 64-candidate tasks over the shorter one (at least 1: the list
@@ -736,6 +741,11 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
     const uint32_t nt = (n + kMarkTile - 1) / kMarkTile;
     hipLaunchKernelGGL(k_mark_dying, dim3(nt < kMarkGrid ? nt : kMarkGrid), dim3(kBlock), 0, s, dst,
                        eid, sup, tau, dl, n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg);
+    return hipGetLastError();
+}
+
+hipError_t take_count(unsigned int* from, unsigned int* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_take_count, dim3(1), dim3(1), 0, s, from, acc);
     return hipGetLastError();
 }
 

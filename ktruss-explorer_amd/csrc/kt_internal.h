@@ -67,6 +67,8 @@ struct DevCounters {
     unsigned int dtasks;              // DELTA tasks of the pending deletions
     unsigned int dticket;             // DELTA task list: workgroups done (reset by the last)
     unsigned int ndel2;               // second deletion count (DELTA iterations run ahead)
+    unsigned int void_del;            // deletions of void DELTA iterations (stays 0)
+    unsigned int pad0;
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
     unsigned long long delta_win;     // longer-list window entries read by DELTA updates
@@ -149,6 +151,9 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
 // out[v] = ptr[v + 1] - ptr[v] for v < nv.
 hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying).
+// acc |= *from; *from = 0 (one thread): takes a deletion count that is
+// checked later, at the next counter read-back.
+hipError_t take_count(unsigned int* from, unsigned int* acc, hipStream_t s);
 hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
                           hipStream_t s);
 // Scratch of a DELTA update for up to `cap` pending deletions.

@@ -195,6 +195,7 @@ int Plan::sync_counters(bool fold) {
     if (fold && delta_ready) KT_HIP_TRY(dev::delta_stats_fold(dws, ctr.as<DevCounters>(), stream));
     KT_HIP_TRY(hipMemcpyAsync(h_ctr, ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
+    if (h_ctr->void_del) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
     return KT_OK;
 }
 
@@ -223,6 +224,7 @@ int Plan::reset() {
         if (rc) return rc;
     }
     pending = 0;
+    void_src = nullptr;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     std::memset(h_ctr, 0, sizeof(DevCounters));
     trace.clear();
@@ -535,6 +537,7 @@ int Plan::decompose() {
                                          sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
         pending = 0;
     }
+    if ((rc = take_void())) return rc;
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
@@ -587,6 +590,7 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
         rc = use_delta ? run_phase_delta(kk, false, 0) : run_phase(kk, false, 0, exact);
         if (rc) return rc;
     }
+    if ((rc = take_void())) return rc;
     KT_HIP_TRY(hipEventRecord(ev_stop, stream));
     KT_HIP_TRY(hipEventSynchronize(ev_stop));
     float ms = 0;
@@ -828,6 +832,13 @@ int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t re
     return KT_OK;
 }
 
+int Plan::take_void() {
+    if (!void_src) return KT_OK;
+    KT_HIP_TRY(dev::take_count(void_src, &ctr.as<DevCounters>()->void_del, stream));
+    void_src = nullptr;
+    return KT_OK;
+}
+
 // The remaining iterations of a DELTA phase at k (k > 3), queued one ahead of
 // their read-backs: iteration j's update (of iteration j - 1's deletions,
 // their number read on the device, grids sized by an upper bound) and mark
@@ -835,24 +846,38 @@ int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t re
 // pinned memory, so the device does not wait for the host between
 // iterations.  The iteration queued after the one that deletes nothing only
 // finishes the last update (no support changes, so its mark deletes nothing);
-// it is not an iteration of the peel and is not recorded.  Returns with
-// `ended` when the phase is over (nothing pending), else with the last
-// iteration's deletions pending and `iter` the next index, for a compaction.
-// On entry `pending` holds the deletions of iteration iter - 1 (> 0).
-int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended) {
+// it is not an iteration of the peel and is not recorded, and its count is
+// checked at the next read-back (take_void) instead of waiting for it now.
+// Returns with `ended` when the phase is over (nothing pending), else with
+// the last iteration's deletions pending and `iter` the next index, for a
+// compaction.  On entry `pending` holds the deletions of iteration iter - 1
+// (> 0), or, with `head`, iteration iter's mark (no update before it) is
+// queued into dl / ctr->ndel and its count is unread.
+int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended, bool head) {
     DevCounters* c = ctr.as<DevCounters>();
     // deletion lists and counts alternate: iteration r (counted from entry)
     // updates from list r & 1 and marks into list (r + 1) & 1; the pending
     // deletions on entry are in dl / ctr->ndel (list 0)
     uint32_t* lists[2] = {dl.as<uint32_t>(), dl_alt.as<uint32_t>()};
     unsigned int* counts[2] = {&c->ndel, &c->ndel2};
-    uint32_t bound = pending;    // deletions the next update consumes, at most
-    uint32_t last_nd = pending;  // deletions of the last recorded iteration
-    // queued iterations whose counts are unread, oldest first: (slot, iteration)
-    std::deque<std::pair<int, uint32_t>> open;
+    // deletions the next update consumes, at most
+    uint32_t bound = head ? (uint32_t)std::min<uint64_t>(live, E) : pending;
+    uint32_t last_nd = bound;  // deletions of the last recorded iteration
+    // queued iterations whose counts are unread, oldest first
+    struct Queued {
+        int slot;
+        uint32_t it;
+        bool updated;  // a DELTA update precedes its mark
+    };
+    std::deque<Queued> open;
     ended = false;
     uint32_t r = 0;  // iterations queued since entry
     int prev_slot = -1;
+    if (head) {  // the first task list writes the head mark's count
+        prev_slot = (int)(iter % kAheadSlots);
+        __atomic_store_n(h_ra + prev_slot, kCountPending, __ATOMIC_RELEASE);
+        open.push_back({prev_slot, iter, false});
+    }
     // iteration r: task list (which also clears the mark's count and writes
     // the previous iteration's deletion count to pinned memory), update, mark
     // (which also finalizes the deletions just updated)
@@ -880,7 +905,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     // deletion count of the i-th queued iteration: written by its successor's
     // task list, or, for the youngest, copied back now
     auto read_count = [&](size_t i, uint32_t& nd) -> int {
-        const int s = open[i].first;
+        const int s = open[i].slot;
         if (i + 1 < open.size()) {
             // the successor's task list stores the count into pinned memory:
             // wait for it there (an event between the kernels would cost the
@@ -905,45 +930,48 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     };
     // reads the oldest queued iteration back and records it
     auto finish = [&](uint32_t& nd) -> int {
-        const int s = open.front().first;
-        const uint32_t it = open.front().second;
+        const Queued q = open.front();
         int rc = read_count(0, nd);
         if (rc) return rc;
         open.pop_front();
         float ms = 0;
-        if (timing) {
-            KT_HIP_TRY(hipEventSynchronize(ev_ra[s][1]));
-            KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[s][0], ev_ra[s][1]));
+        if (q.updated) {
+            if (timing) {
+                KT_HIP_TRY(hipEventSynchronize(ev_ra[q.slot][1]));
+                KT_HIP_TRY(hipEventElapsedTime(&ms, ev_ra[q.slot][0], ev_ra[q.slot][1]));
+            }
+            stats.ms_delta += ms;
+            phase_ms += ms;
+            stats.count_passes++;
+            stats.delta_passes++;
+            stats.delta_edges += last_nd;  // its update consumed the previous iteration's deletions
+            last_pass_delta = true;
         }
-        stats.ms_delta += ms;
-        phase_ms += ms;
-        stats.count_passes++;
-        stats.delta_passes++;
-        stats.delta_edges += last_nd;  // its update consumed the previous iteration's deletions
-        last_pass_delta = true;
         last_nd = nd;
-        iter = it + 1;
-        return record_iteration(k, it, nd, 0, ms);
+        iter = q.it + 1;
+        return record_iteration(k, q.it, nd, 0, ms);
     };
-    // the phase ended at the iteration just read: the younger queued ones are void
+    // the phase ended at the iteration just read: the younger queued ones are
+    // void; the youngest's count is left for the next read-back to check
     auto drain_void = [&]() -> int {
-        for (size_t i = 0; i < open.size(); ++i) {
+        for (size_t i = 0; i + 1 < open.size(); ++i) {
             uint32_t nd = 0;
             const int rc = read_count(i, nd);
             if (rc) return rc;
             if (nd) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
         }
+        if (!open.empty()) void_src = counts[r & 1u];
         open.clear();
         pending = 0;
         ended = true;
         return KT_OK;
     };
-    uint32_t next = iter;  // the next iteration to queue
+    uint32_t next = head ? iter + 1 : iter;  // the next iteration to queue
     for (;;) {
         const int s = (int)(next % kAheadSlots);
         int rc = queue(s);
         if (rc) return rc;
-        open.emplace_back(s, next++);
+        open.push_back({s, next++, true});
         bound = (uint32_t)std::min<uint64_t>(live, E);  // a queued iteration deletes live entries only
         const bool leave = compaction_due();
         while (open.size() > (leave ? 0u : kAhead)) {
@@ -977,6 +1005,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     // the model's choice is read with the next counter read-back (the JUMP's
     // or, when deletions are pending, right before the support pass)
     bool unresolved = pick == KT_PICK_MODEL;
+    if ((rc = take_void())) return rc;
     // a compaction that fell due in the last phase happens before the phase's
     // features are taken (they describe the stored rows)
     if ((rc = compact_delta_state())) return rc;
@@ -1014,6 +1043,21 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             if (unresolved) resolve_choice();
             const uint32_t s = h_ctr->min_sup;
             if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
+            // with DELTA settled for the phase, iteration 0's mark (it deletes
+            // the minimum) opens the run-ahead queue: its count comes back
+            // with the next iteration's
+            if (s != 0xFFFFFFFFu && k > 3 && !counted && pick != KT_PICK_SWEEP && !unresolved &&
+                cfg.kernel == KT_COUNT_DELTA && !compaction_due()) {
+                KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
+                                           sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E,
+                                           k, snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(),
+                                           sb.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, nullptr, nullptr,
+                                           0, src[cur].as<uint32_t>(), live_rows_ptr(), stream));
+                bool ended = false;
+                if ((rc = run_ahead_delta(k, iter, phase_ms, ended, true))) return rc;
+                changed = !ended;
+                continue;
+            }
         }
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,

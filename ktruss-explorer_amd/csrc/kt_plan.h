@@ -102,6 +102,9 @@ struct Plan {
     bool delta_ready = false;
     bool last_pass_delta = false;  // the last support pass was a DELTA update
     uint32_t pending = 0;
+    // count of the void iteration that ended the last run-ahead DELTA phase:
+    // moved into ctr->void_del by take_void, which every read-back checks
+    unsigned int* void_src = nullptr;
     bool delta_capable() const;
 
     int resolve(const kt_count_config& in, CountCfg& out) const;
@@ -124,7 +127,8 @@ struct Plan {
     bool compaction_due() const;  // dead entries are more than a third of the stored ones
     int restore_delta();
     int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
-    int run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended);
+    int run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended, bool head = false);
+    int take_void();
     int record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms);
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
     // defer: leave the model's choice on the device (read with the next
