@@ -200,6 +200,13 @@ __global__ void k_take_count(unsigned int* from, unsigned int* acc) {
 // taken after a device-scope fence) scans the tile totals in place, so the
 // first task of deletion i is tsum[i / kTileD] + lofs[i] and the task total
 // is tsum[ntiles] = ctr->dtasks.
+// A thread takes the deletions i = tile + r * kBlock + threadIdx.x (r <
+// kPerD) and issues every load of a step for all of them before the first
+// wait (indices past the end repeat the last deletion; only their stores are
+// skipped), so a tile costs three dependent memory round trips; the counts
+// go through LDS to be scanned in deletion order.  (With each thread taking
+// kPerD consecutive deletions under a lane condition, the compiler waits for
+// each one's loads before the next: 3 x kPerD round trips per tile.)
 constexpr uint32_t kTileD = 2048;
 constexpr uint32_t kPerD = kTileD / kBlock;  // deletions per thread of a tile
 
@@ -212,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
                                                   uint32_t* tsum, DevCounters* ctr, unsigned int* zero_cnt,
                                                   uint32_t* host_cnt) {
     __shared__ uint32_t s_w[kBlock / kWave];
+    __shared__ uint32_t s_c[kTileD];
     __shared__ uint32_t s_last;
     const uint32_t nd = *nd_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -228,24 +236,52 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
         return;
     }
     if (blockIdx.x >= ntiles) return;  // whole workgroup
+    const uint32_t last = nd - 1;
     uint32_t tile_total = 0;  // the last tile's total (thread kBlock - 1)
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t base = t * kTileD + threadIdx.x;
+        uint32_t d[kPerD], u[kPerD], dv[kPerD];
+#pragma unroll
+        for (uint32_t r = 0; r < kPerD; ++r) {
+            const uint32_t i = base + r * kBlock;
+            d[r] = dl[i < last ? i : last];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kPerD; ++r) {
+            u[r] = src[d[r]];
+            dv[r] = dst[d[r]];
+        }
+        uint32_t ub[kPerD], ue[kPerD], vb[kPerD], ve[kPerD];
+#pragma unroll
+        for (uint32_t r = 0; r < kPerD; ++r) {
+            const uint32_t v = dv[r] & KT_IDMASK;
+            ub[r] = sptr[u[r]];
+            ue[r] = sptr[u[r] + 1];
+            vb[r] = sptr[v];
+            ve[r] = sptr[v + 1];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kPerD; ++r) {
+            const uint32_t i = base + r * kBlock;
+            const bool us = ue[r] - ub[r] <= ve[r] - vb[r];
+            const uint4 m = us ? make_uint4(ub[r], ue[r], vb[r], ve[r] - vb[r])
+                               : make_uint4(vb[r], ve[r], ub[r], ue[r] - ub[r]);
+            uint32_t c = 0;
+            if (i < nd) {
+                meta[i] = m;
+                c = (m.y - m.x + kWave - 1) / kWave;
+            }
+            s_c[r * kBlock + threadIdx.x] = c;
+        }
+        __syncthreads();
+        // the scan runs in deletion order: thread x holds deletions
+        // tile + x * kPerD + [0, kPerD)
         const uint32_t i0 = t * kTileD + threadIdx.x * kPerD;
         uint32_t cnt[kPerD];
         uint32_t sum = 0;
 #pragma unroll
         for (uint32_t r = 0; r < kPerD; ++r) {
-            const uint32_t i = i0 + r;
-            cnt[r] = 0;
-            if (i < nd) {
-                const uint32_t d = dl[i];
-                const uint32_t u = src[d], v = dst[d] & KT_IDMASK;
-                const uint32_t ub = sptr[u], ue = sptr[u + 1], vb = sptr[v], ve = sptr[v + 1];
-                const bool us = ue - ub <= ve - vb;
-                const uint4 m = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
-                meta[i] = m;
-                cnt[r] = (m.y - m.x + kWave - 1) / kWave;
-            }
+            cnt[r] = s_c[threadIdx.x * kPerD + r];
             sum += cnt[r];
         }
         uint32_t inc = sum;  // inclusive scan over the workgroup
