@@ -57,6 +57,12 @@ __device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / k
 // LDS; the global list is reserved with one atomic per workgroup when the
 // LDS list is full and at the end.  A reservation per wave would put one
 // returning atomic per wave on a single address, served one at a time.
+// A live entry's layout word and adjacency slots hold plain ids (sa[e]: the
+// slot in src[e]'s list, which holds dst[e]; sb[e]: the one holding src[e]),
+// so flags are stored, not read, modified and written.
+// The finalization of a previous list runs in workgroups of its own beside
+// the scan (kFinItems entries per thread in flight): its entries' supports
+// are KT_SUP_GONE, so the scan never reads their flags.
 // A deleted entry whose update is done: KT_DEAD in the oriented layout and
 // in both of its adjacency slots.
 __device__ __forceinline__ void finalize_one(uint32_t d, uint32_t* dst, uint32_t* snbr,
@@ -71,6 +77,41 @@ constexpr uint32_t kMarkItems = 8;
 constexpr uint32_t kMarkTile = kBlock * kMarkItems;
 constexpr uint32_t kMarkBuf = 2 * kMarkTile;
 constexpr uint32_t kMarkGrid = 1024;
+constexpr uint32_t kFinItems = 8;
+constexpr uint32_t kFinGrid = 64;  // most finalizing workgroups of a mark launch
+
+// Finalizes fin_dl[0, *fin_nd) with nb workgroups (this one is fb of them).
+__device__ __forceinline__ void finalize_list(const uint32_t* __restrict__ fin_dl,
+                                              const uint32_t* __restrict__ fin_nd, uint32_t fb, uint32_t nb,
+                                              const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                              uint32_t* __restrict__ snbr, const uint32_t* __restrict__ sa,
+                                              const uint32_t* __restrict__ sb) {
+    const uint32_t nf = *fin_nd;
+    const uint32_t stride = nb * kBlock;
+    for (uint32_t t0 = fb * kBlock + threadIdx.x; t0 < nf; t0 += kFinItems * stride) {
+        uint32_t f[kFinItems], fu[kFinItems], fv[kFinItems], fa[kFinItems], fc[kFinItems];
+#pragma unroll
+        for (uint32_t r = 0; r < kFinItems; ++r) {
+            const uint32_t t = t0 + r * stride;
+            f[r] = fin_dl[t < nf ? t : nf - 1];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kFinItems; ++r) {
+            fu[r] = src[f[r]];
+            fv[r] = dst[f[r]] & KT_IDMASK;
+            fa[r] = sa[f[r]];
+            fc[r] = sb[f[r]];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kFinItems; ++r) {
+            if (t0 + r * stride < nf) {
+                dst[f[r]] = fv[r] | KT_DEAD;
+                snbr[fa[r]] = fv[r] | KT_DEAD;
+                snbr[fc[r]] = fu[r] | KT_DEAD;
+            }
+        }
+    }
+}
 
 __device__ __forceinline__ void mark_flush(const uint32_t* s_out, uint32_t* s_cnt, uint32_t* s_base,
                                            uint32_t* dl, unsigned int* ndel) {
@@ -96,20 +137,20 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                                                        const uint32_t* __restrict__ fin_dl,
                                                        const uint32_t* __restrict__ fin_nd,
                                                        const uint32_t* __restrict__ src,
-                                                       uint32_t* __restrict__ ldeg) {
+                                                       uint32_t* __restrict__ ldeg, uint32_t mark_blocks) {
     __shared__ uint32_t s_out[kMarkBuf];
     __shared__ uint32_t s_cnt, s_base;
-    if (threadIdx.x == 0) s_cnt = 0;
-    if (fin_dl) {  // the previous deletions, whose update is done, become KT_DEAD
-        const uint32_t nf = *fin_nd;
-        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x)
-            finalize_one(fin_dl[t], dst, snbr, sa, sb);
+    if (blockIdx.x >= mark_blocks) {  // the previous deletions, whose update is done, become KT_DEAD
+        finalize_list(fin_dl, fin_nd, blockIdx.x - mark_blocks, gridDim.x - mark_blocks, src, dst, snbr, sa,
+                      sb);
+        return;
     }
+    if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t ntile = (n + kMarkTile - 1) / kMarkTile;
-    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += mark_blocks) {
         const uint32_t e0 = tile * kMarkTile + threadIdx.x;
         uint32_t sp[kMarkItems];
 #pragma unroll
@@ -137,13 +178,14 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
             b = __shfl(b, 0);
             if (del) {
                 s_out[b + (uint32_t)__popcll(m & below)] = e;
+                const uint32_t ei = eid[e], a = sa[e], c = sb[e], u = src[e];
+                const uint32_t vid = v[j] & KT_IDMASK;
                 sup[e] = KT_SUP_GONE;
-                dst[e] = v[j] | KT_DYING;
-                tau[eid[e]] = k - 1;
-                const uint32_t a = sa[e], c = sb[e];
-                snbr[a] |= KT_DYING;
-                snbr[c] |= KT_DYING;
-                if (ldeg) atomicSub(ldeg + src[e], 1u);
+                dst[e] = vid | KT_DYING;
+                tau[ei] = k - 1;
+                snbr[a] = vid | KT_DYING;
+                snbr[c] = u | KT_DYING;
+                if (ldeg) atomicSub(ldeg + u, 1u);
             }
         }
         __syncthreads();
@@ -775,8 +817,11 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
         return hipSuccess;
     }
     const uint32_t nt = (n + kMarkTile - 1) / kMarkTile;
-    hipLaunchKernelGGL(k_mark_dying, dim3(nt < kMarkGrid ? nt : kMarkGrid), dim3(kBlock), 0, s, dst,
-                       eid, sup, tau, dl, n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg);
+    const uint32_t mb = nt < kMarkGrid ? nt : kMarkGrid;
+    // finalizing workgroups: fin_max is a bound (the grid strides over the true count)
+    const uint32_t fw = (uint32_t)std::min<uint64_t>(((uint64_t)fin_max + kBlock - 1) / kBlock, kFinGrid);
+    hipLaunchKernelGGL(k_mark_dying, dim3(mb + (fin_dl ? fw : 0u)), dim3(kBlock), 0, s, dst, eid, sup, tau, dl,
+                       n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg, mb);
     return hipGetLastError();
 }
 
