@@ -50,8 +50,9 @@ __device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / k
 // is flagged KT_DYING and listed; it becomes KT_DEAD in k_finalize.
 // The deleted entry's two adjacency slots get KT_DYING as well.
 // The scan reads supports only: a deleted entry's support becomes
-// KT_SUP_GONE (above every threshold), and an entry's flags are read only when
-// its support is below k - 2.
+// KT_SUP_GONE (above every threshold), so an entry below k - 2 is live; its
+// layout word is loaded with the other words its flagging needs (and a flag
+// found there is counted in *bad, never expected).
 // A workgroup walks tiles of kMarkTile entries (every lane loads all of its
 // supports before the first wait) and lists its deletions in
 // LDS; the global list is reserved with one atomic per workgroup when the
@@ -137,7 +138,8 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                                                        const uint32_t* __restrict__ fin_dl,
                                                        const uint32_t* __restrict__ fin_nd,
                                                        const uint32_t* __restrict__ src,
-                                                       uint32_t* __restrict__ ldeg, uint32_t mark_blocks) {
+                                                       uint32_t* __restrict__ ldeg, unsigned int* bad,
+                                                       uint32_t mark_blocks) {
     __shared__ uint32_t s_out[kMarkBuf];
     __shared__ uint32_t s_cnt, s_base;
     if (blockIdx.x >= mark_blocks) {  // the previous deletions, whose update is done, become KT_DEAD
@@ -159,18 +161,10 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
             const uint32_t x = sup[e < n ? e : n - 1];  // unconditional: the loads stay in flight together
             sp[j] = e < n ? x : KT_SUP_GONE;
         }
-        // flags are read only for items where some lane is below k - 2 (all
-        // such loads in flight together; lanes past the end read entry n - 1)
-        uint32_t v[kMarkItems];
 #pragma unroll
         for (uint32_t j = 0; j < kMarkItems; ++j) {
             const uint32_t e = e0 + j * kBlock;
-            v[j] = ballot(sp[j] < k - 2) ? dst[e < n ? e : n - 1] : 0u;
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kMarkItems; ++j) {
-            const uint32_t e = e0 + j * kBlock;
-            const bool del = sp[j] < k - 2 && !(v[j] & (KT_DEAD | KT_DYING));
+            const bool del = sp[j] < k - 2;
             const uint64_t m = ballot(del);
             if (!m) continue;
             uint32_t b = 0;
@@ -178,8 +172,9 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
             b = __shfl(b, 0);
             if (del) {
                 s_out[b + (uint32_t)__popcll(m & below)] = e;
-                const uint32_t ei = eid[e], a = sa[e], c = sb[e], u = src[e];
-                const uint32_t vid = v[j] & KT_IDMASK;
+                const uint32_t dv = dst[e], ei = eid[e], a = sa[e], c = sb[e], u = src[e];
+                const uint32_t vid = dv & KT_IDMASK;
+                if (dv & (KT_DEAD | KT_DYING)) atomicAdd(bad, 1u);
                 sup[e] = KT_SUP_GONE;
                 dst[e] = vid | KT_DYING;
                 tau[ei] = k - 1;
@@ -196,12 +191,12 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
 }
 
 // Minimum live support (the JUMP target); one atomic per workgroup.
-__global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
-                           const uint32_t* __restrict__ sup, DevCounters* ctr) {
+__global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCounters* ctr) {
     __shared__ uint32_t s_min[kBlock / kWave];
     uint32_t m = 0xFFFFFFFFu;
     // eight supports per thread in flight (indices past the end read entry
-    // n - 1, which is then not counted); flags read for a new minimum only
+    // n - 1, which is then not counted); dead and dying entries hold
+    // KT_SUP_GONE, above every live support
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x; e0 < n; e0 += 8 * stride) {
         uint32_t x[8];
@@ -213,7 +208,7 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ dst,
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) {
             const uint32_t e = e0 + j * stride;
-            if (e < n && x[j] < m && !(dst[e] & (KT_DEAD | KT_DYING))) m = x[j];
+            if (e < n && x[j] < m) m = x[j];
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -810,7 +805,7 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
                       const uint32_t* fin_nd, uint32_t fin_max, const uint32_t* src, uint32_t* ldeg,
-                      hipStream_t s) {
+                      unsigned int* bad, hipStream_t s) {
     if (!fin_max) fin_dl = nullptr;
     if (!n) {
         if (fin_dl) return finalize_deleted(fin_dl, fin_nd, fin_max, dst, snbr, sa, sb, s);
@@ -821,7 +816,7 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
     // finalizing workgroups: fin_max is a bound (the grid strides over the true count)
     const uint32_t fw = (uint32_t)std::min<uint64_t>(((uint64_t)fin_max + kBlock - 1) / kBlock, kFinGrid);
     hipLaunchKernelGGL(k_mark_dying, dim3(mb + (fin_dl ? fw : 0u)), dim3(kBlock), 0, s, dst, eid, sup, tau, dl,
-                       n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg, mb);
+                       n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg, bad, mb);
     return hipGetLastError();
 }
 
@@ -830,12 +825,11 @@ hipError_t take_count(unsigned int* from, unsigned int* acc, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
-                          hipStream_t s) {
+hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s) {
     if (!n) return hipSuccess;
     const uint32_t b = blocks_for(n, kBlock);
     // at most 256 workgroups: one atomic each on the one counter
-    hipLaunchKernelGGL(k_min_list, dim3(b < 256 ? b : 256), dim3(kBlock), 0, s, n, dst, sup, ctr);
+    hipLaunchKernelGGL(k_min_list, dim3(b < 256 ? b : 256), dim3(kBlock), 0, s, n, sup, ctr);
     return hipGetLastError();
 }
 

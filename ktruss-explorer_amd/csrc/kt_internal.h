@@ -68,7 +68,7 @@ struct DevCounters {
     unsigned int dticket;             // DELTA task list: workgroups done (reset by the last)
     unsigned int ndel2;               // second deletion count (DELTA iterations run ahead)
     unsigned int void_del;            // deletions of void DELTA iterations (stays 0)
-    unsigned int pad0;
+    unsigned int mark_bad;            // entries the DELTA mark found deleted already (stays 0)
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
     unsigned long long delta_win;     // longer-list window entries read by DELTA updates
@@ -137,25 +137,28 @@ hipError_t build_worklist(const uint32_t* src, const uint32_t* dst, uint8_t* sta
                           bool directed, DevCounters* ctr, hipStream_t s);
 hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E,
                             DevCounters* ctr, hipStream_t s);
-// DELTA support maintenance (kt_delta.hip).
+// DELTA support maintenance (kt_delta.hip).  A deleted entry's support is
+// KT_SUP_GONE from its mark on, so a support below a threshold is a live
+// entry's.
 // The deletions are appended to dl, counted in *ndel (zero on entry).  fin_dl
 // / fin_nd (optional, at most fin_max entries): a previous deletion list
 // whose update is done, finalized (KT_DEAD) by the same kernel.  ldeg
 // (optional): live length of every oriented row, decremented at src[e] for
-// every deleted entry e.
+// every deleted entry e.  *bad counts entries below the threshold that were
+// flagged already (none while the supports keep the rule above).
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
                       const uint32_t* fin_nd, uint32_t fin_max, const uint32_t* src, uint32_t* ldeg,
-                      hipStream_t s);
+                      unsigned int* bad, hipStream_t s);
 // out[v] = ptr[v + 1] - ptr[v] for v < nv.
 hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s);
-// Minimum support of the live entries (those neither dead nor dying).
+// Minimum support of the live entries (those neither dead nor dying: the
+// others' supports are KT_SUP_GONE).
 // acc |= *from; *from = 0 (one thread): takes a deletion count that is
 // checked later, at the next counter read-back.
 hipError_t take_count(unsigned int* from, unsigned int* acc, hipStream_t s);
-hipError_t min_live_delta(uint32_t n, const uint32_t* dst, const uint32_t* sup, DevCounters* ctr,
-                          hipStream_t s);
+hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s);
 // Scratch of a DELTA update for up to `cap` pending deletions.
 struct DeltaScratch {
     unsigned long long* bstat;  // per k_delta workgroup: elems, win, live, tris

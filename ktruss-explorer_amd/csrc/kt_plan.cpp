@@ -196,6 +196,7 @@ int Plan::sync_counters(bool fold) {
     KT_HIP_TRY(hipMemcpyAsync(h_ctr, ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
     if (h_ctr->void_del) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
+    if (h_ctr->mark_bad) return fail(KT_ERR_STATE, "the DELTA mark found a deleted entry below the threshold");
     return KT_OK;
 }
 
@@ -897,7 +898,8 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                    tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
-                                   bound, src[cur].as<uint32_t>(), live_rows_ptr(), stream));
+                                   bound, src[cur].as<uint32_t>(), live_rows_ptr(),
+                                   &ctr.as<DevCounters>()->mark_bad, stream));
         prev_slot = s;
         ++r;
         return KT_OK;
@@ -1037,8 +1039,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         if ((rc = compact_delta_state())) return rc;
         if (iter == 0 && jump) {
             if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
-            KT_HIP_TRY(dev::min_live_delta(E, dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
-                                           ctr.as<DevCounters>(), stream));
+            KT_HIP_TRY(dev::min_live_delta(E, sup[cur].as<uint32_t>(), ctr.as<DevCounters>(), stream));
             if ((rc = sync_counters())) return rc;
             if (unresolved) resolve_choice();
             const uint32_t s = h_ctr->min_sup;
@@ -1052,7 +1053,8 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
                                            sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E,
                                            k, snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(),
                                            sb.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, nullptr, nullptr,
-                                           0, src[cur].as<uint32_t>(), live_rows_ptr(), stream));
+                                           0, src[cur].as<uint32_t>(), live_rows_ptr(),
+                                   &ctr.as<DevCounters>()->mark_bad, stream));
                 bool ended = false;
                 if ((rc = run_ahead_delta(k, iter, phase_ms, ended, true))) return rc;
                 changed = !ended;
@@ -1063,7 +1065,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
                                    snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                    &ctr.as<DevCounters>()->ndel, nullptr, nullptr, 0, src[cur].as<uint32_t>(),
-                                   live_rows_ptr(), stream));
+                                   live_rows_ptr(), &ctr.as<DevCounters>()->mark_bad, stream));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;
