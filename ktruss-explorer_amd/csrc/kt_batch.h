@@ -56,14 +56,16 @@ hipError_t batch_expand(const BatchDev& b, uint32_t blocks, hipStream_t s);
 hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, uint32_t blocks,
                       hipStream_t s);
 // dl != nullptr: DELTA marking (KT_DYING on the entry and its two adjacency
-// slots snbr[sa[e]], snbr[sb[e]]; deleted entries appended to dl, count *dn);
+// slots snbr[sa[e]], snbr[sb[e]]; deleted entries appended to dl, count *dn;
+// src: the entries' rows; *bad counts entries below the threshold that were
+// flagged already, none while deleted entries hold KT_SUP_GONE);
 // fin_dl / fin_nd: the previous step's deletions, whose update is done,
-// become KT_DEAD in the same kernel.
+// become KT_DEAD in the same launch (workgroups beyond `blocks`).
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
                       unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
-                      const uint32_t* fin_dl, const unsigned int* fin_nd, uint32_t blocks,
-                      hipStream_t s);
+                      const uint32_t* fin_dl, const unsigned int* fin_nd, const uint32_t* src,
+                      unsigned int* bad, uint32_t blocks, hipStream_t s);
 hipError_t batch_min(const TaskList& tl, const uint32_t* dst, const uint32_t* sup, uint32_t* gmin,
                      uint32_t blocks, hipStream_t s);
 }  // namespace dev

@@ -60,8 +60,18 @@ __global__ void k_b_zero(TaskList tl, const uint32_t* __restrict__ dst, uint32_t
 // slice and the workgroup reserves room in dl with one atomic at the end (a
 // wave whose slice fills reserves for it alone): a reservation per 64-entry
 // chunk would put one returning atomic per chunk on a single address.
+// A deleted entry's support is KT_SUP_GONE from its DELTA mark on, so an
+// entry below k - 2 is live: its layout word is loaded with the other words
+// its flagging needs (a flag found there is counted in *bad, never expected),
+// and flags are stored, not read, modified and written (a live entry's slots
+// hold plain ids: sa[i], in src[i]'s list, holds dst[i]; sb[i] holds src[i]).
+// The previous step's deletions are finalized by kBFinBlocks workgroups of
+// their own beside the scan, kBFinItems entries per thread in flight (their
+// supports are KT_SUP_GONE: the scan never reads their flags).
 constexpr uint32_t kBMarkCap = 1024;  // LDS list entries per wave
 constexpr uint32_t kMarkGroup = 8;    // 64-entry chunks whose loads are in flight together
+constexpr uint32_t kBFinBlocks = 64;
+constexpr uint32_t kBFinItems = 8;
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -78,17 +88,40 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
                                                 const uint32_t* __restrict__ sa,
                                                 const uint32_t* __restrict__ sb,
                                                 const uint32_t* __restrict__ fin_dl,
-                                                const unsigned int* __restrict__ fin_nd) {
+                                                const unsigned int* __restrict__ fin_nd,
+                                                const uint32_t* __restrict__ src, unsigned int* bad,
+                                                uint32_t mark_blocks) {
     __shared__ uint32_t s_list[DELTA ? (256 / kWave) * kBMarkCap : 1];
-    if (DELTA && fin_dl) {  // the previous step's deletions: KT_DEAD, entry and both slots
+    if (DELTA && blockIdx.x >= mark_blocks) {
+        // the previous step's deletions: KT_DEAD, entry and both slots
+        if (!fin_dl) return;
         const uint32_t nf = *fin_nd;
-        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
-            const uint32_t d = fin_dl[t];
-            dst[d] = (dst[d] & ~KT_DYING) | KT_DEAD;
-            const uint32_t a = sa[d], b = sb[d];
-            snbr[a] = (snbr[a] & ~KT_DYING) | KT_DEAD;
-            snbr[b] = (snbr[b] & ~KT_DYING) | KT_DEAD;
+        const uint32_t stride = (gridDim.x - mark_blocks) * blockDim.x;
+        for (uint32_t t0 = (blockIdx.x - mark_blocks) * blockDim.x + threadIdx.x; t0 < nf;
+             t0 += kBFinItems * stride) {
+            uint32_t f[kBFinItems], fu[kBFinItems], fv[kBFinItems], fa[kBFinItems], fb[kBFinItems];
+#pragma unroll
+            for (uint32_t r = 0; r < kBFinItems; ++r) {
+                const uint32_t t = t0 + r * stride;
+                f[r] = fin_dl[t < nf ? t : nf - 1];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kBFinItems; ++r) {
+                fu[r] = src[f[r]];
+                fv[r] = dst[f[r]] & KT_IDMASK;
+                fa[r] = sa[f[r]];
+                fb[r] = sb[f[r]];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kBFinItems; ++r) {
+                if (t0 + r * stride < nf) {
+                    dst[f[r]] = fv[r] | KT_DEAD;
+                    snbr[fa[r]] = fv[r] | KT_DEAD;
+                    snbr[fb[r]] = fu[r] | KT_DEAD;
+                }
+            }
         }
+        return;
     }
     __shared__ uint32_t s_n[256 / kWave], s_base;
     const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -96,7 +129,8 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
     uint32_t* my = s_list + (DELTA ? wid * kBMarkCap : 0);
     uint32_t cnt = 0;  // wave-uniform length of the wave's LDS list
     const uint32_t n = *tl.ntasks;
-    for (uint32_t t = gwave(); t < n; t += nwaves()) {
+    const uint32_t nw = mark_blocks * (256 / kWave);
+    for (uint32_t t = gwave(); t < n; t += nw) {
         const uint4 d = tl.desc[t];
         const uint32_t k = gk[d.z];
         uint32_t ndel = 0, smin = 0xFFFFFFFFu;
@@ -115,9 +149,12 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
                 }
                 if (del) {
                     my[cnt + (uint32_t)__popcll(m & below)] = i;
-                    const uint32_t a = sa[i], b = sb[i];
-                    snbr[a] |= KT_DYING;  // the deleted edge's adjacency slots
-                    snbr[b] |= KT_DYING;
+                    const uint32_t dv = dst[i], a = sa[i], b = sb[i], u = src[i];
+                    const uint32_t vid = dv & KT_IDMASK;
+                    if (dv & (KT_DEAD | KT_DYING)) atomicAdd(bad, 1u);
+                    dst[i] = vid | KT_DYING;  // the entry and its two adjacency slots
+                    snbr[a] = vid | KT_DYING;
+                    snbr[b] = u | KT_DYING;
                 }
                 cnt += (uint32_t)__popcll(m);
             }
@@ -125,10 +162,10 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
         };
         if (DELTA) {
             // supports only (a deleted entry's support is KT_SUP_GONE), eight
-            // chunks in flight; flags read where some lane is below k - 2
+            // chunks in flight
             const uint32_t last = d.y - 1;
             for (uint32_t base = d.x; base < d.y; base += kMarkGroup * kWave) {
-                uint32_t sp[kMarkGroup], v[kMarkGroup];
+                uint32_t sp[kMarkGroup];
 #pragma unroll
                 for (uint32_t j = 0; j < kMarkGroup; ++j) {
                     const uint32_t i = base + j * kWave + lane;
@@ -138,14 +175,8 @@ __global__ __launch_bounds__(256) void k_b_mark(TaskList tl, uint32_t* __restric
 #pragma unroll
                 for (uint32_t j = 0; j < kMarkGroup; ++j) {
                     const uint32_t i = base + j * kWave + lane;
-                    v[j] = ballot(sp[j] < k - 2) ? dst[i < last ? i : last] : 0u;
-                }
-#pragma unroll
-                for (uint32_t j = 0; j < kMarkGroup; ++j) {
-                    const uint32_t i = base + j * kWave + lane;
-                    const bool del = sp[j] < k - 2 && !(v[j] & (KT_DEAD | KT_DYING));
+                    const bool del = sp[j] < k - 2;
                     if (del) {
-                        dst[i] = v[j] | KT_DYING;
                         sup[i] = KT_SUP_GONE;
                         tau[i] = k - 1;
                     } else if (sp[j] >= k - 2 && sp[j] < smin) {
@@ -398,14 +429,15 @@ hipError_t batch_zero(const TaskList& tl, const uint32_t* dst, uint32_t* sup, ui
 hipError_t batch_mark(const TaskList& tl, uint32_t* dst, uint32_t* sup, uint32_t* tau,
                       const uint32_t* gk, uint32_t* gndel, uint32_t* gmin, uint32_t* dl,
                       unsigned int* dn, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
-                      const uint32_t* fin_dl, const unsigned int* fin_nd, uint32_t blocks,
-                      hipStream_t s) {
+                      const uint32_t* fin_dl, const unsigned int* fin_nd, const uint32_t* src,
+                      unsigned int* bad, uint32_t blocks, hipStream_t s) {
     if (dl)
-        hipLaunchKernelGGL(k_b_mark<true>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
-                           gk, gndel, gmin, dl, dn, snbr, sa, sb, fin_dl, fin_nd);
+        hipLaunchKernelGGL(k_b_mark<true>, dim3(blocks + (fin_dl ? kBFinBlocks : 0u)), dim3(kStreamBlock), 0,
+                           s, tl, dst, sup, tau, gk, gndel, gmin, dl, dn, snbr, sa, sb, fin_dl, fin_nd, src,
+                           bad, blocks);
     else
         hipLaunchKernelGGL(k_b_mark<false>, dim3(blocks), dim3(kStreamBlock), 0, s, tl, dst, sup, tau,
-                           gk, gndel, gmin, dl, dn, snbr, sa, sb, nullptr, nullptr);
+                           gk, gndel, gmin, dl, dn, snbr, sa, sb, nullptr, nullptr, nullptr, nullptr, blocks);
     return hipGetLastError();
 }
 

@@ -355,7 +355,7 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
         KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
                                    bd.gndel, bd.gmin, lists[p], counts[p], snbr.as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), lists[p ^ 1u], counts[p ^ 1u],
-                                   stream_blocks(), stream));
+                                   src.as<uint32_t>(), &bd.bc->count.mark_bad, stream_blocks(), stream));
         KT_HIP_TRY(dev::delta_prepare(lists[p], counts[p], E, src.as<uint32_t>(), dst.as<uint32_t>(),
                                       sptr.as<uint32_t>(), dws, &bd.bc->count, stream));
         if (c0) KT_HIP_TRY(hipEventRecord(c0, stream));
@@ -370,7 +370,7 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
     }
     KT_HIP_TRY(dev::batch_mark(atl, dst.as<uint32_t>(), sup.as<uint32_t>(), tau.as<uint32_t>(), bd.gk,
                                bd.gndel, bd.gmin, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                               nullptr, stream_blocks(), stream));
+                               nullptr, nullptr, nullptr, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_control(bd, 2, stream));
     KT_HIP_TRY(dev::batch_expand(bd, stream_blocks(), stream));
     KT_HIP_TRY(dev::batch_zero(dtl, dst.as<uint32_t>(), sup.as<uint32_t>(), stream_blocks(), stream));
@@ -486,6 +486,8 @@ int Batch::decompose() {
         stats.phases += ph;
         stats.count_passes += pa;
     }
+    if (h_bc->count.mark_bad)
+        return fail(KT_ERR_STATE, "the DELTA mark found a deleted entry below the threshold");
     stats.graphs = G;
     stats.entries = E;
     stats.steps = h_bc->steps;
