@@ -223,9 +223,13 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCoun
     }
 }
 
-__global__ void k_take_count(unsigned int* from, unsigned int* acc) {
-    *acc |= *from;
-    *from = 0;
+__global__ void k_phase_start(unsigned int* from, DevCounters* ctr, int zero_ndel, int reset_min) {
+    if (from) {
+        ctr->void_del |= *from;
+        *from = 0;
+    }
+    if (zero_ndel) ctr->ndel = 0;
+    if (reset_min) ctr->min_sup = 0xFFFFFFFFu;
 }
 
 // Task list of the pending deletions, one kernel.  Per deletion i (d = dl[i]
@@ -820,8 +824,9 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
     return hipGetLastError();
 }
 
-hipError_t take_count(unsigned int* from, unsigned int* acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_take_count, dim3(1), dim3(1), 0, s, from, acc);
+hipError_t phase_start(unsigned int* from, DevCounters* ctr, bool zero_ndel, bool reset_min,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_phase_start, dim3(1), dim3(1), 0, s, from, ctr, zero_ndel ? 1 : 0, reset_min ? 1 : 0);
     return hipGetLastError();
 }
 

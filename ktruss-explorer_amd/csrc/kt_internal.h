@@ -155,9 +155,11 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
 hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying: the
 // others' supports are KT_SUP_GONE).
-// acc |= *from; *from = 0 (one thread): takes a deletion count that is
-// checked later, at the next counter read-back.
-hipError_t take_count(unsigned int* from, unsigned int* acc, hipStream_t s);
+// One thread at the start of a DELTA phase: ctr->void_del |= *from; *from = 0
+// (from: a deletion count checked at the next counter read-back; optional),
+// then ctr->ndel = 0 (zero_ndel) and ctr->min_sup = ~0 (reset_min).
+hipError_t phase_start(unsigned int* from, DevCounters* ctr, bool zero_ndel, bool reset_min,
+                       hipStream_t s);
 hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s);
 // Scratch of a DELTA update for up to `cap` pending deletions.
 struct DeltaScratch {

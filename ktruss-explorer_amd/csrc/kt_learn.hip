@@ -70,6 +70,8 @@ __device__ void block_sum_store(unsigned long long (&v)[N], bool is_max_3,
 // large the graph.  One kernel: the rows (live lengths when given, else the
 // stored ones) and the sampled entries (dst, sup).
 constexpr uint32_t kFeatSample = 1u << 22;
+constexpr uint32_t kRowItems = 8;    // rows per thread whose loads are in flight together
+constexpr uint32_t kChunkItems = 8;  // sampled chunks per wave whose loads are in flight together
 
 __global__ __launch_bounds__(kFeatThreads) void k_feat_scan(const uint32_t* __restrict__ ptr,
                                                            const uint32_t* __restrict__ rowlen, uint32_t n,
@@ -78,30 +80,43 @@ __global__ __launch_bounds__(kFeatThreads) void k_feat_scan(const uint32_t* __re
                                                            uint32_t stride, uint32_t k, int sup_valid,
                                                            unsigned long long* part) {
     unsigned long long v[6] = {0, 0, 0, 0, 0, 0};  // rows, s1, s2, max, big, huge
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        const uint32_t len = rowlen ? rowlen[r] : ptr[r + 1] - ptr[r];
-        if (!len) continue;
-        v[0] += 1;
-        v[1] += len;
-        v[2] += (unsigned long long)len * len;
-        v[3] = len > v[3] ? len : v[3];
-        if (len > 256) v[4] += len;
-        if (len > 1024) v[5] += len;
+    // kRowItems rows per thread with their loads in flight together (rows past
+    // the end read row n - 1 and count as empty)
+    const uint32_t nth = gridDim.x * blockDim.x;
+    for (uint32_t r0 = blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += kRowItems * nth) {
+        uint32_t len[kRowItems];
+#pragma unroll
+        for (uint32_t q = 0; q < kRowItems; ++q) {
+            const uint32_t r = r0 + q * nth, rr = r < n ? r : n - 1;
+            const uint32_t l = rowlen ? rowlen[rr] : ptr[rr + 1] - ptr[rr];
+            len[q] = r < n ? l : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kRowItems; ++q) {
+            const uint32_t l = len[q];
+            if (!l) continue;
+            v[0] += 1;
+            v[1] += l;
+            v[2] += (unsigned long long)l * l;
+            v[3] = l > v[3] ? l : v[3];
+            if (l > 256) v[4] += l;
+            if (l > 1024) v[5] += l;
+        }
     }
     block_sum_store<6>(v, true, part + (size_t)blockIdx.x * KT_FEAT_ACC);
     __syncthreads();  // block_sum_store's LDS is reused below
     unsigned long long w[6] = {0, 0, 0, 0, 0, 0};  // 5 bins, live entries (sampled)
     const float denom = (float)(k > 2 ? k - 2 : 1);
     const uint32_t lane = threadIdx.x & 63;
-    // four sampled chunks per round, their loads in flight together (indices
-    // past the end read entry E - 1 and are not counted)
+    // kChunkItems sampled chunks per round, their loads in flight together
+    // (indices past the end read entry E - 1 and are not counted)
     const uint64_t cstep = (uint64_t)gridDim.x * (blockDim.x / 64) * stride * 64;
     for (uint64_t c = ((uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * stride * 64; c < E;
-         c += 4 * cstep) {
-        uint32_t dv[4], sv[4];
-        bool in[4];
+         c += kChunkItems * cstep) {
+        uint32_t dv[kChunkItems], sv[kChunkItems];
+        bool in[kChunkItems];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (uint32_t q = 0; q < kChunkItems; ++q) {
             const uint64_t e = c + q * cstep + lane;
             in[q] = e < E;
             const uint64_t ee = in[q] ? e : E - 1;
@@ -109,7 +124,7 @@ __global__ __launch_bounds__(kFeatThreads) void k_feat_scan(const uint32_t* __re
             sv[q] = sup[ee];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (uint32_t q = 0; q < kChunkItems; ++q) {
             if (!in[q] || (dv[q] & (KT_DEAD | KT_DYING))) continue;
             w[5] += 1;
             if (sup_valid) {
@@ -256,13 +271,19 @@ __global__ __launch_bounds__(64) void k_feat_finish(const unsigned long long* pa
     __shared__ unsigned long long acc[KT_FEAT_ACC];
     __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
     const int lane = threadIdx.x;
-    // every lane folds kFeatBlocks / 64 workgroups, then the wave reduces
+    // every lane folds kFeatBlocks / 64 workgroups (all of their partials
+    // loaded before the first wait), then the wave reduces
+    static_assert(kFeatBlocks % 64 == 0, "whole waves of partials");
+    unsigned long long x[KT_FEAT_ACC][kFeatBlocks / 64];
+#pragma unroll
+    for (int j = 0; j < KT_FEAT_ACC; ++j)
+#pragma unroll
+        for (uint32_t q = 0; q < kFeatBlocks / 64; ++q) x[j][q] = part[(size_t)(lane + 64 * q) * KT_FEAT_ACC + j];
+#pragma unroll
     for (int j = 0; j < KT_FEAT_ACC; ++j) {
         unsigned long long r = 0;
-        for (uint32_t b = lane; b < kFeatBlocks; b += 64) {
-            const unsigned long long x = part[(size_t)b * KT_FEAT_ACC + j];
-            r = j == 3 ? (x > r ? x : r) : r + x;
-        }
+#pragma unroll
+        for (uint32_t q = 0; q < kFeatBlocks / 64; ++q) r = j == 3 ? (x[j][q] > r ? x[j][q] : r) : r + x[j][q];
         for (int off = 32; off > 0; off >>= 1) {
             const unsigned long long o = __shfl_xor(r, off);
             r = j == 3 ? (o > r ? o : r) : r + o;

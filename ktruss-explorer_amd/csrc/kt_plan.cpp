@@ -833,9 +833,9 @@ int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t re
     return KT_OK;
 }
 
-int Plan::take_void() {
-    if (!void_src) return KT_OK;
-    KT_HIP_TRY(dev::take_count(void_src, &ctr.as<DevCounters>()->void_del, stream));
+int Plan::take_void(bool zero_ndel, bool reset_min) {
+    if (!void_src && !zero_ndel && !reset_min) return KT_OK;
+    KT_HIP_TRY(dev::phase_start(void_src, ctr.as<DevCounters>(), zero_ndel, reset_min, stream));
     void_src = nullptr;
     return KT_OK;
 }
@@ -1007,7 +1007,10 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     // the model's choice is read with the next counter read-back (the JUMP's
     // or, when deletions are pending, right before the support pass)
     bool unresolved = pick == KT_PICK_MODEL;
-    if ((rc = take_void())) return rc;
+    // with nothing pending the deletion count is free to clear now; the JUMP
+    // minimum is reset here too (one kernel instead of three)
+    const bool start_clear = pending == 0;
+    if ((rc = take_void(start_clear, jump))) return rc;
     // a compaction that fell due in the last phase happens before the phase's
     // features are taken (they describe the stored rows)
     if ((rc = compact_delta_state())) return rc;
@@ -1035,10 +1038,9 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             counted = true;
             stats.count_passes++;
         }
-        if ((rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
+        if (!(iter == 0 && start_clear) && (rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
         if ((rc = compact_delta_state())) return rc;
         if (iter == 0 && jump) {
-            if ((rc = set_counter(offsetof(DevCounters, min_sup), 0xFFFFFFFFu))) return rc;
             KT_HIP_TRY(dev::min_live_delta(E, sup[cur].as<uint32_t>(), ctr.as<DevCounters>(), stream));
             if ((rc = sync_counters())) return rc;
             if (unresolved) resolve_choice();

@@ -128,7 +128,8 @@ struct Plan {
     int restore_delta();
     int support_pass(const CountCfg& cfg, uint32_t k, float& ms);
     int run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended, bool head = false);
-    int take_void();
+    // also clears ctr->ndel / resets ctr->min_sup in the same one-thread kernel
+    int take_void(bool zero_ndel = false, bool reset_min = false);
     int record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms);
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
     // defer: leave the model's choice on the device (read with the next
