@@ -548,11 +548,31 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
     uint32_t full = 0xFFFFFFFFu;     // deletion whose whole longer list is in buf
     uint32_t sampled = 0xFFFFFFFFu;  // deletion whose block samples are in smp
     unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
+    // deletion i: its descriptor, its first task and the next deletion's (every
+    // deletion has a task); the next task's candidates are loaded before this
+    // task's search when they belong to the same deletion, so their round trip
+    // overlaps this task's slot->entry reads
+    uint4 m = make_uint4(0, 0, 0, 0);
+    uint32_t fi = 0, fnext = 0xFFFFFFFFu;
+    uint32_t xpre = KT_DEAD;
+    bool pre = false;  // xpre holds task t's candidates (wave-uniform)
+    if (t0 < t1) {
+        fi = first_task(lofs, tsum, i);
+        fnext = i + 1 < nd ? first_task(lofs, tsum, i + 1) : 0xFFFFFFFFu;
+        m = meta[i];
+    }
     for (uint32_t t = t0; t < t1;) {
-        while (i + 1 < nd && first_task(lofs, tsum, i + 1) <= t) ++i;
+        if (t >= fnext) {
+            while (t >= fnext) {  // fnext < ~0 means i + 1 < nd
+                ++i;
+                fi = fnext;
+                fnext = i + 1 < nd ? first_task(lofs, tsum, i + 1) : 0xFFFFFFFFu;
+            }
+            m = meta[i];
+            pre = false;
+        }
         const uint32_t tc = t++;  // this task; the body may continue below
-        const uint4 m = meta[i];
-        const uint32_t j0 = m.x + (tc - first_task(lofs, tsum, i)) * kWave;
+        const uint32_t j0 = m.x + (tc - fi) * kWave;
         const uint32_t j = j0 + lane;
         acc_elems += min(kWave, m.y - j0);
         // a longer list that fits the slice is staged with the candidates'
@@ -565,7 +585,9 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
         Staged<kStage> st;
         if (fresh) st.load(snbr + m.z, m.w, 1u, lane);
         else if (fresh_smp) st.load(snbr + m.z, nb, kWave, lane);
-        const uint32_t x = j < m.y ? snbr[j] : KT_DEAD;
+        const uint32_t x = pre ? xpre : (j < m.y ? snbr[j] : KT_DEAD);
+        pre = t < t1 && t < fnext;
+        if (pre) xpre = j + kWave < m.y ? snbr[j + kWave] : KT_DEAD;
         const bool cand = !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
         acc_live += (uint32_t)__popcll(cm);
