@@ -209,8 +209,13 @@ def main():
                     help="skip the comparison run of the fixed default variant")
     ap.add_argument("--cpu-scale", type=int, default=14)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["rmat", "batch"], default="rmat",
-                    help="rmat: one R-MAT graph (default); batch: the many-graph workload")
+    ap.add_argument("--workload", choices=["rmat", "batch", "skewed"], default="rmat",
+                    help="rmat: one R-MAT graph (default); batch: the many-graph workload; "
+                         "skewed: the Chung-Lu graph with planted cliques, every variant swept")
+    ap.add_argument("--skew-scale", type=int, default=22,
+                    help="with --workload skewed: 2^scale vertices")
+    ap.add_argument("--probe-k", type=int, default=8,
+                    help="with --workload skewed: the k every variant's k-truss run finds")
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--batch-mode", choices=["delta", "recount"], default="delta")
     ap.add_argument("--tune-steps", type=int, default=6)
@@ -228,6 +233,9 @@ def main():
     K.init(0)
     if args.workload == "batch":
         bench_batch(args, K)
+        return
+    if args.workload == "skewed":
+        bench_skewed(args, K)
         return
     bench_rmat(args, K)
 
@@ -368,6 +376,127 @@ def bench_rmat(args, K):
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on R-MAT scale {args.cpu_scale}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.edge_factor, args.seed)
+    print(json.dumps(out), flush=True)
+
+
+def undirected_keys(src, dst):
+    """Sorted unique (min, max) pairs of a COO, packed into uint64."""
+    a = np.minimum(src, dst).astype(np.uint64)
+    b = np.maximum(src, dst).astype(np.uint64)
+    return np.unique((a << np.uint64(32)) | b)
+
+
+def bench_skewed(args, K):
+    """The skewed graph with planted cliques: Chung-Lu with 2^skew_scale
+    vertices, exponent 2.1, mean degree 16, largest expected degree 2^15, and
+    64 planted cliques of 20-200 vertices (deep k).  Every variant of the
+    reference's design space (3 orientations x COO+CSR / tiled x recount
+    all / affected), tiles of 4096 vertices, finds the k-truss at --probe-k
+    from scratch; their edge sets must agree with each other and with the
+    edges whose truss number is at least --probe-k.  Then the learned
+    per-phase pick decomposes the graph (the measured steps) beside one
+    decomposition by the default variant (bit-exact)."""
+    log(f"generating Chung-Lu 2^{args.skew_scale}, exponent 2.1, 64 cliques, seed {args.seed}")
+    n, s, d = K.generate_chung_lu(1 << args.skew_scale, 2.1, 16.0, max_weight=float(1 << 15),
+                                  num_cliques=64, clique_min=20, clique_max=200, seed=args.seed)
+    g = K.Graph(n, s, d, pairs=True)
+    log(f"graph: n={n} m={g.m} max degree={g.max_degree}")
+
+    sweep = []
+    ref = None
+    for v in K.all_variants():
+        v.tile_size = 4096
+        p = K.Plan(g, v)
+        K.device_synchronize()
+        t0 = time.perf_counter()
+        es, ed, st = p.ktruss(args.probe_k)
+        K.device_synchronize()
+        wall = time.perf_counter() - t0
+        keys = undirected_keys(es, ed)
+        if ref is None:
+            ref = keys
+        elif not np.array_equal(keys, ref):
+            raise SystemExit(f"{v.describe()}: the {args.probe_k}-truss differs from the first variant's")
+        sweep.append({"variant": v.describe(), "ms": round(1e3 * wall, 2),
+                      "device_ms": round(st["ms_total"], 2), "iterations": st["iterations"],
+                      "truss_edges": int(len(keys))})
+        log(f"{v.describe()}: {args.probe_k}-truss of {len(keys)} edges in {1e3 * wall:.1f} ms "
+            f"({st['iterations']} iterations)")
+        del p
+
+    import learn
+    model, configs = learn.load_model()
+    plan = K.Plan(g, K.default_variant())
+    plan.set_candidates(configs)
+    plan.set_pick(K.PICK_MODEL, model)
+    plan.set_timing(False)
+    for i in range(args.warmup):
+        st = plan.decompose()
+        log(f"warmup {i}: {st['ms_total']:.1f} ms device, k_max {st['k_max']}")
+    K.device_synchronize()
+    per_step = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = plan.decompose()
+        per_step.append(st["ms_total"])
+        log(f"step {i}: {st['ms_total']:.1f} ms device")
+    K.device_synchronize()
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+    tau = plan.truss_numbers()
+    if not np.array_equal(undirected_keys(s[tau >= args.probe_k], d[tau >= args.probe_k]), ref):
+        raise SystemExit(f"the edges of truss number >= {args.probe_k} are not the swept {args.probe_k}-truss")
+    ph = plan.phase_log()
+    names = [c.describe() for c in plan.candidates]
+    picks = {}
+    for c in ph[:, 2].astype(int):
+        picks[names[c]] = picks.get(names[c], 0) + 1
+    del plan
+
+    default = None
+    bit_exact = None
+    if not args.skip_default:
+        pd = K.Plan(g, K.default_variant())
+        K.device_synchronize()
+        t0 = time.perf_counter()
+        sd = pd.decompose()
+        K.device_synchronize()
+        wall = time.perf_counter() - t0
+        bit_exact = bool(np.array_equal(pd.truss_numbers(), tau))
+        default = {"ms_per_step": 1e3 * wall, "device_ms": sd["ms_total"], "edges_per_s": g.m / wall,
+                   "k_max": sd["k_max"], "iterations": sd["iterations"]}
+        log(f"default variant: {1e3 * wall:.1f} ms, k_max {sd['k_max']}, bit-exact {bit_exact}")
+        del pd
+    out = {
+        "metric": "full k-truss edges/s at 1 GPU, skewed graph with planted cliques "
+                  "(learned pick vs default variant), bit-exact",
+        "value": g.m / (ms_per_step * 1e-3),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"Chung-Lu 2^{args.skew_scale} vertices, exponent 2.1, mean degree 16, "
+                         f"largest weight 2^15, 64 planted cliques of 20-200 vertices, seed {args.seed}: "
+                         "full k-truss decomposition; every variant's k-truss at the probe k"),
+            "n": n,
+            "m": g.m,
+            "k_max": st["k_max"],
+            "iterations": st["iterations"],
+            "median_device_ms": statistics.median(per_step) if per_step else None,
+            "phase_picks": picks,
+            "default_variant": default,
+            "speedup_vs_default": (default["ms_per_step"] / ms_per_step) if default else None,
+            "bit_exact_vs_default": bit_exact,
+            "probe_k": args.probe_k,
+            "variant_sweep": sweep,
+        },
+    }
     print(json.dumps(out), flush=True)
 
 

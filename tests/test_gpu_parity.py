@@ -106,6 +106,28 @@ def test_rmat12_all_variants_agree():
         assert np.array_equal(tau, tau_o[: len(s)]), v.describe()
 
 
+def test_skewed_graph_with_cliques_all_variants_4096_tiles():
+    """The skewed-graph sweep's shape at a size the oracle finishes in
+    seconds: Chung-Lu, exponent 2.1, planted cliques of 20-60 vertices (deep
+    k: a clique of c vertices has truss number c, so the largest is 20 or
+    more), 4096-vertex tiles (4 per dimension); every variant and DELTA give
+    the oracle's truss numbers."""
+    n, s, d = K.generate_chung_lu(1 << 14, 2.1, 16.0, max_weight=2048.0, num_cliques=8,
+                                  clique_min=20, clique_max=60, seed=5)
+    ss, dd = K.symmetrize(s, d)
+    tau_o, _, _ = O.decompose(n, ss, dd)
+    assert tau_o.max() >= 20
+    g = K.Graph(n, s, d, pairs=True)
+    variants = K.all_variants()
+    for v in variants:
+        v.tile_size = 4096
+    variants.append(K.make_variant(count_kernel="delta"))
+    for v in variants:
+        p = K.Plan(g, v)
+        p.decompose()
+        assert np.array_equal(p.truss_numbers(), tau_o[: len(s)]), v.describe()
+
+
 def _golden(scale):
     import hashlib
     import json
