@@ -248,17 +248,18 @@ __global__ void k_phase_start(unsigned int* from, DevCounters* ctr, int zero_nde
 // go through LDS to be scanned in deletion order.  (With each thread taking
 // kPerD consecutive deletions under a lane condition, the compiler waits for
 // each one's loads before the next: 3 x kPerD round trips per tile.)
+// lm.k: the pending list came from an update and is flagged here, its words
+// loaded beside the task list's; lm.fin_dl: finalized by the whole grid first.
 constexpr uint32_t kTileD = 2048;
 constexpr uint32_t kPerD = kTileD / kBlock;  // deletions per thread of a tile
 
 __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
-                                                  const uint32_t* __restrict__ src,
-                                                  const uint32_t* __restrict__ dst,
+                                                  const uint32_t* __restrict__ src, const uint32_t* dst,
                                                   const uint32_t* __restrict__ sptr,
                                                   uint4* __restrict__ meta, uint32_t* __restrict__ lofs,
                                                   uint32_t* tsum, DevCounters* ctr, unsigned int* zero_cnt,
-                                                  uint32_t* host_cnt) {
+                                                  uint32_t* host_cnt, dev::ListMark lm) {
     __shared__ uint32_t s_w[kBlock / kWave];
     __shared__ uint32_t s_c[kTileD];
     __shared__ uint32_t s_last;
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
         if (zero_cnt) *zero_cnt = 0;
         if (host_cnt) __hip_atomic_store(host_cnt, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    if (lm.fin_dl) finalize_list(lm.fin_dl, lm.fin_nd, blockIdx.x, gridDim.x, src, lm.dst, lm.snbr, lm.sa, lm.sb);
     const uint32_t ntiles = (nd + kTileD - 1) / kTileD;
     const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     if (ntiles == 0) {
@@ -291,6 +293,27 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
         for (uint32_t r = 0; r < kPerD; ++r) {
             u[r] = src[d[r]];
             dv[r] = dst[d[r]];
+        }
+        if (lm.k) {
+            uint32_t e[kPerD], a[kPerD], b[kPerD];
+#pragma unroll
+            for (uint32_t r = 0; r < kPerD; ++r) {
+                e[r] = lm.eid[d[r]];
+                a[r] = lm.sa[d[r]];
+                b[r] = lm.sb[d[r]];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kPerD; ++r) {
+                if (base + r * kBlock < nd) {
+                    const uint32_t v = dv[r] & KT_IDMASK;
+                    lm.dst[d[r]] = v | KT_DYING;
+                    lm.sup[d[r]] = KT_SUP_GONE;
+                    lm.tau[e[r]] = lm.k - 1;
+                    lm.snbr[a[r]] = v | KT_DYING;
+                    lm.snbr[b[r]] = u[r] | KT_DYING;
+                    if (lm.ldeg) atomicSub(lm.ldeg + u[r], 1u);
+                }
+            }
         }
         uint32_t ub[kPerD], ue[kPerD], vb[kPerD], ve[kPerD];
 #pragma unroll
@@ -413,7 +436,13 @@ __device__ __forceinline__ uint32_t first_task(const uint32_t* lofs, const uint3
 // the ownership test reads no flags of the oriented layout: a triangle whose
 // other two edges survive (the common case) costs two slot->entry reads and
 // two atomic decrements.
+// kCross: the decrements return the old support and the entries taken from
+// thr = k - 2 to k - 3 are listed in `next`, first in the wave's LDS list
+// (written out with one reservation when it fills), the rest with one
+// reservation per workgroup at the end; a task's returned supports are
+// examined during the next task, after its loads are issued.
 constexpr uint32_t kDBlock = 256;  // k_delta workgroup (512 and 1024 measured the same)
+constexpr uint32_t kCrossCap = 256;  // per-wave LDS list of the next deletions
 constexpr uint32_t kStage = 512;
 constexpr uint32_t kDeltaGrid = 2048;  // most k_delta workgroups per launch
 constexpr size_t kStatBytes = (size_t)kDeltaGrid * 4 * 8;
@@ -511,6 +540,7 @@ __device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t 
     return q;
 }
 
+template <bool kCross>
 __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
@@ -519,11 +549,38 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                                                   const uint32_t* __restrict__ snbr,
                                                   const uint32_t* __restrict__ seid,
                                                   uint32_t* sup, DevCounters* ctr,
-                                                  unsigned long long* __restrict__ bstat) {
+                                                  unsigned long long* __restrict__ bstat, uint32_t thr,
+                                                  uint32_t* __restrict__ next, unsigned int* next_cnt) {
     __shared__ uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample)];
+    __shared__ uint32_t s_cross[kCross ? (kDBlock / kWave) * kCrossCap : 1];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample);
     uint32_t* smp = buf + kStage;
+    uint32_t* cl = s_cross + (kCross ? (threadIdx.x / kWave) * kCrossCap : 0u);
+    uint32_t nc = 0;  // entries in cl (wave-uniform)
+    // the last task's decremented entries and their old supports (0: none;
+    // thr >= 2)
+    uint32_t pe1 = 0, pe2 = 0, po1 = 0, po2 = 0;
+    auto cross = [&]() {
+        const uint64_t below = (1ull << lane) - 1ull;
+        const bool c1 = po1 == thr, c2 = po2 == thr;
+        po1 = po2 = 0;
+        const uint64_t m1 = ballot(c1), m2 = ballot(c2);
+        if (!(m1 | m2)) return;
+        if (c1) cl[nc + (uint32_t)__popcll(m1 & below)] = pe1;
+        nc += (uint32_t)__popcll(m1);
+        if (c2) cl[nc + (uint32_t)__popcll(m2 & below)] = pe2;
+        nc += (uint32_t)__popcll(m2);
+        if (nc > kCrossCap - 2 * kWave) {  // room for one more task's entries
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(next_cnt, nc);
+            at = __shfl(at, 0);
+            wave_lds_sync();
+            for (uint32_t q = lane; q < nc; q += kWave) next[at + q] = cl[q];
+            wave_lds_sync();
+            nc = 0;
+        }
+    };
     const uint32_t nd = *nd_dev;
     const uint32_t T = ctr->dtasks;
     const uint32_t W = nwaves(), w = gwave();
@@ -588,6 +645,7 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
         const uint32_t x = pre ? xpre : (j < m.y ? snbr[j] : KT_DEAD);
         pre = t < t1 && t < fnext;
         if (pre) xpre = j + kWave < m.y ? snbr[j + kWave] : KT_DEAD;
+        if (kCross) cross();
         const bool cand = !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
         acc_live += (uint32_t)__popcll(cm);
@@ -646,12 +704,37 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                 own = !((y1 && e1 < d) || (y2 && e2 < d));
             }
             if (own) {
-                if (!y1) atomicSub(sup + e1, 1u);
-                if (!y2) atomicSub(sup + e2, 1u);
+                if (kCross) {
+                    pe1 = e1;
+                    pe2 = e2;
+                    if (!y1) po1 = atomicSub(sup + e1, 1u);
+                    if (!y2) po2 = atomicSub(sup + e2, 1u);
+                } else {
+                    if (!y1) atomicSub(sup + e1, 1u);
+                    if (!y2) atomicSub(sup + e2, 1u);
+                }
                 tri = true;
             }
         }
         acc_tri += (uint32_t)__popcll(ballot(tri));
+    }
+    if (kCross) {  // the last task's entries, then the waves' remaining ones:
+                   // one reservation per workgroup
+        cross();
+        __shared__ uint32_t s_nc[kDBlock / kWave];
+        __shared__ uint32_t s_at;
+        const uint32_t wid = threadIdx.x / kWave;
+        if (lane == 0) s_nc[wid] = nc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t q = 0; q < kDBlock / kWave; ++q) tot += s_nc[q];
+            s_at = tot ? atomicAdd(next_cnt, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t at = s_at;
+        for (uint32_t q = 0; q < wid; ++q) at += s_nc[q];
+        for (uint32_t q = lane; q < nc; q += kWave) next[at + q] = cl[q];
     }
     // statistics: one slot per workgroup, updated in place (a launch has at
     // most kDeltaGrid workgroups and launches on one stream do not overlap);
@@ -878,26 +961,39 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
-                         unsigned int* zero_cnt, uint32_t* host_cnt) {
-    if (!nd_max && !zero_cnt && !host_cnt) return hipSuccess;
+                         unsigned int* zero_cnt, uint32_t* host_cnt, const ListMark* lm) {
+    if (!nd_max && !zero_cnt && !host_cnt && !lm) return hipSuccess;
+    ListMark m{};
+    if (lm) {
+        m = *lm;
+        if ((m.k && (m.k < 3 || !m.dst || !m.eid || !m.sup || !m.tau || !m.snbr || !m.sa || !m.sb)) ||
+            (m.fin_dl && (!m.dst || !m.fin_nd || !m.snbr || !m.sa || !m.sb)))
+            return hipErrorInvalidValue;
+    }
     const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
     const uint32_t tb = tiles < 256 ? (tiles ? tiles : 1u) : 256u;
     hipLaunchKernelGGL(k_dprep, dim3(tb), dim3(kBlock), 0, s, dl, nd_dev, src, dst, sptr, ws.meta,
-                       ws.ntask, ws.tsum, ctr, zero_cnt, host_cnt);
+                       ws.ntask, ws.tsum, ctr, zero_cnt, host_cnt, m);
     return hipGetLastError();
 }
 
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
-                       const DeltaScratch& ws, DevCounters* ctr, hipStream_t s) {
+                       const DeltaScratch& ws, DevCounters* ctr, hipStream_t s, uint32_t thr,
+                       uint32_t* next, unsigned int* next_cnt) {
     if (!nd_max) return hipSuccess;
+    if (next && (!next_cnt || thr < 2)) return hipErrorInvalidValue;
     // four waves per deletion, 256 to 8192 waves
     const uint32_t wpb = kDBlock / kWave;
     const uint64_t want = (uint64_t)nd_max * 4;
     const uint32_t w = want < 256 ? 256u : (want > 8192 ? 8192u : (uint32_t)want);
     const uint32_t db = (w + wpb - 1) / wpb;
-    hipLaunchKernelGGL(k_delta, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask, ws.tsum, snbr,
-                       seid, sup, ctr, ws.bstat);
+    if (next)
+        hipLaunchKernelGGL(k_delta<true>, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask,
+                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, thr, next, next_cnt);
+    else
+        hipLaunchKernelGGL(k_delta<false>, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask,
+                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, 0u, nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -69,6 +69,8 @@ struct DevCounters {
     unsigned int ndel2;               // second deletion count (DELTA iterations run ahead)
     unsigned int void_del;            // deletions of void DELTA iterations (stays 0)
     unsigned int mark_bad;            // entries the DELTA mark found deleted already (stays 0)
+    unsigned int ndel3;               // third deletion count (DELTA iterations run ahead)
+    unsigned int pad0;
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
     unsigned long long delta_win;     // longer-list window entries read by DELTA updates
@@ -174,20 +176,50 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
 // nd_dev: the number of pending deletions, read on the device (dl[0, *nd_dev));
 // nd_max bounds it and sizes the grids.  delta_prepare lists the tasks (one
 // kernel), delta_apply is the update proper (k_delta).
-// zero_cnt (optional): a counter set to 0 by the same kernel (the next mark's);
-// host_cnt (optional, device address of pinned host memory): receives *nd_dev.
+// zero_cnt (optional): a counter set to 0 by the same kernel (the next list's);
+// host_cnt (optional, device address of pinned host memory): receives *nd_dev;
+// lm (optional): list work fused into the same kernel, see ListMark.
+struct ListMark;
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
-                         unsigned int* zero_cnt = nullptr, uint32_t* host_cnt = nullptr);
+                         unsigned int* zero_cnt = nullptr, uint32_t* host_cnt = nullptr,
+                         const ListMark* lm = nullptr);
+// Work the task-list kernel can do besides the task list: flag the pending
+// list when an update made it (its entries' supports fell below k - 2: KT_DYING
+// on the entry and both adjacency slots, KT_SUP_GONE, truss number k - 1, live
+// row lengths), and finalize the list before it (KT_DEAD), whose update is
+// done.  A live entry's slots hold plain ids (sa[e], in src[e]'s list, holds
+// dst[e]; sb[e] holds src[e]), so flags are stored, not read-modified-written.
+struct ListMark {
+    uint32_t* dst;             // the oriented layout (the same array as delta_prepare's dst)
+    const uint32_t* eid;
+    uint32_t* sup;
+    uint32_t* tau;
+    uint32_t* snbr;
+    const uint32_t* sa;
+    const uint32_t* sb;
+    uint32_t* ldeg;            // optional: live row lengths
+    uint32_t k;                // 0: the pending list is flagged already
+    const uint32_t* fin_dl;    // optional: the list to finalize
+    const unsigned int* fin_nd;
+};
 // The update's statistics are summed per workgroup slot; delta_stats_fold
 // adds them to ctr->delta_* and clears the slots, delta_stats_reset clears
 // them (neither may be captured in a graph that replays before stats are read).
 hipError_t delta_stats_reset(const DeltaScratch& ws, hipStream_t s);
 hipError_t delta_stats_fold(const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+// next (optional): the entries whose support the update takes from thr = k - 2
+// to k - 3 are appended to next, counted in *next_cnt (zero on entry).  Every
+// live entry is at k - 2 or above before the update (the ones below are the
+// pending deletions) and supports only fall, so these are exactly the next
+// mark pass's deletions, each listed once.  The decrements then return the old
+// support: under contention on hub edges' supports that costs the update
+// time, so it pays where the update is light.
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
-                       const DeltaScratch& ws, DevCounters* ctr, hipStream_t s);
+                       const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
+                       uint32_t thr = 0, uint32_t* next = nullptr, unsigned int* next_cnt = nullptr);
 hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                             uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                             hipStream_t s);

@@ -662,7 +662,7 @@ int Plan::ensure_delta() {
             return rc;
     if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
         (rc = sa0.alloc(Eb)) || (rc = sb0.alloc(Eb)) || (rc = sa.alloc(Eb)) || (rc = sb.alloc(Eb)) ||
-        (rc = dl.alloc(Eb)) || (rc = dl_alt.alloc(Eb)) || (rc = ldeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
+        (rc = dl.alloc(Eb)) || (rc = dl_alt.alloc(Eb)) || (rc = dl3.alloc(Eb)) || (rc = ldeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
         (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))) ||
         (rc = ctsum.alloc((size_t)ct * 4)) || (rc = cxs.alloc(2 * Eb + 4)))
         return rc;
@@ -841,34 +841,52 @@ int Plan::take_void(bool zero_ndel, bool reset_min) {
 }
 
 // The remaining iterations of a DELTA phase at k (k > 3), queued one ahead of
-// their read-backs: iteration j's update (of iteration j - 1's deletions,
-// their number read on the device, grids sized by an upper bound) and mark
-// are queued before the deletion count of iteration j - 1 comes back through
-// pinned memory, so the device does not wait for the host between
-// iterations.  The iteration queued after the one that deletes nothing only
-// finishes the last update (no support changes, so its mark deletes nothing);
-// it is not an iteration of the peel and is not recorded, and its count is
-// checked at the next read-back (take_void) instead of waiting for it now.
-// Returns with `ended` when the phase is over (nothing pending), else with
-// the last iteration's deletions pending and `iter` the next index, for a
+// their read-backs.  Every live entry has support k - 2 or more before an
+// update (the smaller ones are the pending deletions), so the next deletions
+// are the entries whose support the update takes from k - 2 to k - 3.  An
+// iteration is heavy or light, chosen from the last two counts read back:
+//   heavy: task list, update, mark (a pass over the stored supports that
+//     flags and lists the next deletions and finalizes the ones just
+//     updated);
+//   light: task list (which also flags the pending list and finalizes the one
+//     before it when a light update made them), update (whose decrements
+//     return the old support and list the entries that cross, so no pass over
+//     the stored supports is needed).  Returning decrements wait for their
+//     turn at L2 under contention on hub edges' supports, so heavy updates
+//     keep the plain ones.
+// The task list also clears the count of the list the iteration fills and
+// writes the count of its own list into pinned host memory with a
+// system-scope store.  Iteration j's kernels are queued before iteration
+// j - 1's count comes back, so the device does not wait for the host
+// between iterations; grids are sized from upper bounds and kernels read the
+// true counts on the device.  The iteration queued after the one that
+// deletes nothing only finishes the last update and finalization (its list
+// is empty); it is not an iteration of the peel and is not recorded, and its
+// count is checked at the next read-back (take_void) instead of waiting for
+// it now.  Returns with `ended` when the phase is over (nothing pending,
+// every deletion finalized), else with the last iteration's deletions
+// flagged and pending in dl / ctr->ndel and `iter` the next index, for a
 // compaction.  On entry `pending` holds the deletions of iteration iter - 1
-// (> 0), or, with `head`, iteration iter's mark (no update before it) is
-// queued into dl / ctr->ndel and its count is unread.
+// (> 0, flagged by the mark pass), or, with `head`, iteration iter's mark
+// pass is queued into dl / ctr->ndel and its count is unread.
 int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended, bool head) {
     DevCounters* c = ctr.as<DevCounters>();
-    // deletion lists and counts alternate: iteration r (counted from entry)
-    // updates from list r & 1 and marks into list (r + 1) & 1; the pending
+    // three deletion lists rotate: iteration r (counted from entry) updates
+    // list r % 3 and fills list (r + 1) % 3; list (r + 2) % 3, updated by the
+    // iteration before, is the one a task list may finalize; the pending
     // deletions on entry are in dl / ctr->ndel (list 0)
-    uint32_t* lists[2] = {dl.as<uint32_t>(), dl_alt.as<uint32_t>()};
-    unsigned int* counts[2] = {&c->ndel, &c->ndel2};
+    DBuf* bufs[3] = {&dl, &dl_alt, &dl3};
+    uint32_t* lists[3] = {dl.as<uint32_t>(), dl_alt.as<uint32_t>(), dl3.as<uint32_t>()};
+    unsigned int* counts[3] = {&c->ndel, &c->ndel2, &c->ndel3};
     // deletions the next update consumes, at most
     uint32_t bound = head ? (uint32_t)std::min<uint64_t>(live, E) : pending;
     uint32_t last_nd = bound;  // deletions of the last recorded iteration
+    uint32_t prev_nd = bound;  // and of the one before
     // queued iterations whose counts are unread, oldest first
     struct Queued {
         int slot;
         uint32_t it;
-        bool updated;  // a DELTA update precedes its mark
+        bool updated;  // an update produced its deletions
     };
     std::deque<Queued> open;
     ended = false;
@@ -879,27 +897,52 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         __atomic_store_n(h_ra + prev_slot, kCountPending, __ATOMIC_RELEASE);
         open.push_back({prev_slot, iter, false});
     }
-    // iteration r: task list (which also clears the mark's count and writes
-    // the previous iteration's deletion count to pinned memory), update, mark
-    // (which also finalizes the deletions just updated)
-    auto queue = [&](int s) -> int {
-        const uint32_t in = r & 1u, out = in ^ 1u;
-        __atomic_store_n(h_ra + s, kCountPending, __ATOMIC_RELEASE);  // the successor overwrites it
-        // the task list's grid follows the last count read back (it strides over
-        // the true count); the update's follows the upper bound
-        KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], std::max<uint32_t>(last_nd, 1u),
+    // the last queued iteration was light: the list it filled is not flagged
+    // and the list it updated is not finalized
+    bool unflagged = false;
+    dev::ListMark lm{};
+    lm.dst = dst[cur].as<uint32_t>();
+    lm.eid = eid[cur].as<uint32_t>();
+    lm.sup = sup[cur].as<uint32_t>();
+    lm.tau = tau.as<uint32_t>();
+    lm.snbr = snbr_b[acur].as<uint32_t>();
+    lm.sa = sa.as<uint32_t>();
+    lm.sb = sb.as<uint32_t>();
+    lm.ldeg = live_rows_ptr();
+    // task list of list rr % 3 (its grid follows `hint`: it strides over the
+    // true counts), with the flagging and finalization a light iteration left
+    auto prepare = [&](uint32_t rr, uint32_t hint, unsigned int* zero_cnt, uint32_t* host_cnt) -> int {
+        const uint32_t in = rr % 3, prev = (rr + 2) % 3;
+        dev::ListMark m = lm;
+        m.k = unflagged ? k : 0u;
+        m.fin_dl = unflagged ? lists[prev] : nullptr;
+        m.fin_nd = counts[prev];
+        KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], std::max<uint32_t>(hint, 1u),
                                       src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                      sptr_b[acur].as<uint32_t>(), dws, c, stream, counts[out],
-                                      prev_slot >= 0 ? d_ra + prev_slot : nullptr));
+                                      sptr_b[acur].as<uint32_t>(), dws, c, stream, zero_cnt, host_cnt, &m));
+        return KT_OK;
+    };
+    // iteration r; the task list's hint is the last count read back (the size
+    // of the list it may finalize), the update's grid follows the upper bound
+    auto queue = [&](int s) -> int {
+        const uint32_t in = r % 3, out = (r + 1) % 3;
+        const bool light = std::max(last_nd, prev_nd) <= kLightDeletions;
+        __atomic_store_n(h_ra + s, kCountPending, __ATOMIC_RELEASE);  // the successor overwrites it
+        int rc = prepare(r, last_nd, counts[out], prev_slot >= 0 ? d_ra + prev_slot : nullptr);
+        if (rc) return rc;
         if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
         KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
-                                    seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream));
+                                    seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream,
+                                    light ? k - 2 : 0u, light ? lists[out] : nullptr,
+                                    light ? counts[out] : nullptr));
         if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
-        KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
-                                   tau.as<uint32_t>(), lists[out], E, k, snbr_b[acur].as<uint32_t>(),
-                                   sa.as<uint32_t>(), sb.as<uint32_t>(), counts[out], lists[in], counts[in],
-                                   bound, src[cur].as<uint32_t>(), live_rows_ptr(),
-                                   &ctr.as<DevCounters>()->mark_bad, stream));
+        if (!light)
+            KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
+                                       sup[cur].as<uint32_t>(), tau.as<uint32_t>(), lists[out], E, k,
+                                       snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
+                                       counts[out], lists[in], counts[in], bound, src[cur].as<uint32_t>(),
+                                       live_rows_ptr(), &c->mark_bad, stream));
+        unflagged = light;
         prev_slot = s;
         ++r;
         return KT_OK;
@@ -923,7 +966,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
                 std::this_thread::yield();
             }
         } else {
-            KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[r & 1u], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[r % 3], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
             KT_HIP_TRY(hipEventSynchronize(ev_ra[s][2]));
         }
@@ -949,12 +992,14 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
             stats.delta_edges += last_nd;  // its update consumed the previous iteration's deletions
             last_pass_delta = true;
         }
+        prev_nd = last_nd;
         last_nd = nd;
         iter = q.it + 1;
         return record_iteration(k, q.it, nd, 0, ms);
     };
     // the phase ended at the iteration just read: the younger queued ones are
-    // void; the youngest's count is left for the next read-back to check
+    // void; the youngest's count is left for the next read-back to check.
+    // Without a younger one, the list a light youngest updated is finalized.
     auto drain_void = [&]() -> int {
         for (size_t i = 0; i + 1 < open.size(); ++i) {
             uint32_t nd = 0;
@@ -962,7 +1007,14 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
             if (rc) return rc;
             if (nd) return fail(KT_ERR_STATE, "void DELTA iteration deleted entries");
         }
-        if (!open.empty()) void_src = counts[r & 1u];
+        if (!open.empty()) {
+            void_src = counts[r % 3];
+        } else if (unflagged) {
+            const uint32_t upd = (r + 2) % 3;
+            KT_HIP_TRY(dev::finalize_deleted(lists[upd], counts[upd], std::max<uint32_t>(prev_nd, 1u),
+                                             dst[cur].as<uint32_t>(), snbr_b[acur].as<uint32_t>(),
+                                             sa.as<uint32_t>(), sb.as<uint32_t>(), stream));
+        }
         open.clear();
         pending = 0;
         ended = true;
@@ -981,9 +1033,14 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
             if ((rc = finish(nd))) return rc;
             if (nd == 0) return drain_void();
         }
-        if (leave) {  // everything read back: the compaction follows in the caller
+        if (leave) {
+            // everything read back: after a light youngest its list is flagged
+            // and the one before finalized (a task list without an update);
+            // the pending list becomes dl / ctr->ndel and the compaction
+            // follows in the caller
+            if (unflagged && (rc = prepare(r, std::max(prev_nd, last_nd), nullptr, nullptr))) return rc;
             pending = last_nd;
-            if (r & 1u) std::swap(dl.p, dl_alt.p);  // the pending list becomes dl
+            std::swap(dl.p, bufs[r % 3]->p);
             return set_counter(offsetof(DevCounters, ndel), pending);
         }
     }

@@ -58,6 +58,9 @@ struct Plan {
     // the pinned deletion count
     static constexpr uint32_t kAhead = 1, kAheadSlots = kAhead + 2;
     static constexpr uint32_t kCountPending = 0xFFFFFFFFu;  // h_ra slot not written yet
+    // a run-ahead iteration is light (its update lists the next deletions, no
+    // mark pass) while the last two counts read back are at most this
+    static constexpr uint32_t kLightDeletions = 64;
     hipEvent_t ev_ra[kAheadSlots][3] = {};
     uint32_t* h_ra = nullptr;
     uint32_t* d_ra = nullptr;  // h_ra's device address
@@ -94,7 +97,7 @@ struct Plan {
     DBuf sptr0, snbr0, seid0, sa0, sb0;
     int acur = 0;
     uint32_t S_cur = 0;
-    DBuf dl, dl_alt, sup_snap, dscratch, ctsum, cxs;  // dl_alt: second deletion list (run-ahead)
+    DBuf dl, dl_alt, dl3, sup_snap, dscratch, ctsum, cxs;  // dl_alt, dl3: run-ahead deletion lists
     DBuf ldeg;  // live length of every oriented row (phase features of DELTA plans)
     // the rows' live lengths are kept only when phases are picked (features)
     uint32_t* live_rows_ptr() const { return pick == KT_PICK_FIXED ? nullptr : ldeg.as<uint32_t>(); }
