@@ -191,8 +191,10 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
 }
 
 // Minimum live support (the JUMP target); one atomic per workgroup.
-__global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCounters* ctr) {
+__global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCounters* ctr,
+                           uint32_t* host_min) {
     __shared__ uint32_t s_min[kBlock / kWave];
+    __shared__ uint32_t s_last;
     uint32_t m = 0xFFFFFFFFu;
     // eight supports per thread in flight (indices past the end read entry
     // n - 1, which is then not counted); dead and dying entries hold
@@ -220,6 +222,21 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCoun
     if (threadIdx.x == 0) {
         for (uint32_t w = 1; w < kBlock / kWave; ++w) m = s_min[w] < m ? s_min[w] : m;
         if (m != 0xFFFFFFFFu) atomicMin(&ctr->min_sup, m);
+    }
+    if (!host_min) return;  // uniform
+    // the last workgroup to finish (a ticket taken after a device-scope
+    // fence) publishes the minimum
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&ctr->mticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t v = __hip_atomic_load(&ctr->min_sup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ctr->mticket = 0;
+        __hip_atomic_store(host_min, v == 0xFFFFFFFFu ? dev::kNoLiveSupport : v, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -935,11 +952,12 @@ hipError_t phase_start(unsigned int* from, DevCounters* ctr, bool zero_ndel, boo
     return hipGetLastError();
 }
 
-hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s) {
-    if (!n) return hipSuccess;
+hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s,
+                          uint32_t* host_min) {
+    if (!n) return host_min ? hipErrorInvalidValue : hipSuccess;
     const uint32_t b = blocks_for(n, kBlock);
     // at most 256 workgroups: one atomic each on the one counter
-    hipLaunchKernelGGL(k_min_list, dim3(b < 256 ? b : 256), dim3(kBlock), 0, s, n, sup, ctr);
+    hipLaunchKernelGGL(k_min_list, dim3(b < 256 ? b : 256), dim3(kBlock), 0, s, n, sup, ctr, host_min);
     return hipGetLastError();
 }
 

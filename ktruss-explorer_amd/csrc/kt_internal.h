@@ -70,7 +70,7 @@ struct DevCounters {
     unsigned int void_del;            // deletions of void DELTA iterations (stays 0)
     unsigned int mark_bad;            // entries the DELTA mark found deleted already (stays 0)
     unsigned int ndel3;               // third deletion count (DELTA iterations run ahead)
-    unsigned int pad0;
+    unsigned int mticket;             // JUMP minimum: workgroups done (reset by the last)
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
     unsigned long long delta_win;     // longer-list window entries read by DELTA updates
@@ -156,13 +156,17 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
 // out[v] = ptr[v + 1] - ptr[v] for v < nv.
 hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying: the
-// others' supports are KT_SUP_GONE).
+// others' supports are KT_SUP_GONE).  host_min (optional, device address of
+// pinned host memory): the last workgroup to finish stores the minimum there
+// (kNoLiveSupport when nothing is live) with a system-scope store.
+constexpr uint32_t kNoLiveSupport = 0xFFFFFFFEu;
 // One thread at the start of a DELTA phase: ctr->void_del |= *from; *from = 0
 // (from: a deletion count checked at the next counter read-back; optional),
 // then ctr->ndel = 0 (zero_ndel) and ctr->min_sup = ~0 (reset_min).
 hipError_t phase_start(unsigned int* from, DevCounters* ctr, bool zero_ndel, bool reset_min,
                        hipStream_t s);
-hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s);
+hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s,
+                          uint32_t* host_min = nullptr);
 // Scratch of a DELTA update for up to `cap` pending deletions.
 struct DeltaScratch {
     unsigned long long* bstat;  // per k_delta workgroup: elems, win, live, tris

@@ -20,7 +20,8 @@ uint32_t mlp_num_params(uint32_t n_out);
 hipError_t phase_features(const uint32_t* ptr, const uint32_t* rowlen, const uint32_t* dst,
                           const uint32_t* sup, uint32_t n, uint32_t E, uint32_t E0, uint32_t k,
                           bool sup_valid, unsigned long long* acc, float* feat, const float* P,
-                          uint32_t C, const float* R, unsigned int* choice, hipStream_t s);
+                          uint32_t C, const float* R, unsigned int* choice, hipStream_t s,
+                          uint32_t* host_choice = nullptr);
 // R: the input range, lo[KT_MLP_IN] then hi[KT_MLP_IN]; inputs are clamped to it.
 hipError_t mlp_forward(const float* P, uint32_t C, const float* R, const float* X, uint32_t batch,
                        float* Y, hipStream_t s);

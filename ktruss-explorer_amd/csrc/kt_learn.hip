@@ -267,7 +267,8 @@ __global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float
 __global__ __launch_bounds__(64) void k_feat_finish(const unsigned long long* part, uint32_t E0,
                                                     uint32_t stride, uint32_t k, int sup_valid,
                                                     float* feat, const float* __restrict__ P, int C,
-                                                    const float* __restrict__ R, unsigned int* choice) {
+                                                    const float* __restrict__ R, unsigned int* choice,
+                                                    uint32_t* host_choice) {
     __shared__ unsigned long long acc[KT_FEAT_ACC];
     __shared__ float s_x[kIn], s_h1[kHid], s_h2[kHid], s_o[KT_MLP_MAX_OUT];
     const int lane = threadIdx.x;
@@ -322,6 +323,7 @@ __global__ __launch_bounds__(64) void k_feat_finish(const unsigned long long* pa
         for (int c = 1; c < C; ++c)
             if (s_o[c] < s_o[best]) best = c;
         *choice = (unsigned int)best;
+        if (host_choice) __hip_atomic_store(host_choice, (uint32_t)best, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -334,14 +336,15 @@ uint32_t mlp_num_params(uint32_t C) { return (uint32_t)layout((int)C).total; }
 hipError_t phase_features(const uint32_t* ptr, const uint32_t* rowlen, const uint32_t* dst,
                           const uint32_t* sup, uint32_t n, uint32_t E, uint32_t E0, uint32_t k,
                           bool sup_valid, unsigned long long* acc, float* feat, const float* P,
-                          uint32_t C, const float* R, unsigned int* choice, hipStream_t s) {
+                          uint32_t C, const float* R, unsigned int* choice, hipStream_t s,
+                          uint32_t* host_choice) {
     const uint32_t stride = E > kFeatSample ? (E + kFeatSample - 1) / kFeatSample : 1u;
     hipLaunchKernelGGL(k_feat_scan, dim3(kFeatBlocks), dim3(kFeatThreads), 0, s, ptr, rowlen, n, dst, sup, E,
                        stride, k, sup_valid ? 1 : 0, acc);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_feat_finish, dim3(1), dim3(64), 0, s, acc, E0, stride, k, sup_valid ? 1 : 0,
-                       feat, P, (int)C, R, choice);
+                       feat, P, (int)C, R, choice, host_choice);
     return hipGetLastError();
 }
 

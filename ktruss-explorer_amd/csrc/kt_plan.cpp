@@ -151,7 +151,7 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     for (auto& slot : ev_ra)
         for (auto& ev : slot) KT_HIP_TRY(hipEventCreate(&ev));
     // fine-grained: a kernel's system-scope store lands in host memory
-    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, kAheadSlots * sizeof(uint32_t),
+    KT_HIP_TRY(hipHostMalloc((void**)&h_ra, (kAheadSlots + 2) * sizeof(uint32_t),
                              hipHostMallocMapped | hipHostMallocCoherent));
     KT_HIP_TRY(hipHostGetDevicePointer((void**)&d_ra, h_ra, 0));
 
@@ -339,7 +339,8 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
 
 // Phase features of the surviving graph (supports used only when `exact`)
 // and, in MODEL mode, the cost model's pick among the candidates.
-int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer, bool live_rows) {
+int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer, bool live_rows,
+                     uint32_t* host_choice) {
     cfg = fixed_cfg;
     chosen = 0;
     phase_has_feat = false;
@@ -366,7 +367,7 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bo
                                    feat_acc.as<unsigned long long>(), feat_log.as<float>() + ph * KT_MLP_IN,
                                    by_model ? model.as<float>() : nullptr, model_C,
                                    by_model ? model.as<float>() + model_n : nullptr,
-                                   &ctr.as<DevCounters>()->choice, stream));
+                                   &ctr.as<DevCounters>()->choice, stream, by_model ? host_choice : nullptr));
     phase_has_feat = true;
     if (by_model && !defer) {
         const int rc = sync_counters();
@@ -833,6 +834,21 @@ int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t re
     return KT_OK;
 }
 
+int Plan::poll_pinned(uint32_t i, uint32_t& v) {
+    for (uint32_t spin = 1;; ++spin) {
+        if (__atomic_load_n(h_ra + i, __ATOMIC_ACQUIRE) != kCountPending) break;
+        if (spin % 256 == 0) {
+            const hipError_t q = hipStreamQuery(stream);
+            if (q == hipSuccess && __atomic_load_n(h_ra + i, __ATOMIC_ACQUIRE) == kCountPending)
+                return fail(KT_ERR_STATE, "a value the device was to store in pinned memory never arrived");
+            if (q != hipSuccess && q != hipErrorNotReady) KT_HIP_TRY(q);
+        }
+        std::this_thread::yield();
+    }
+    v = h_ra[i];
+    return KT_OK;
+}
+
 int Plan::take_void(bool zero_ndel, bool reset_min) {
     if (!void_src && !zero_ndel && !reset_min) return KT_OK;
     KT_HIP_TRY(dev::phase_start(void_src, ctr.as<DevCounters>(), zero_ndel, reset_min, stream));
@@ -954,17 +970,8 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         if (i + 1 < open.size()) {
             // the successor's task list stores the count into pinned memory:
             // wait for it there (an event between the kernels would cost the
-            // device a gap); a stream that went idle without it is an error
-            for (uint32_t spin = 1;; ++spin) {
-                if (__atomic_load_n(h_ra + s, __ATOMIC_ACQUIRE) != kCountPending) break;
-                if (spin % 256 == 0) {
-                    const hipError_t q = hipStreamQuery(stream);
-                    if (q == hipSuccess && __atomic_load_n(h_ra + s, __ATOMIC_ACQUIRE) == kCountPending)
-                        return fail(KT_ERR_STATE, "deletion count of a queued DELTA iteration never arrived");
-                    if (q != hipSuccess && q != hipErrorNotReady) KT_HIP_TRY(q);
-                }
-                std::this_thread::yield();
-            }
+            // device a gap)
+            return poll_pinned((uint32_t)s, nd);
         } else {
             KT_HIP_TRY(hipMemcpyAsync(h_ra + s, counts[r % 3], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             KT_HIP_TRY(hipEventRecord(ev_ra[s][2], stream));
@@ -1071,7 +1078,21 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     // a compaction that fell due in the last phase happens before the phase's
     // features are taken (they describe the stored rows)
     if ((rc = compact_delta_state())) return rc;
-    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen, true, true))) return rc;
+    // With nothing pending and no support pass to open the phase, the JUMP
+    // minimum is taken before the features and stored into pinned memory by
+    // its last workgroup, so the host queues the phase's first mark while the
+    // features and the model's pick run; the choice arrives the same way.  The
+    // phase then starts without a counter read-back.
+    const bool early = jump && start_clear && reuse && pick != KT_PICK_SWEEP && k > 3 &&
+                       (pick == KT_PICK_MODEL || fixed_cfg.kernel == KT_COUNT_DELTA);
+    if (early) {
+        __atomic_store_n(h_ra + kSlotMin, kCountPending, __ATOMIC_RELEASE);
+        __atomic_store_n(h_ra + kSlotChoice, kCountPending, __ATOMIC_RELEASE);
+        KT_HIP_TRY(dev::min_live_delta(E, sup[cur].as<uint32_t>(), ctr.as<DevCounters>(), stream,
+                                       d_ra + kSlotMin));
+    }
+    if ((rc = phase_pick(k, reuse && !pending, cfg, chosen, true, true, early ? d_ra + kSlotChoice : nullptr)))
+        return rc;
     auto resolve_choice = [&]() {
         chosen = std::min<uint32_t>(h_ctr->choice, nc - 1);
         cfg = cands[chosen];
@@ -1079,7 +1100,39 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     };
     std::vector<double> cand_ms(nc, 0.0);
     double phase_ms = 0.0;
-    do {
+    bool peel = true;  // the loop below runs
+    if (early) {
+        uint32_t mv = 0;
+        if ((rc = poll_pinned(kSlotMin, mv))) return rc;
+        if (mv != dev::kNoLiveSupport && mv + 3 > k) k = mv + 3;
+        KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
+                                   tau.as<uint32_t>(), dl.as<uint32_t>(), E, k, snbr_b[acur].as<uint32_t>(),
+                                   sa.as<uint32_t>(), sb.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, nullptr,
+                                   nullptr, 0, src[cur].as<uint32_t>(), live_rows_ptr(),
+                                   &ctr.as<DevCounters>()->mark_bad, stream));
+        if (unresolved) {
+            uint32_t ch = 0;
+            if ((rc = poll_pinned(kSlotChoice, ch))) return rc;
+            chosen = std::min<uint32_t>(ch, nc - 1);
+            cfg = cands[chosen];
+            unresolved = false;
+        }
+        if (cfg.kernel == KT_COUNT_DELTA && !compaction_due()) {
+            // the first mark heads the run-ahead queue
+            bool ended = false;
+            if ((rc = run_ahead_delta(k, iter, phase_ms, ended, true))) return rc;
+            changed = !ended;
+        } else {  // the first mark's count, read back as the loop does
+            if ((rc = sync_counters())) return rc;
+            const uint32_t ndel = h_ctr->ndel;
+            pending = ndel;
+            if ((rc = record_iteration(k, iter, ndel, 0, 0.f))) return rc;
+            changed = ndel > 0;
+            ++iter;
+        }
+        peel = changed;
+    }
+    if (peel) do {
         bool counted = false;
         const uint64_t rec_before = h_ctr->recount_edges;
         float dummy = 0;

@@ -62,8 +62,14 @@ struct Plan {
     // mark pass) while the last two counts read back are at most this
     static constexpr uint32_t kLightDeletions = 64;
     hipEvent_t ev_ra[kAheadSlots][3] = {};
+    // pinned words: kAheadSlots deletion counts, then the JUMP minimum and the
+    // model's choice of a phase start (written by the kernels that make them)
+    static constexpr uint32_t kSlotMin = kAheadSlots, kSlotChoice = kAheadSlots + 1;
     uint32_t* h_ra = nullptr;
     uint32_t* d_ra = nullptr;  // h_ra's device address
+    // waits for pinned word h_ra[i] to leave kCountPending; a stream that went
+    // idle without it is an error
+    int poll_pinned(uint32_t i, uint32_t& v);
 
     std::vector<std::array<uint64_t, 5>> trace;
     kt_stats stats{};
@@ -138,8 +144,9 @@ struct Plan {
     // defer: leave the model's choice on the device (read with the next
     // counter read-back) instead of synchronising for it now
     // live_rows: row features from the live row lengths (DELTA phases keep them)
+    // host_choice: device address of pinned memory the choice is also stored to
     int phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer = false,
-                   bool live_rows = false);
+                   bool live_rows = false, uint32_t* host_choice = nullptr);
     int compact_now();
     // Reads the device counters; fold: add the DELTA update's per-workgroup
     // statistics first (once a run's last update is done).
