@@ -89,7 +89,7 @@ typedef struct kt_variant {
     uint32_t tiles_per_dim;        /* -t: tiles per dimension (tiled format)             */
     uint32_t num_parallel_tiles;   /* -y: x-tile groups counted in parallel (tiled)      */
     uint32_t tile_size;            /* vertices per tile; 0 = ceil(n / tiles_per_dim)      */
-    uint32_t chunk;                /* worklist entries per wave task; 0 = default        */
+    uint32_t chunk;                /* entries per count task; 0 = 128 (DELTA: 64)        */
     uint32_t schedule;             /* KT_SCHED_* bits for kt_plan_decompose              */
     uint32_t count_kernel;         /* KT_COUNT_*: support-count strategy                 */
     uint32_t cap_lds;              /* LDS slice per wave in entries; 0 = automatic       */
@@ -239,7 +239,9 @@ typedef struct kt_count_config {
     uint32_t count_kernel; /* KT_COUNT_*                              */
     uint32_t block_size;   /* threads per workgroup, multiple of 64    */
     uint32_t cap_lds;      /* LDS slice per wave; 0 = automatic        */
-    uint32_t chunk;        /* items per wave task; 0 = default         */
+    uint32_t chunk;        /* items per wave task; 0 = default (128;   */
+                           /* 64 for DELTA, whose full count is the   */
+                           /* pass that opens a run)                  */
 } kt_count_config;
 
 enum { KT_PICK_FIXED = 0u, KT_PICK_SWEEP = 1u, KT_PICK_MODEL = 2u };

@@ -87,7 +87,8 @@ class CountConfig(ctypes.Structure):
 
     def describe(self):
         names = {0: "auto", 1: "row", 2: "search", 3: "merge", 4: "delta"}
-        return f"{names[self.count_kernel]}/b{self.block_size}/c{self.cap_lds or 'auto'}/k{self.chunk or 128}"
+        chunk = self.chunk or (64 if self.count_kernel == COUNT_KERNELS["delta"] else 128)
+        return f"{names[self.count_kernel]}/b{self.block_size}/c{self.cap_lds or 'auto'}/k{chunk}"
 
 
 class Stats(ctypes.Structure):
