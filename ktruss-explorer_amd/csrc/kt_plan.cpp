@@ -1070,8 +1070,9 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     rec.fill(-1.f);
     CountCfg cfg = fixed_cfg;
     uint32_t chosen = 0;
-    // the model's choice is read with the next counter read-back (the JUMP's
-    // or, when deletions are pending, right before the support pass)
+    // the model's choice is read from pinned memory at an early phase start
+    // (below), else with the next counter read-back (the JUMP's or, when
+    // deletions are pending, right before the support pass)
     bool unresolved = pick == KT_PICK_MODEL;
     // with nothing pending the deletion count is free to clear now; the JUMP
     // minimum is reset here too (one kernel instead of three)
