@@ -17,11 +17,15 @@
 //               handled by the deleted edge of lowest entry index among its
 //               three edges, which decrements the supports of the triangle's
 //               edges that survive -- so every destroyed triangle is removed
-//               exactly once from every surviving edge;
-//   k_finalize  the deleted entries become KT_DEAD in the oriented layout and
-//               in both of their adjacency-list slots.
-// Between the mark and k_finalize a deleted entry and both of its adjacency
-// slots carry KT_DYING (bit 30; ids stay below 2^30).
+//               exactly once from every surviving edge (k_delta<true> also
+//               lists the entries whose support falls below k - 2: the next
+//               deletions, without a mark pass);
+//   finalize    the deleted entries become KT_DEAD in the oriented layout and
+//               in both of their adjacency-list slots: in workgroups of the
+//               next mark's launch, in the next task list after a light
+//               iteration, or in k_finalize.
+// Between its flagging and its finalization a deleted entry and both of its
+// adjacency slots carry KT_DYING (bit 30; ids stay below 2^30).
 // Adjacency layout (symmetric CSR over the relabelled ids):
 //   sptr[n+1]   list starts
 //   snbr[2E]    neighbour id, KT_DEAD set once the edge is gone; a vertex's
