@@ -1,0 +1,234 @@
+// kt_build.hip -- gfx950 construction of the stored layouts (the graph's
+// preprocessing: degree relabel, orientation, CSR rows, full adjacency).
+//
+// Degree orientation (sortByDegree + undirected2directedCOO + coo2coocsr,
+// graph.cu:152-206 and :372): vertices are relabelled by ascending degree,
+// ties by id (a stable radix sort of the degrees over the ids; the
+// reference's quicksort tie order is not result-bearing); every edge becomes
+// (min, max) of its relabelled ends; the entries are sorted by that pair,
+// which gives the rows by source with destinations ascending.  The full
+// adjacency of an oriented layout (the DELTA lists): every entry (s, d)
+// appears in s's list as d and in d's list as s; sorting the 2E
+// (vertex, neighbour) pairs gives each vertex's list ascending (in-neighbours,
+// which are smaller, then out-neighbours).  Both produce exactly the arrays
+// the host builders (kt_host.cpp build_layout / build_adjacency) produce.
+// Row starts are lower bounds of each vertex in the sorted keys.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "kt_internal.h"
+
+namespace kt {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+uint32_t grid(uint64_t n) {
+    const uint64_t b = (n + kBlock - 1) / kBlock;
+    return (uint32_t)(b < 65535 ? (b ? b : 1) : 65535);
+}
+
+uint32_t bits_for(uint64_t v) {  // bits of v (0 for 0)
+    uint32_t b = 0;
+    while (b < 64 && (v >> b)) ++b;
+    return b;
+}
+
+__global__ void k_iota(uint32_t* __restrict__ out, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = i;
+}
+
+__global__ void k_invert(const uint32_t* __restrict__ new2old, uint32_t* __restrict__ old2new, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        old2new[new2old[i]] = i;
+}
+
+// key of edge e: (min, max) of its relabelled ends; value: e
+__global__ void k_orient(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                         const uint32_t* __restrict__ old2new, uint32_t m, uint64_t* __restrict__ keys,
+                         uint32_t* __restrict__ vals) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const uint32_t x = old2new[a[e]], y = old2new[b[e]];
+        const uint32_t s = x < y ? x : y, d = x < y ? y : x;
+        keys[e] = ((uint64_t)s << 32) | d;
+        vals[e] = e;
+    }
+}
+
+__global__ void k_unpack_layout(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                uint32_t m, uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                uint32_t* __restrict__ pos) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < m; p += gridDim.x * blockDim.x) {
+        const uint64_t k = keys[p];
+        src[p] = (uint32_t)(k >> 32);
+        dst[p] = (uint32_t)k;
+        pos[vals[p]] = p;
+    }
+}
+
+// starts[v] = first index of sorted keys whose high word is >= v, v <= n
+__global__ void k_row_starts(const uint64_t* __restrict__ keys, uint32_t len, uint32_t n,
+                             uint32_t* __restrict__ starts) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= n; v += gridDim.x * blockDim.x) {
+        const uint64_t x = (uint64_t)v << 32;
+        uint32_t lo = 0, hi = len;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (keys[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        starts[v] = lo;
+    }
+}
+
+__global__ void k_max_row(const uint32_t* __restrict__ ptr, uint32_t n, uint32_t* max_row) {
+    uint32_t m = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t l = ptr[v + 1] - ptr[v];
+        m = l > m ? l : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(max_row, m);
+}
+
+// two list items per entry p = (s, d): d in s's list, s in d's list
+__global__ void k_adj_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t E,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < E; p += gridDim.x * blockDim.x) {
+        const uint32_t s = src[p], d = dst[p];
+        keys[2 * (size_t)p] = ((uint64_t)s << 32) | d;
+        keys[2 * (size_t)p + 1] = ((uint64_t)d << 32) | s;
+        vals[2 * (size_t)p] = p;
+        vals[2 * (size_t)p + 1] = p;
+    }
+}
+
+// slot j of the sorted items: its neighbour and entry; the entry's slot in its
+// source's list (neighbour above the list's vertex) is sa, the other sb
+__global__ void k_unpack_adj(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                             uint32_t S, uint32_t* __restrict__ snbr, uint32_t* __restrict__ seid,
+                             uint32_t* __restrict__ sa, uint32_t* __restrict__ sb) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < S; j += gridDim.x * blockDim.x) {
+        const uint64_t k = keys[j];
+        const uint32_t x = (uint32_t)(k >> 32), nb = (uint32_t)k, p = vals[j];
+        snbr[j] = nb;
+        seid[j] = p;
+        if (nb > x) sa[p] = j;
+        else sb[p] = j;
+    }
+}
+
+// device buffer released when the scope ends (construction is not timed)
+struct Tmp {
+    void* p = nullptr;
+    ~Tmp() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+#define KT_BUILD_TRY(x)                   \
+    do {                                  \
+        const hipError_t e_ = (x);        \
+        if (e_ != hipSuccess) return e_;  \
+    } while (0)
+
+// sorts (keys, vals) of len items on their low `bits` bits into (ko, vo)
+hipError_t sort_pairs(const uint64_t* ki, uint64_t* ko, const uint32_t* vi, uint32_t* vo, uint32_t len,
+                      uint32_t bits, hipStream_t s) {
+    size_t bytes = 0;
+    KT_BUILD_TRY(rocprim::radix_sort_pairs(nullptr, bytes, ki, ko, vi, vo, len, 0u, bits, s));
+    Tmp t;
+    KT_BUILD_TRY(t.alloc(bytes));
+    KT_BUILD_TRY(rocprim::radix_sort_pairs(t.p, bytes, ki, ko, vi, vo, len, 0u, bits, s));
+    return hipStreamSynchronize(s);  // t is released on return
+}
+
+}  // namespace
+
+namespace dev {
+
+hipError_t build_degree_layout(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b,
+                               const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
+                               uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
+                               hipStream_t s) {
+    // relabel: vertices sorted by degree, stable in id
+    Tmp ids, old2new;
+    KT_BUILD_TRY(ids.alloc((size_t)n * 4));
+    KT_BUILD_TRY(old2new.alloc((size_t)n * 4));
+    hipLaunchKernelGGL(k_iota, dim3(grid(n)), dim3(kBlock), 0, s, ids.as<uint32_t>(), n);
+    KT_BUILD_TRY(hipGetLastError());
+    {
+        size_t bytes = 0;
+        const uint32_t bits = bits_for(max_degree) ? bits_for(max_degree) : 1u;
+        Tmp dsorted;
+        KT_BUILD_TRY(dsorted.alloc((size_t)n * 4));
+        KT_BUILD_TRY(rocprim::radix_sort_pairs(nullptr, bytes, degree, dsorted.as<uint32_t>(), ids.as<uint32_t>(),
+                                               new2old, n, 0u, bits, s));
+        Tmp t;
+        KT_BUILD_TRY(t.alloc(bytes));
+        KT_BUILD_TRY(rocprim::radix_sort_pairs(t.p, bytes, degree, dsorted.as<uint32_t>(), ids.as<uint32_t>(),
+                                               new2old, n, 0u, bits, s));
+        KT_BUILD_TRY(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(k_invert, dim3(grid(n)), dim3(kBlock), 0, s, new2old, old2new.as<uint32_t>(), n);
+    KT_BUILD_TRY(hipGetLastError());
+    // entries sorted by (source, destination)
+    Tmp k0, k1, v0, v1;
+    KT_BUILD_TRY(k0.alloc((size_t)m * 8));
+    KT_BUILD_TRY(k1.alloc((size_t)m * 8));
+    KT_BUILD_TRY(v0.alloc((size_t)m * 4));
+    KT_BUILD_TRY(v1.alloc((size_t)m * 4));
+    if (m) {
+        hipLaunchKernelGGL(k_orient, dim3(grid(m)), dim3(kBlock), 0, s, a, b, old2new.as<uint32_t>(), m,
+                           k0.as<uint64_t>(), v0.as<uint32_t>());
+        KT_BUILD_TRY(hipGetLastError());
+        KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), m,
+                                32 + (bits_for(n) ? bits_for(n) : 1u), s));
+        hipLaunchKernelGGL(k_unpack_layout, dim3(grid(m)), dim3(kBlock), 0, s, k1.as<uint64_t>(),
+                           v1.as<uint32_t>(), m, src, dst, pos);
+        KT_BUILD_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_row_starts, dim3(grid((uint64_t)n + 1)), dim3(kBlock), 0, s, k1.as<uint64_t>(), m, n,
+                       ptr);
+    KT_BUILD_TRY(hipGetLastError());
+    KT_BUILD_TRY(hipMemsetAsync(max_row, 0, 4, s));
+    hipLaunchKernelGGL(k_max_row, dim3(grid(n)), dim3(kBlock), 0, s, ptr, n, max_row);
+    KT_BUILD_TRY(hipGetLastError());
+    return hipStreamSynchronize(s);
+}
+
+hipError_t build_adjacency_device(uint32_t n, uint32_t E, const uint32_t* src, const uint32_t* dst, uint32_t* sptr,
+                                  uint32_t* snbr, uint32_t* seid, uint32_t* sa, uint32_t* sb, hipStream_t s) {
+    const uint32_t S = 2 * E;
+    Tmp k0, k1, v0, v1;
+    KT_BUILD_TRY(k0.alloc((size_t)S * 8));
+    KT_BUILD_TRY(k1.alloc((size_t)S * 8));
+    KT_BUILD_TRY(v0.alloc((size_t)S * 4));
+    KT_BUILD_TRY(v1.alloc((size_t)S * 4));
+    if (E) {
+        hipLaunchKernelGGL(k_adj_keys, dim3(grid(E)), dim3(kBlock), 0, s, src, dst, E, k0.as<uint64_t>(),
+                           v0.as<uint32_t>());
+        KT_BUILD_TRY(hipGetLastError());
+        KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), S,
+                                32 + (bits_for(n) ? bits_for(n) : 1u), s));
+        hipLaunchKernelGGL(k_unpack_adj, dim3(grid(S)), dim3(kBlock), 0, s, k1.as<uint64_t>(), v1.as<uint32_t>(),
+                           S, snbr, seid, sa, sb);
+        KT_BUILD_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_row_starts, dim3(grid((uint64_t)n + 1)), dim3(kBlock), 0, s, k1.as<uint64_t>(), S, n,
+                       sptr);
+    KT_BUILD_TRY(hipGetLastError());
+    return hipStreamSynchronize(s);
+}
+
+}  // namespace dev
+}  // namespace kt

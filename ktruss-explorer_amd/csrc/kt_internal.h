@@ -249,6 +249,21 @@ hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_
                    uint32_t* ptr_out, uint32_t* src_out, uint32_t* dst_out, uint32_t* eid_out,
                    uint32_t* sup_out, uint32_t* tile_sums, uint32_t* xs, DevCounters* ctr,
                    hipStream_t s);
+// Layout construction on the device (kt_build.hip), each the same arrays as
+// its host twin in kt_host.cpp; temporaries are allocated and released inside
+// and the stream is synchronized on return.
+// Degree orientation of m unique pairs (a[e], b[e]) over n vertices with the
+// given degrees (at most max_degree): ptr (n + 1), src / dst (m, rows by
+// source, destinations ascending), new2old (n), pos (m: pair -> position)
+// and *max_row (the longest row).
+hipError_t build_degree_layout(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b,
+                               const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
+                               uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
+                               hipStream_t s);
+// Full adjacency of an oriented layout of E entries (src < dst each): sptr
+// (n + 1), snbr / seid (2E), sa / sb (E: the entry's slot in src's / dst's list).
+hipError_t build_adjacency_device(uint32_t n, uint32_t E, const uint32_t* src, const uint32_t* dst, uint32_t* sptr,
+                                  uint32_t* snbr, uint32_t* seid, uint32_t* sa, uint32_t* sb, hipStream_t s);
 }  // namespace dev
 
 }  // namespace kt

@@ -110,14 +110,23 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     directed = v.directedness != KT_UNDIRECTED;
     tiled = v.format == KT_TILED_COOCSR;
     affected = v.recount == KT_RECOUNT_AFFECTED;
+    // the degree orientation's COO+CSR layout (the default variant's, and
+    // DELTA's) is built on the device; the others on the host
+    const bool dev_layout = v.directedness == KT_DIRECTED_BY_DEGREE && !tiled;
     HostLayout L;
-    rc = build_layout(g, v.directedness, L);
-    if (rc) return rc;
-    n = L.n;
-    E0 = L.E;
-    max_row = L.max_row;
-    new2old = std::move(L.new2old);
-    pos_of_edge = std::move(L.pos_of_edge);
+    if (dev_layout) {
+        if (g->m >= (1ull << 32) - 1) return fail(KT_ERR_ARG, "too many stored entries");
+        n = g->n;
+        E0 = (uint32_t)g->m;
+    } else {
+        rc = build_layout(g, v.directedness, L);
+        if (rc) return rc;
+        n = L.n;
+        E0 = L.E;
+        max_row = L.max_row;
+        new2old = std::move(L.new2old);
+        pos_of_edge = std::move(L.pos_of_edge);
+    }
     if (tiled) {
         if (v.tile_size == 0) {
             if (v.tiles_per_dim == 0) return fail(KT_ERR_ARG, "tiles_per_dim must be positive");
@@ -132,14 +141,6 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     } else {
         tile_size = n ? n : 1;
         ytiles = 1;
-    }
-    // LDS slice per wave: the largest power of two up to the longest row that
-    // keeps a workgroup within 64 KiB
-    {
-        kt_count_config c{v.count_kernel, v.block_size, v.cap_lds, v.chunk};
-        if ((rc = resolve(c, fixed_cfg))) return rc;
-        v.count_kernel = fixed_cfg.kernel;
-        cap_lds = fixed_cfg.cap;
     }
     query_factor = 8;
 
@@ -170,18 +171,42 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = stripes_x.alloc(kStripeBytes)) ||
         (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))))
         return rc;
-    KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
-    if (E0) {
-        KT_HIP_TRY(hipMemcpyAsync(src0.p, L.src.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(dst0.p, L.dst.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+    if (dev_layout) {
+        DBuf da, db, ddeg, dn2o, dpos, dmax;
+        if ((rc = da.alloc(Eb)) || (rc = db.alloc(Eb)) || (rc = ddeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
+            (rc = dn2o.alloc(std::max<size_t>(Pb - 4, 4))) || (rc = dpos.alloc(Eb)) || (rc = dmax.alloc(4)))
+            return rc;
+        if (E0) {
+            KT_HIP_TRY(hipMemcpyAsync(da.p, g->ua.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+            KT_HIP_TRY(hipMemcpyAsync(db.p, g->ub.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        }
+        if (n) KT_HIP_TRY(hipMemcpyAsync(ddeg.p, g->degree.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(dev::build_degree_layout(n, E0, da.as<uint32_t>(), db.as<uint32_t>(), ddeg.as<uint32_t>(),
+                                            g->max_degree, ptr0.as<uint32_t>(), src0.as<uint32_t>(),
+                                            dst0.as<uint32_t>(), dn2o.as<uint32_t>(), dpos.as<uint32_t>(),
+                                            dmax.as<uint32_t>(), stream));
+        new2old.resize(n);
+        pos_of_edge.resize(E0);
+        if (n) KT_HIP_TRY(hipMemcpy(new2old.data(), dn2o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        if (E0) KT_HIP_TRY(hipMemcpy(pos_of_edge.data(), dpos.p, (size_t)E0 * 4, hipMemcpyDeviceToHost));
+        KT_HIP_TRY(hipMemcpy(&max_row, dmax.p, 4, hipMemcpyDeviceToHost));
+    } else {
+        KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
+        if (E0) {
+            KT_HIP_TRY(hipMemcpyAsync(src0.p, L.src.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+            KT_HIP_TRY(hipMemcpyAsync(dst0.p, L.dst.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        }
     }
     KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(dev::fill_u32(stripes_x.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
-    if (directed && !tiled) {  // kept for the adjacency lists DELTA builds on first use
-        h_ptr = std::move(L.ptr);
-        h_src = std::move(L.src);
-        h_dst = std::move(L.dst);
+    // LDS slice per wave: the largest power of two up to the longest row that
+    // keeps a workgroup within 64 KiB
+    {
+        kt_count_config c{v.count_kernel, v.block_size, v.cap_lds, v.chunk};
+        if ((rc = resolve(c, fixed_cfg))) return rc;
+        v.count_kernel = fixed_cfg.kernel;
+        cap_lds = fixed_cfg.cap;
     }
     graph = graph_retain(g);
     return KT_OK;
@@ -651,12 +676,7 @@ bool Plan::delta_capable() const {
 int Plan::ensure_delta() {
     if (delta_ready) return KT_OK;
     if (n >= KT_DYING) return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^30 vertices");
-    if (h_ptr.size() != (size_t)n + 1 || h_src.size() != E0)
-        return fail(KT_ERR_STATE, "oriented host layout not kept for this plan");
-    std::vector<uint32_t> hs_ptr, hs_nbr, hs_eid, hs_a, hs_b;
-    int rc = build_adjacency(n, E0, h_ptr.data(), h_src.data(), h_dst.data(), hs_ptr, hs_nbr,
-                             hs_eid, hs_a, hs_b);
-    if (rc) return rc;
+    int rc;
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4, Pb = ((size_t)n + 1) * 4;
     const uint32_t ct = dev::compact_tiles_for(2 * E0) + 2;
     for (int b = 0; b < 2; ++b)
@@ -670,13 +690,10 @@ int Plan::ensure_delta() {
         (rc = ctsum.alloc((size_t)ct * 4)) || (rc = cxs.alloc(2 * Eb + 4)))
         return rc;
     dev::delta_scratch_carve(dscratch.p, E0, dws);
-    KT_HIP_TRY(hipMemcpyAsync(sptr0.p, hs_ptr.data(), Pb, hipMemcpyHostToDevice, stream));
-    if (E0) {
-        KT_HIP_TRY(hipMemcpyAsync(snbr0.p, hs_nbr.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(seid0.p, hs_eid.data(), 2 * (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(sa0.p, hs_a.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipMemcpyAsync(sb0.p, hs_b.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-    }
+    // the full adjacency lists of the pristine oriented layout, built on the device
+    KT_HIP_TRY(dev::build_adjacency_device(n, E0, src0.as<uint32_t>(), dst0.as<uint32_t>(), sptr0.as<uint32_t>(),
+                                           snbr0.as<uint32_t>(), seid0.as<uint32_t>(), sa0.as<uint32_t>(),
+                                           sb0.as<uint32_t>(), stream));
     if ((rc = restore_delta())) return rc;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     delta_ready = true;

@@ -93,10 +93,9 @@ struct Plan {
     hipEvent_t ev_sw[2 * KT_MAX_CANDIDATES] = {};
     std::vector<std::array<float, KT_PHASE_REC>> phase_log;
 
-    // DELTA support maintenance: full adjacency lists (built on first use from
-    // the oriented host layout kept here) and the list of entries deleted by
-    // the last mark pass whose triangles are still counted in the supports
-    std::vector<uint32_t> h_ptr, h_src, h_dst;
+    // DELTA support maintenance: full adjacency lists (built on the device on
+    // first use from the pristine oriented layout) and the lists of entries
+    // deleted by mark passes whose triangles are still counted in the supports
     // adjacency lists: working pair (sptr/snbr/seid)[acur] of S_cur slots,
     // compacted into the other of the pair; pristine copies for every run
     DBuf sptr_b[2], snbr_b[2], seid_b[2], sa, sb;
