@@ -232,3 +232,172 @@ hipError_t build_adjacency_device(uint32_t n, uint32_t E, const uint32_t* src, c
 
 }  // namespace dev
 }  // namespace kt
+
+// ---------------------------------------------------------------------------
+// Input validation (kt_graph_create) on the device: range and self-loop
+// checks (graph.cu:54), then the entries sorted by (min, max, direction): a
+// symmetric input is valid when every (min, max) pair holds exactly one
+// entry of each direction (graph.cu:52-53 asserts symmetry), a pair input
+// when no pair repeats.  A valid input's unique edges are then the sorted
+// pairs in order, so edge ids are sorted positions (halved for a symmetric
+// input).  Errors report the lowest offending input entry.
+namespace kt {
+namespace {
+
+constexpr uint32_t kNoError = 0xFFFFFFFFu;
+
+__global__ void k_check_entries(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t m,
+                                uint32_t n, uint32_t* bad_range, uint32_t* bad_loop) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const uint32_t s = src[i], d = dst[i];
+        if (s >= n || d >= n) atomicMin(bad_range, i);
+        else if (s == d) atomicMin(bad_loop, i);
+    }
+}
+
+// key: min << 33 | max << 1 | (src > dst); value: the entry
+__global__ void k_pair_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t m,
+                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const uint32_t s = src[i], d = dst[i];
+        const uint32_t a = s < d ? s : d, b = s < d ? d : s;
+        keys[i] = ((uint64_t)a << 33) | ((uint64_t)b << 1) | (s > d ? 1u : 0u);
+        vals[i] = i;
+    }
+}
+
+// sym: position p starts a pair (even p) that must be (p, p + 1) with
+// directions 0 then 1; a single entry is asymmetric, anything else a
+// duplicate.  pairs: p repeating p - 1's pair is a duplicate.
+__global__ void k_check_pairs(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t m,
+                              int sym, uint32_t* bad_asym, uint32_t* bad_dup) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < m; p += gridDim.x * blockDim.x) {
+        const uint64_t pk = keys[p] >> 1;
+        const bool start = p == 0 || (keys[p - 1] >> 1) != pk;
+        if (sym) {
+            if (!start) continue;
+            const bool one = p + 1 >= m || (keys[p + 1] >> 1) != pk;
+            if (one) {
+                atomicMin(bad_asym, vals[p]);
+                continue;
+            }
+            const bool two = p + 2 >= m || (keys[p + 2] >> 1) != pk;
+            if (!two || (keys[p] & 1) != 0 || (keys[p + 1] & 1) != 1) atomicMin(bad_dup, vals[p]);
+        } else if (!start) {
+            atomicMin(bad_dup, vals[p - 1] < vals[p] ? vals[p - 1] : vals[p]);
+        }
+    }
+}
+
+// unique edge q = p >> shift: its ends, and every input entry's edge id
+__global__ void k_unique_edges(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t m,
+                               uint32_t shift, uint32_t* __restrict__ ua, uint32_t* __restrict__ ub,
+                               uint32_t* __restrict__ uid, uint32_t* __restrict__ degree) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < m; p += gridDim.x * blockDim.x) {
+        const uint32_t q = p >> shift;
+        uid[vals[p]] = q;
+        if ((p & ((1u << shift) - 1)) == 0) {
+            const uint64_t k = keys[p];
+            const uint32_t a = (uint32_t)(k >> 33), b = (uint32_t)((k >> 1) & 0xFFFFFFFFu);
+            ua[q] = a;
+            ub[q] = b;
+            atomicAdd(degree + a, 1u);
+            atomicAdd(degree + b, 1u);
+        }
+    }
+}
+
+__global__ void k_max_u32(const uint32_t* __restrict__ x, uint32_t n, uint32_t* out) {
+    uint32_t m = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        m = x[v] > m ? x[v] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+}  // namespace
+
+namespace dev {
+
+hipError_t validate_input(uint32_t n, uint32_t m, const uint32_t* src, const uint32_t* dst, bool sym,
+                          InputCheck& res, std::vector<uint32_t>& ua, std::vector<uint32_t>& ub,
+                          std::vector<uint32_t>& uid, std::vector<uint32_t>& degree, uint32_t& max_degree) {
+    hipStream_t s = nullptr;
+    KT_BUILD_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    Tmp ds, dd, flags;
+    KT_BUILD_TRY(ds.alloc((size_t)m * 4));
+    KT_BUILD_TRY(dd.alloc((size_t)m * 4));
+    KT_BUILD_TRY(flags.alloc(5 * 4));
+    uint32_t* f = flags.as<uint32_t>();  // range, loop, asymmetric, duplicate, max degree
+    KT_BUILD_TRY(hipMemcpyAsync(ds.p, src, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    KT_BUILD_TRY(hipMemcpyAsync(dd.p, dst, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    KT_BUILD_TRY(hipMemsetAsync(f, 0xFF, 4 * 4, s));
+    KT_BUILD_TRY(hipMemsetAsync(f + 4, 0, 4, s));
+    hipLaunchKernelGGL(k_check_entries, dim3(grid(m)), dim3(kBlock), 0, s, ds.as<uint32_t>(), dd.as<uint32_t>(), m,
+                       n, f, f + 1);
+    KT_BUILD_TRY(hipGetLastError());
+    uint32_t hf[5];
+    KT_BUILD_TRY(hipMemcpyAsync(hf, f, 2 * 4, hipMemcpyDeviceToHost, s));
+    KT_BUILD_TRY(hipStreamSynchronize(s));
+    res = InputCheck{};
+    if (hf[0] != kNoError || hf[1] != kNoError) {  // the lowest entry that is wrong
+        const bool range = hf[0] <= hf[1];
+        res.status = range ? KT_ERR_RANGE : KT_ERR_SELF_LOOP;
+        res.at = range ? hf[0] : hf[1];
+        return hipSuccess;
+    }
+    Tmp k0, k1, v0, v1;
+    KT_BUILD_TRY(k0.alloc((size_t)m * 8));
+    KT_BUILD_TRY(k1.alloc((size_t)m * 8));
+    KT_BUILD_TRY(v0.alloc((size_t)m * 4));
+    KT_BUILD_TRY(v1.alloc((size_t)m * 4));
+    hipLaunchKernelGGL(k_pair_keys, dim3(grid(m)), dim3(kBlock), 0, s, ds.as<uint32_t>(), dd.as<uint32_t>(), m,
+                       k0.as<uint64_t>(), v0.as<uint32_t>());
+    KT_BUILD_TRY(hipGetLastError());
+    KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), m,
+                            33 + (bits_for(n) ? bits_for(n) : 1u), s));
+    hipLaunchKernelGGL(k_check_pairs, dim3(grid(m)), dim3(kBlock), 0, s, k1.as<uint64_t>(), v1.as<uint32_t>(), m,
+                       sym ? 1 : 0, f + 2, f + 3);
+    KT_BUILD_TRY(hipGetLastError());
+    KT_BUILD_TRY(hipMemcpyAsync(hf + 2, f + 2, 2 * 4, hipMemcpyDeviceToHost, s));
+    KT_BUILD_TRY(hipStreamSynchronize(s));
+    if (hf[2] != kNoError || hf[3] != kNoError) {
+        const bool asym = hf[2] <= hf[3];
+        res.status = asym ? KT_ERR_ASYMMETRIC : KT_ERR_DUPLICATE;
+        res.at = asym ? hf[2] : hf[3];
+        return hipSuccess;
+    }
+    const uint32_t shift = sym ? 1u : 0u;
+    const uint32_t mu = m >> shift;
+    Tmp dua, dub, duid, ddeg;
+    KT_BUILD_TRY(dua.alloc((size_t)mu * 4));
+    KT_BUILD_TRY(dub.alloc((size_t)mu * 4));
+    KT_BUILD_TRY(duid.alloc((size_t)m * 4));
+    KT_BUILD_TRY(ddeg.alloc((size_t)n * 4));
+    KT_BUILD_TRY(hipMemsetAsync(ddeg.p, 0, (size_t)n * 4, s));
+    hipLaunchKernelGGL(k_unique_edges, dim3(grid(m)), dim3(kBlock), 0, s, k1.as<uint64_t>(), v1.as<uint32_t>(), m,
+                       shift, dua.as<uint32_t>(), dub.as<uint32_t>(), duid.as<uint32_t>(), ddeg.as<uint32_t>());
+    KT_BUILD_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_max_u32, dim3(grid(n)), dim3(kBlock), 0, s, ddeg.as<uint32_t>(), n, f + 4);
+    KT_BUILD_TRY(hipGetLastError());
+    ua.resize(mu);
+    ub.resize(mu);
+    uid.resize(m);
+    degree.resize(n);
+    KT_BUILD_TRY(hipMemcpyAsync(ua.data(), dua.p, (size_t)mu * 4, hipMemcpyDeviceToHost, s));
+    KT_BUILD_TRY(hipMemcpyAsync(ub.data(), dub.p, (size_t)mu * 4, hipMemcpyDeviceToHost, s));
+    KT_BUILD_TRY(hipMemcpyAsync(uid.data(), duid.p, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+    KT_BUILD_TRY(hipMemcpyAsync(degree.data(), ddeg.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    KT_BUILD_TRY(hipMemcpyAsync(&max_degree, f + 4, 4, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+}  // namespace dev
+}  // namespace kt

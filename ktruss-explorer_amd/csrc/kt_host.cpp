@@ -126,6 +126,9 @@ void bucket_by_min(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* 
 
 }  // namespace
 
+// inputs of at least this many entries are validated on the device
+constexpr uint64_t kDeviceValidateMin = 1ull << 16;
+
 int graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, uint32_t flags,
                  kt_graph** out) {
     if (!out) return fail(KT_ERR_ARG, "out is NULL");
@@ -135,6 +138,36 @@ int graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
     if (flags != KT_INPUT_SYMMETRIC && flags != KT_INPUT_PAIRS)
         return fail(KT_ERR_ARG, "unknown input flags");
     if (m >= (1ull << 32)) return fail(KT_ERR_ARG, "too many entries");
+    // large inputs are validated on the device once it is initialised
+    // (kt_build.hip: the same checks and the same edges, ids and degrees)
+    if (m >= kDeviceValidateMin && device_initialized()) {
+        auto* g = new (std::nothrow) kt_graph();
+        if (!g) return fail(KT_ERR_NOMEM, "graph");
+        dev::InputCheck chk;
+        const hipError_t e = dev::validate_input(n, (uint32_t)m, src, dst, flags == KT_INPUT_SYMMETRIC, chk, g->ua,
+                                                 g->ub, g->uid, g->degree, g->max_degree);
+        if (e != hipSuccess) {
+            delete g;
+            return fail(KT_ERR_HIP, std::string("input validation: ") + hipGetErrorString(e));
+        }
+        if (chk.status != KT_OK) {
+            const uint64_t i = chk.at;
+            char buf[160];
+            if (chk.status == KT_ERR_RANGE || chk.status == KT_ERR_SELF_LOOP)
+                std::snprintf(buf, sizeof buf, "entry %llu = (%u, %u) with n = %u", (unsigned long long)i, src[i],
+                              dst[i], n);
+            else
+                std::snprintf(buf, sizeof buf, "entry %llu = (%u, %u)%s", (unsigned long long)i, src[i], dst[i],
+                              chk.status == KT_ERR_ASYMMETRIC ? " has no reverse entry" : " repeats");
+            delete g;
+            return fail(chk.status, buf);
+        }
+        g->n = n;
+        g->m_input = m;
+        g->m = g->ua.size();
+        *out = g;
+        return KT_OK;
+    }
     // range and self-loop checks (graph.cu:54)
     std::atomic<int> bad{0};
     std::atomic<uint64_t> bad_at{0};

@@ -260,11 +260,25 @@ hipError_t build_degree_layout(uint32_t n, uint32_t m, const uint32_t* a, const 
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
                                hipStream_t s);
+// Input validation of m entries over n vertices (kt_graph_create's checks):
+// res.status is KT_OK or the error of the lowest offending entry res.at;
+// for a valid input, the unique edges (ua < ub, sorted), every entry's edge
+// (uid) and the degrees, in host vectors.
+struct InputCheck {
+    int status = KT_OK;
+    uint64_t at = 0;
+};
+hipError_t validate_input(uint32_t n, uint32_t m, const uint32_t* src, const uint32_t* dst, bool sym,
+                          InputCheck& res, std::vector<uint32_t>& ua, std::vector<uint32_t>& ub,
+                          std::vector<uint32_t>& uid, std::vector<uint32_t>& degree, uint32_t& max_degree);
 // Full adjacency of an oriented layout of E entries (src < dst each): sptr
 // (n + 1), snbr / seid (2E), sa / sb (E: the entry's slot in src's / dst's list).
 hipError_t build_adjacency_device(uint32_t n, uint32_t E, const uint32_t* src, const uint32_t* dst, uint32_t* sptr,
                                   uint32_t* snbr, uint32_t* seid, uint32_t* sa, uint32_t* sb, hipStream_t s);
 }  // namespace dev
+
+// the gfx950 device has been initialised (kt_init or a plan)
+bool device_initialized();
 
 }  // namespace kt
 

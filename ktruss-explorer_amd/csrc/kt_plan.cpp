@@ -76,6 +76,8 @@ int init_device(int device) {
     return KT_OK;
 }
 
+bool device_initialized() { return g_device_checked; }
+
 int ensure_device() {
     if (g_device_checked) return hipSetDevice(g_device) == hipSuccess ? KT_OK
                                                                       : fail(KT_ERR_HIP, "hipSetDevice");
