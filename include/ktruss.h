@@ -161,6 +161,26 @@ int kt_graph_info(const kt_graph* g, uint32_t* n, uint64_t* m_undirected, uint64
 int kt_read_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst);
 /* writeCOOGraphToFile (graph.cu:96-104). */
 int kt_write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst);
+/* Truss numbers as text: the graph TSV's layout with the weight column holding
+ * each entry's truss number, "dst+1 \t src+1 \t tau" per line (what
+ * `ktruss -D -o` writes).  No counterpart in the reference, which outputs a
+ * k-truss only; kt_read_tsv reads such a file as a graph. */
+int kt_write_truss_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                       const uint32_t* tau);
+/* kt_read_tsv that also returns the third column (malloc'ed, kt_free). */
+int kt_read_truss_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst,
+                      uint32_t** tau);
+/* Binary edge cache: a 32-byte little-endian header ("KTEDGES1", n, flags,
+ * m, checksum) then src[m] and dst[m] as u32, so large inputs skip text
+ * parsing.  No counterpart in the reference.  The reader checks the size
+ * against m and the checksum, and returns malloc'ed arrays (kt_free) and the
+ * KT_INPUT_* flags the entries were written with (flags may be NULL). */
+int kt_write_edges_bin(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                       uint32_t flags);
+int kt_read_edges_bin(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst,
+                      uint32_t* flags);
+/* 1 when the file starts with the binary edge cache's magic, else 0. */
+int kt_is_edges_bin(const char* path);
 /* verify (verify.cu:12-40) without the abort: 0 when both COO graphs hold the
  * same entries, 1 when they differ. */
 int kt_verify_edges(uint64_t m_a, const uint32_t* src_a, const uint32_t* dst_a,

@@ -2,7 +2,8 @@
 //
 // Same option letters, meanings and defaults as the reference's Config /
 // parseArgs (config.h:163-224 there); additions: -D (full decomposition),
-// -s (synthetic input instead of -g), -c none (skip verification).
+// -s (synthetic input instead of -g), -c none (skip verification), -B (write
+// the input as a binary edge cache; -g reads such a cache by its magic).
 // Unlike the reference (which prints and exits with status 0 on a bad
 // option), a bad option exits with status 2.
 #pragma once
@@ -39,14 +40,17 @@ struct Config {
     // additions
     bool decompose = false;
     const char* synthetic = nullptr;  // rmat:SCALE:EF:SEED | er:N:DEG:SEED | cl:N:GAMMA:DEG:SEED
+    const char* binCacheName = nullptr;  // -B: binary edge cache written from the input
 };
 
 inline void usage() {
     std::fprintf(stderr,
         "\nUsage:  ./ktruss [options]\n"
         "\nOptions:\n"
-        "    -g <graphFileName>       input graph, one \"dst src w\" triple per line, 1-based\n"
-        "                             (default = data/loc-gowalla_edges_adj.tsv)\n"
+        "    -g <graphFileName>       input graph, one \"dst src w\" triple per line, 1-based, or a\n"
+        "                             binary edge cache written by -B (default =\n"
+        "                             data/loc-gowalla_edges_adj.tsv)\n"
+        "    -B <binFileName>         also write the input as a binary edge cache\n"
         "    -k <k>                   k (default = 3)\n"
         "    -c <compareFileName>     file the result is verified against; \"none\" skips the check\n"
         "                             (default = data/loc-gowalla_edges_adj.tsv.3.reference.tsv)\n"
@@ -63,7 +67,8 @@ inline void usage() {
         "    -y <numParallelTiles>    x-tile classes counted in parallel for tiled-coocsr (default = 8)\n"
         "    -v <verbosity>           0: k-truss time as CSV, 1: phase times (default), 2: per-iteration\n"
         "    -D                       full decomposition: truss number of every edge (-o writes\n"
-        "                             \"dst src truss\" lines, -k is ignored)\n"
+        "                             \"dst src truss\" lines; the k-truss verified is the\n"
+        "                             entries with truss number >= k)\n"
         "    -s <spec>                synthetic input instead of -g: rmat:SCALE:EF:SEED,\n"
         "                             er:N:AVGDEG:SEED or cl:N:GAMMA:AVGDEG:SEED\n"
         "    -h                       help\n\n");
@@ -85,7 +90,7 @@ static const char* const kProcessors[] = {"cpu", "gpu"};
 
 inline int parseArgs(int argc, char** argv, Config& c) {
     int opt;
-    while ((opt = getopt(argc, argv, "g:k:o:w:x:c:p:d:f:e:r:b:t:y:v:hDs:")) >= 0) {
+    while ((opt = getopt(argc, argv, "g:k:o:w:x:c:p:d:f:e:r:b:t:y:v:hDs:B:")) >= 0) {
         int e = 0;
         switch (opt) {
             case 'g': c.graphFileName = optarg; break;
@@ -126,6 +131,7 @@ inline int parseArgs(int argc, char** argv, Config& c) {
             case 'v': c.verbosity = (unsigned)std::atoi(optarg); break;
             case 'D': c.decompose = true; break;
             case 's': c.synthetic = optarg; break;
+            case 'B': c.binCacheName = optarg; break;
             case 'h': usage(); return 1;
             default:
                 std::fprintf(stderr, "\nUnrecognized option!\n");

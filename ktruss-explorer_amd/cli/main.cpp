@@ -89,10 +89,14 @@ int main(int argc, char** argv) {
     if (c.synthetic) {
         rc = make_synthetic(c.synthetic, &n, &m, &src, &dst);
         flags = KT_INPUT_PAIRS;
+    } else if (kt_is_edges_bin(c.graphFileName)) {
+        rc = kt_read_edges_bin(c.graphFileName, &n, &m, &src, &dst, &flags);
     } else {
         rc = kt_read_tsv(c.graphFileName, &n, &m, &src, &dst);
     }
     if (rc) return report(1, "reading the graph");
+    if (c.binCacheName && (rc = kt_write_edges_bin(c.binCacheName, n, m, src, dst, flags)))
+        return report(1, "writing the binary edge cache");
     kt_graph* graph = nullptr;
     if ((rc = kt_graph_create(n, m, src, dst, flags, &graph))) return report(1, "kt_graph_create");
     uint64_t m_und = 0, m_in = 0;
@@ -175,10 +179,8 @@ int main(int argc, char** argv) {
     if (c.outFileName) {
         if (c.verbosity >= 1) std::printf("Writing result to file: %s\n", c.outFileName);
         if (c.decompose) {
-            FILE* fp = std::fopen(c.outFileName, "w");
-            if (!fp) return report(1, "opening the output file");
-            for (uint64_t e = 0; e < m_in; ++e) std::fprintf(fp, "%u\t%u\t%u\n", dst[e] + 1, src[e] + 1, tau[e]);
-            std::fclose(fp);
+            if ((rc = kt_write_truss_tsv(c.outFileName, m_in, src, dst, tau.data())))
+                return report(1, "kt_write_truss_tsv");
         } else if ((rc = kt_write_tsv(c.outFileName, out_m, out_src.data(), out_dst.data()))) {
             return report(1, "kt_write_tsv");
         }

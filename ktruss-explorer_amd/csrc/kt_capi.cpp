@@ -97,6 +97,28 @@ int kt_write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32
     KT_GUARD(return kt::write_tsv(path, m, src, dst);)
 }
 
+int kt_read_truss_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst,
+                      uint32_t** tau) {
+    KT_GUARD(return kt::read_truss_tsv(path, n, m, src, dst, tau);)
+}
+
+int kt_write_truss_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                       const uint32_t* tau) {
+    KT_GUARD(return kt::write_truss_tsv(path, m, src, dst, tau);)
+}
+
+int kt_write_edges_bin(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                       uint32_t flags) {
+    KT_GUARD(return kt::write_edges_bin(path, n, m, src, dst, flags);)
+}
+
+int kt_read_edges_bin(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst,
+                      uint32_t* flags) {
+    KT_GUARD(return kt::read_edges_bin(path, n, m, src, dst, flags);)
+}
+
+int kt_is_edges_bin(const char* path) { KT_GUARD(return kt::is_edges_bin(path);) }
+
 int kt_verify_edges(uint64_t m_a, const uint32_t* src_a, const uint32_t* dst_a, uint64_t m_b,
                     const uint32_t* src_b, const uint32_t* dst_b) {
     if ((m_a && (!src_a || !dst_a)) || (m_b && (!src_b || !dst_b)))

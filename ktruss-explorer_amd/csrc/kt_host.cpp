@@ -400,92 +400,6 @@ int build_adjacency(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t*
     return KT_OK;
 }
 
-// ---------------------------------------------------------------------------
-// TSV I/O: createCOOFromFile (graph.cu:37-79), writeCOOGraphToFile
-// (graph.cu:96-104).
-
-int read_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst) {
-    if (!path || !n || !m || !src || !dst) return fail(KT_ERR_ARG, "NULL argument");
-    FILE* fp = std::fopen(path, "rb");
-    if (!fp) return fail(KT_ERR_IO, std::string("cannot open ") + path);
-    std::fseek(fp, 0, SEEK_END);
-    const long sz = std::ftell(fp);
-    std::fseek(fp, 0, SEEK_SET);
-    std::vector<char> buf(sz > 0 ? (size_t)sz + 1 : 1, 0);
-    if (sz > 0 && std::fread(buf.data(), 1, (size_t)sz, fp) != (size_t)sz) {
-        std::fclose(fp);
-        return fail(KT_ERR_IO, std::string("short read of ") + path);
-    }
-    std::fclose(fp);
-    std::vector<uint32_t> vals;
-    vals.reserve((size_t)sz / 4 + 3);
-    const char* p = buf.data();
-    const char* end = p + (sz > 0 ? sz : 0);
-    while (p < end) {
-        while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
-        if (p >= end) break;
-        if (*p < '0' || *p > '9') return fail(KT_ERR_IO, std::string("non-numeric token in ") + path);
-        uint64_t v = 0;
-        while (p < end && *p >= '0' && *p <= '9') {
-            v = v * 10 + (uint64_t)(*p - '0');
-            if (v > 0xFFFFFFFFull) return fail(KT_ERR_IO, "id out of range");
-            ++p;
-        }
-        vals.push_back((uint32_t)v);
-    }
-    if (vals.size() % 3) return fail(KT_ERR_IO, std::string("truncated triple in ") + path);
-    const uint64_t cnt = vals.size() / 3;
-    uint32_t* s = (uint32_t*)std::malloc((cnt ? cnt : 1) * sizeof(uint32_t));
-    uint32_t* d = (uint32_t*)std::malloc((cnt ? cnt : 1) * sizeof(uint32_t));
-    if (!s || !d) {
-        std::free(s);
-        std::free(d);
-        return fail(KT_ERR_NOMEM, "tsv arrays");
-    }
-    uint32_t nn = 0;
-    for (uint64_t i = 0; i < cnt; ++i) {
-        const uint32_t b = vals[3 * i], a = vals[3 * i + 1];  // "dst src w"
-        if (a == b) {
-            std::free(s);
-            std::free(d);
-            return fail(KT_ERR_SELF_LOOP, "Edges from a vertex to itself are not allowed!");
-        }
-        if (a == 0 || b == 0) {
-            std::free(s);
-            std::free(d);
-            return fail(KT_ERR_RANGE, "vertex ids are 1-based");
-        }
-        s[i] = a - 1;
-        d[i] = b - 1;
-        nn = std::max(nn, std::max(a, b));
-    }
-    *n = nn;
-    *m = cnt;
-    *src = s;
-    *dst = d;
-    return KT_OK;
-}
-
-int write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst) {
-    if (!path || (m && (!src || !dst))) return fail(KT_ERR_ARG, "NULL argument");
-    FILE* fp = std::fopen(path, "wb");
-    if (!fp) return fail(KT_ERR_IO, std::string("cannot open ") + path);
-    std::vector<char> buf;
-    buf.reserve(1 << 20);
-    char tmp[48];
-    for (uint64_t e = 0; e < m; ++e) {
-        const int len = std::snprintf(tmp, sizeof tmp, "%u\t%u\t%u\n", dst[e] + 1, src[e] + 1, 1u);
-        buf.insert(buf.end(), tmp, tmp + len);
-        if (buf.size() > (1 << 20) - 64) {
-            std::fwrite(buf.data(), 1, buf.size(), fp);
-            buf.clear();
-        }
-    }
-    std::fwrite(buf.data(), 1, buf.size(), fp);
-    std::fclose(fp);
-    return KT_OK;
-}
-
 int verify_edges(uint64_t ma, const uint32_t* sa, const uint32_t* da, uint64_t mb,
                  const uint32_t* sb, const uint32_t* db) {
     if (ma != mb) return 1;

@@ -37,6 +37,12 @@ int build_adjacency(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t*
 
 int read_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst);
 int write_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst);
+int read_truss_tsv(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst, uint32_t** tau);
+int write_truss_tsv(const char* path, uint64_t m, const uint32_t* src, const uint32_t* dst, const uint32_t* tau);
+int write_edges_bin(const char* path, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                    uint32_t flags);
+int read_edges_bin(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, uint32_t** dst, uint32_t* flags);
+int is_edges_bin(const char* path);
 int verify_edges(uint64_t ma, const uint32_t* sa, const uint32_t* da, uint64_t mb,
                  const uint32_t* sb, const uint32_t* db);
 

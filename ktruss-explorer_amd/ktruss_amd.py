@@ -37,7 +37,8 @@ EXPORTS = [
     "kt_abi_version", "kt_last_error", "kt_status_string", "kt_init", "kt_device_synchronize",
     "kt_free",
     "kt_variant_default", "kt_graph_create", "kt_graph_destroy", "kt_graph_info",
-    "kt_read_tsv", "kt_write_tsv", "kt_verify_edges", "kt_generate_rmat",
+    "kt_read_tsv", "kt_write_tsv", "kt_read_truss_tsv", "kt_write_truss_tsv", "kt_write_edges_bin",
+    "kt_read_edges_bin", "kt_is_edges_bin", "kt_verify_edges", "kt_generate_rmat",
     "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
     "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
@@ -161,6 +162,14 @@ def lib():
         L.kt_read_tsv.argtypes = [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(_u32p),
                                   ctypes.POINTER(_u32p)]
         L.kt_write_tsv.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u32p, _u32p]
+        L.kt_read_truss_tsv.argtypes = [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(_u32p),
+                                        ctypes.POINTER(_u32p), ctypes.POINTER(_u32p)]
+        L.kt_write_truss_tsv.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u32p, _u32p, _u32p]
+        L.kt_write_edges_bin.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, _u32p, _u32p,
+                                         ctypes.c_uint32]
+        L.kt_read_edges_bin.argtypes = [ctypes.c_char_p, _u32p, _u64p, ctypes.POINTER(_u32p),
+                                        ctypes.POINTER(_u32p), _u32p]
+        L.kt_is_edges_bin.argtypes = [ctypes.c_char_p]
         L.kt_verify_edges.argtypes = [ctypes.c_uint64, _u32p, _u32p, ctypes.c_uint64, _u32p, _u32p]
         L.kt_generate_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _u32p,
@@ -613,6 +622,49 @@ def write_tsv(path, src, dst):
     s, sp = _u32(src)
     d, dp = _u32(dst)
     _check(lib().kt_write_tsv(path.encode(), len(s), sp, dp), "kt_write_tsv")
+
+
+def read_truss_tsv(path):
+    """(n, src, dst, tau) of a file written by write_truss_tsv (or any TSV:
+    tau is then its weight column)."""
+    n = ctypes.c_uint32()
+    m, sp, dp = _gen_out()
+    tp = _u32p()
+    _check(lib().kt_read_truss_tsv(path.encode(), ctypes.byref(n), ctypes.byref(m), ctypes.byref(sp),
+                                   ctypes.byref(dp), ctypes.byref(tp)), "kt_read_truss_tsv")
+    return (n.value, _take_array(sp, m.value), _take_array(dp, m.value), _take_array(tp, m.value))
+
+
+def write_truss_tsv(path, src, dst, tau):
+    s, sp = _u32(src)
+    d, dp = _u32(dst)
+    t, tp = _u32(tau)
+    if not (len(s) == len(d) == len(t)):
+        raise ValueError("src, dst and tau differ in length")
+    _check(lib().kt_write_truss_tsv(path.encode(), len(s), sp, dp, tp), "kt_write_truss_tsv")
+
+
+def write_edges_bin(path, n, src, dst, pairs=False):
+    s, sp = _u32(src)
+    d, dp = _u32(dst)
+    if len(s) != len(d):
+        raise ValueError("src and dst differ in length")
+    _check(lib().kt_write_edges_bin(path.encode(), n, len(s), sp, dp,
+                                    INPUT_PAIRS if pairs else INPUT_SYMMETRIC), "kt_write_edges_bin")
+
+
+def read_edges_bin(path):
+    """(n, src, dst, pairs) of a binary edge cache."""
+    n = ctypes.c_uint32()
+    m, sp, dp = _gen_out()
+    fl = ctypes.c_uint32()
+    _check(lib().kt_read_edges_bin(path.encode(), ctypes.byref(n), ctypes.byref(m), ctypes.byref(sp),
+                                   ctypes.byref(dp), ctypes.byref(fl)), "kt_read_edges_bin")
+    return n.value, _take_array(sp, m.value), _take_array(dp, m.value), fl.value == INPUT_PAIRS
+
+
+def is_edges_bin(path):
+    return bool(lib().kt_is_edges_bin(path.encode()))
 
 
 def verify_edges(src_a, dst_a, src_b, dst_b):

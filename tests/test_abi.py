@@ -5,6 +5,7 @@ generators are deterministic.  The CLI's argument handling is checked too."""
 import ctypes
 import os
 import re
+import struct
 import subprocess
 
 import numpy as np
@@ -184,3 +185,120 @@ def test_cli_help_and_bad_options():
 def test_cli_missing_graph_is_an_error(tmp_path):
     r = run_cli("-g", str(tmp_path / "nothing.tsv"))
     assert r.returncode == 1 and "reading the graph" in r.stderr
+
+
+def big_symmetric(n=1 << 16, m=1 << 17, seed=7):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, n, m).astype(np.uint32)
+    b = rng.integers(0, n, m).astype(np.uint32)
+    b[a == b] = (b[a == b] + 1) % n
+    return np.concatenate([a, b]), np.concatenate([b, a])
+
+
+def test_tsv_large_file_matches_oracle_io(tmp_path):
+    """Files of several MiB are parsed and formatted in parallel chunks; the
+    bytes and the parsed entries equal the oracle's serial reader/writer."""
+    s, d = big_symmetric()
+    p, q = str(tmp_path / "k.tsv"), str(tmp_path / "o.tsv")
+    K.write_tsv(p, s, d)
+    O.write_tsv(q, s, d)
+    assert os.path.getsize(p) > (1 << 21)
+    assert open(p, "rb").read() == open(q, "rb").read()
+    n, s2, d2 = K.read_tsv(p)
+    no, so, do = O.read_tsv(p)
+    assert n == no == int(max(s.max(), d.max())) + 1
+    assert np.array_equal(s2, so) and np.array_equal(d2, do)
+    assert np.array_equal(s2, s) and np.array_equal(d2, d)
+    # mixed whitespace and no final newline parse the same
+    text = open(p).read().replace("\t", "  ", 1000).rstrip("\n")
+    open(p, "w").write(text)
+    n3, s3, d3 = K.read_tsv(p)
+    assert n3 == n and np.array_equal(s3, s) and np.array_equal(d3, d)
+
+
+def test_tsv_large_file_errors_are_the_first_in_the_file(tmp_path):
+    s, d = big_symmetric()
+    lines = [f"{b + 1}\t{a + 1}\t1\n" for a, b in zip(s.tolist(), d.tolist())]
+    cases = []
+    loop = list(lines)
+    loop[150000] = "5\t5\t1\n"
+    loop[200000] = "0\t3\t1\n"
+    cases.append((loop, K.ERR_SELF_LOOP, "itself"))
+    zero = list(lines)
+    zero[100000] = "0\t3\t1\n"
+    zero[200000] = "5\t5\t1\n"
+    cases.append((zero, K.ERR_RANGE, "1-based"))
+    word = list(lines)
+    word[-3] = "7\tx\t1\n"
+    cases.append((word, K.ERR_IO, "non-numeric"))
+    big = list(lines)
+    big[20000] = "4294967296\t1\t1\n"
+    cases.append((big, K.ERR_IO, "out of range"))
+    cases.append((lines + ["1\t2\n"], K.ERR_IO, "truncated"))
+    for i, (ls, code, text) in enumerate(cases):
+        p = str(tmp_path / f"bad{i}.tsv")
+        open(p, "w").write("".join(ls))
+        with pytest.raises(K.KtError) as e:
+            K.read_tsv(p)
+        assert e.value.status == code and text in str(e.value), (i, str(e.value))
+
+
+def test_truss_tsv_round_trip_and_format(tmp_path):
+    s = np.array([0, 1, 1, 2, 9], np.uint32)
+    d = np.array([1, 0, 2, 1, 4294967294], np.uint32)
+    t = np.array([2, 2, 3, 3, 4294967295], np.uint32)
+    p = str(tmp_path / "t.tsv")
+    K.write_truss_tsv(p, s, d, t)
+    assert open(p).read() == "".join(f"{b + 1}\t{a + 1}\t{w}\n" for a, b, w in zip(s.tolist(), d.tolist(), t.tolist()))
+    n, s2, d2, t2 = K.read_truss_tsv(p)
+    assert n == 4294967295 and np.array_equal(s2, s) and np.array_equal(d2, d) and np.array_equal(t2, t)
+    # the graph reader takes it too (the third column is the weight)
+    _, s3, d3 = K.read_tsv(p)
+    assert np.array_equal(s3, s) and np.array_equal(d3, d)
+    with pytest.raises(ValueError):
+        K.write_truss_tsv(p, s, d, t[:2])
+
+
+def test_binary_edge_cache_format_and_errors(tmp_path):
+    s, d = big_symmetric(n=1000, m=5000, seed=3)
+    p = str(tmp_path / "g.ktb")
+    K.write_edges_bin(p, 1000, s, d)
+    raw = open(p, "rb").read()
+    assert len(raw) == 32 + 8 * len(s)
+    magic, n, flags, m, _ = struct.unpack("<8sIIQQ", raw[:32])
+    assert (magic, n, flags, m) == (b"KTEDGES1", 1000, K.INPUT_SYMMETRIC, len(s))
+    assert np.array_equal(np.frombuffer(raw[32:32 + 4 * m], "<u4"), s)
+    assert np.array_equal(np.frombuffer(raw[32 + 4 * m:], "<u4"), d)
+    assert K.is_edges_bin(p)
+    n2, s2, d2, pairs = K.read_edges_bin(p)
+    assert n2 == 1000 and not pairs and np.array_equal(s2, s) and np.array_equal(d2, d)
+    K.write_edges_bin(p, 1000, s[:10], d[:10], pairs=True)
+    assert K.read_edges_bin(p)[3] is True
+    # empty
+    K.write_edges_bin(p, 0, [], [])
+    n0, s0, _, _ = K.read_edges_bin(p)
+    assert n0 == 0 and len(s0) == 0
+    bad = bytearray(raw)
+    for where in (40, 32 + 4 * m + 8):  # one src, one dst value
+        b2 = bytearray(raw)
+        b2[where] ^= 1
+        open(p, "wb").write(bytes(b2))
+        with pytest.raises(K.KtError) as e:
+            K.read_edges_bin(p)
+        assert e.value.status == K.ERR_IO and "checksum" in str(e.value)
+    # two entries swapped
+    b2 = bytearray(raw)
+    b2[32:36], b2[36:40] = raw[36:40], raw[32:36]
+    if raw[32:36] != raw[36:40]:
+        open(p, "wb").write(bytes(b2))
+        with pytest.raises(K.KtError):
+            K.read_edges_bin(p)
+    open(p, "wb").write(bytes(bad[:-4]))
+    with pytest.raises(K.KtError) as e:
+        K.read_edges_bin(p)
+    assert "size" in str(e.value)
+    open(p, "wb").write(b"KTEDGES2" + bytes(bad[8:]))
+    with pytest.raises(K.KtError) as e:
+        K.read_edges_bin(p)
+    assert "not a binary edge file" in str(e.value)
+    assert not K.is_edges_bin(str(tmp_path / "missing.ktb"))

@@ -65,3 +65,25 @@ def test_cli_compare_file_and_decompose(small_graphs, tmp_path):
     got = {(int(a) - 1, int(b) - 1): int(t) for a, b, t in rows}
     exp = {(int(b), int(a)): int(t) for a, b, t in zip(s, d, tau_o)}
     assert got == exp
+
+
+def test_cli_binary_edge_cache(small_graphs, tmp_path):
+    """-B writes the input as a binary edge cache; -g reads it back by its
+    magic and gives the same truss file as the text input."""
+    n, s, d = small_graphs["plc300"]
+    graph = tmp_path / "g.tsv"
+    K.write_tsv(str(graph), s, d)
+    cache = tmp_path / "g.ktb"
+    out_txt = tmp_path / "tau_txt.tsv"
+    r = run_cli("-g", str(graph), "-B", str(cache), "-D", "-p", "gpu", "-o", str(out_txt), "-c", "none", "-v", "0")
+    assert r.returncode == 0, r.stderr
+    assert K.is_edges_bin(str(cache)) and not K.is_edges_bin(str(graph))
+    n2, s2, d2, pairs = K.read_edges_bin(str(cache))
+    assert (n2, pairs) == (n, False) and np.array_equal(s2, s) and np.array_equal(d2, d)
+    out_bin = tmp_path / "tau_bin.tsv"
+    r = run_cli("-g", str(cache), "-D", "-p", "gpu", "-o", str(out_bin), "-c", "none", "-v", "0")
+    assert r.returncode == 0, r.stderr
+    assert out_bin.read_bytes() == out_txt.read_bytes()
+    tau_o, _, _ = O.decompose(n, s, d)
+    _, ts, td, tt = K.read_truss_tsv(str(out_bin))
+    assert np.array_equal(ts, s) and np.array_equal(td, d) and np.array_equal(tt, tau_o)
