@@ -1,12 +1,14 @@
 // kt_build.hip -- gfx950 construction of the stored layouts (the graph's
 // preprocessing: degree relabel, orientation, CSR rows, full adjacency).
 //
-// Degree orientation (sortByDegree + undirected2directedCOO + coo2coocsr,
-// graph.cu:152-206 and :372): vertices are relabelled by ascending degree,
-// ties by id (a stable radix sort of the degrees over the ids; the
-// reference's quicksort tie order is not result-bearing); every edge becomes
-// (min, max) of its relabelled ends; the entries are sorted by that pair,
-// which gives the rows by source with destinations ascending.  The full
+// Stored layouts (sortByDegree + undirected2directedCOO + coo2coocsr,
+// graph.cu:152-237 and :372): for the degree orientation, vertices are
+// relabelled by ascending degree, ties by id (a stable radix sort of the
+// degrees over the ids; the reference's quicksort tie order is not
+// result-bearing); the other orientations keep ids.  Every edge becomes
+// (min, max) of its ends, and an undirected layout also stores (max, min);
+// the entries are sorted by that pair, which gives the rows by source with
+// destinations ascending (the tiled formats use the same rows).  The full
 // adjacency of an oriented layout (the DELTA lists): every entry (s, d)
 // appears in s's list as d and in d's list as s; sorting the 2E
 // (vertex, neighbour) pairs gives each vertex's list ascending (in-neighbours,
@@ -45,25 +47,34 @@ __global__ void k_invert(const uint32_t* __restrict__ new2old, uint32_t* __restr
 }
 
 // key of edge e: (min, max) of its relabelled ends; value: e
+// edge e's stored entries: (min, max) of its relabelled ends (old2new NULL:
+// ids kept), and for an undirected layout also (max, min) at m + e
 __global__ void k_orient(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
-                         const uint32_t* __restrict__ old2new, uint32_t m, uint64_t* __restrict__ keys,
+                         const uint32_t* __restrict__ old2new, uint32_t m, int both, uint64_t* __restrict__ keys,
                          uint32_t* __restrict__ vals) {
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
-        const uint32_t x = old2new[a[e]], y = old2new[b[e]];
+        const uint32_t x = old2new ? old2new[a[e]] : a[e], y = old2new ? old2new[b[e]] : b[e];
         const uint32_t s = x < y ? x : y, d = x < y ? y : x;
         keys[e] = ((uint64_t)s << 32) | d;
         vals[e] = e;
+        if (both) {
+            keys[(size_t)m + e] = ((uint64_t)d << 32) | s;
+            vals[(size_t)m + e] = e;
+        }
     }
 }
 
+// an edge's position: its one entry in a directed layout, the (max, min)
+// copy (the one in the higher row) in an undirected layout
 __global__ void k_unpack_layout(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                uint32_t m, uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                uint32_t E, int both, uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                 uint32_t* __restrict__ pos) {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < m; p += gridDim.x * blockDim.x) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < E; p += gridDim.x * blockDim.x) {
         const uint64_t k = keys[p];
-        src[p] = (uint32_t)(k >> 32);
-        dst[p] = (uint32_t)k;
-        pos[vals[p]] = p;
+        const uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)k;
+        src[p] = s;
+        dst[p] = d;
+        if (!both || s > d) pos[vals[p]] = p;
     }
 }
 
@@ -156,17 +167,22 @@ hipError_t sort_pairs(const uint64_t* ki, uint64_t* ko, const uint32_t* vi, uint
 
 namespace dev {
 
-hipError_t build_degree_layout(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b,
+hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b, int directedness,
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
                                hipStream_t s) {
-    // relabel: vertices sorted by degree, stable in id
+    const bool by_degree = directedness == KT_DIRECTED_BY_DEGREE, both = directedness == KT_UNDIRECTED;
+    const uint64_t E64 = both ? 2ull * m : (uint64_t)m;
+    if (E64 >= 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t E = (uint32_t)E64;
+    // relabel: vertices sorted by degree, stable in id (other orientations
+    // keep ids)
     Tmp ids, old2new;
     KT_BUILD_TRY(ids.alloc((size_t)n * 4));
     KT_BUILD_TRY(old2new.alloc((size_t)n * 4));
-    hipLaunchKernelGGL(k_iota, dim3(grid(n)), dim3(kBlock), 0, s, ids.as<uint32_t>(), n);
+    hipLaunchKernelGGL(k_iota, dim3(grid(n)), dim3(kBlock), 0, s, by_degree ? ids.as<uint32_t>() : new2old, n);
     KT_BUILD_TRY(hipGetLastError());
-    {
+    if (by_degree) {
         size_t bytes = 0;
         const uint32_t bits = bits_for(max_degree) ? bits_for(max_degree) : 1u;
         Tmp dsorted;
@@ -179,25 +195,28 @@ hipError_t build_degree_layout(uint32_t n, uint32_t m, const uint32_t* a, const 
                                                new2old, n, 0u, bits, s));
         KT_BUILD_TRY(hipStreamSynchronize(s));
     }
-    hipLaunchKernelGGL(k_invert, dim3(grid(n)), dim3(kBlock), 0, s, new2old, old2new.as<uint32_t>(), n);
-    KT_BUILD_TRY(hipGetLastError());
-    // entries sorted by (source, destination)
-    Tmp k0, k1, v0, v1;
-    KT_BUILD_TRY(k0.alloc((size_t)m * 8));
-    KT_BUILD_TRY(k1.alloc((size_t)m * 8));
-    KT_BUILD_TRY(v0.alloc((size_t)m * 4));
-    KT_BUILD_TRY(v1.alloc((size_t)m * 4));
-    if (m) {
-        hipLaunchKernelGGL(k_orient, dim3(grid(m)), dim3(kBlock), 0, s, a, b, old2new.as<uint32_t>(), m,
-                           k0.as<uint64_t>(), v0.as<uint32_t>());
-        KT_BUILD_TRY(hipGetLastError());
-        KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), m,
-                                32 + (bits_for(n) ? bits_for(n) : 1u), s));
-        hipLaunchKernelGGL(k_unpack_layout, dim3(grid(m)), dim3(kBlock), 0, s, k1.as<uint64_t>(),
-                           v1.as<uint32_t>(), m, src, dst, pos);
+    if (by_degree) {
+        hipLaunchKernelGGL(k_invert, dim3(grid(n)), dim3(kBlock), 0, s, new2old, old2new.as<uint32_t>(), n);
         KT_BUILD_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_row_starts, dim3(grid((uint64_t)n + 1)), dim3(kBlock), 0, s, k1.as<uint64_t>(), m, n,
+    // entries sorted by (source, destination)
+    Tmp k0, k1, v0, v1;
+    KT_BUILD_TRY(k0.alloc((size_t)E * 8));
+    KT_BUILD_TRY(k1.alloc((size_t)E * 8));
+    KT_BUILD_TRY(v0.alloc((size_t)E * 4));
+    KT_BUILD_TRY(v1.alloc((size_t)E * 4));
+    if (m) {
+        hipLaunchKernelGGL(k_orient, dim3(grid(m)), dim3(kBlock), 0, s, a, b,
+                           by_degree ? old2new.as<uint32_t>() : (const uint32_t*)nullptr, m, both ? 1 : 0,
+                           k0.as<uint64_t>(), v0.as<uint32_t>());
+        KT_BUILD_TRY(hipGetLastError());
+        KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), E,
+                                32 + (bits_for(n) ? bits_for(n) : 1u), s));
+        hipLaunchKernelGGL(k_unpack_layout, dim3(grid(E)), dim3(kBlock), 0, s, k1.as<uint64_t>(),
+                           v1.as<uint32_t>(), E, both ? 1 : 0, src, dst, pos);
+        KT_BUILD_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_row_starts, dim3(grid((uint64_t)n + 1)), dim3(kBlock), 0, s, k1.as<uint64_t>(), E, n,
                        ptr);
     KT_BUILD_TRY(hipGetLastError());
     KT_BUILD_TRY(hipMemsetAsync(max_row, 0, 4, s));

@@ -252,11 +252,13 @@ hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_
 // Layout construction on the device (kt_build.hip), each the same arrays as
 // its host twin in kt_host.cpp; temporaries are allocated and released inside
 // and the stream is synchronized on return.
-// Degree orientation of m unique pairs (a[e], b[e]) over n vertices with the
-// given degrees (at most max_degree): ptr (n + 1), src / dst (m, rows by
-// source, destinations ascending), new2old (n), pos (m: pair -> position)
-// and *max_row (the longest row).
-hipError_t build_degree_layout(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b,
+// Stored layout of m unique pairs (a[e], b[e]) over n vertices in the
+// orientation `directedness` (KT_DIRECTED_BY_DEGREE relabels by the given
+// degrees, at most max_degree; the others keep ids): ptr (n + 1), src / dst
+// (m entries, or 2m for KT_UNDIRECTED; rows by source, destinations
+// ascending), new2old (n), pos (m: pair -> its entry in the higher row) and
+// *max_row (the longest row).
+hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b, int directedness,
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
                                hipStream_t s);

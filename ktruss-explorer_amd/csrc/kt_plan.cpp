@@ -112,22 +112,13 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     directed = v.directedness != KT_UNDIRECTED;
     tiled = v.format == KT_TILED_COOCSR;
     affected = v.recount == KT_RECOUNT_AFFECTED;
-    // the degree orientation's COO+CSR layout (the default variant's, and
-    // DELTA's) is built on the device; the others on the host
-    const bool dev_layout = v.directedness == KT_DIRECTED_BY_DEGREE && !tiled;
-    HostLayout L;
-    if (dev_layout) {
-        if (g->m >= (1ull << 32) - 1) return fail(KT_ERR_ARG, "too many stored entries");
+    // the stored layout of every orientation is built on the device
+    // (kt_build.hip); tiled formats use the same rows
+    {
+        const uint64_t stored = directed ? g->m : 2 * g->m;
+        if (stored >= (1ull << 32) - 1) return fail(KT_ERR_ARG, "too many stored entries");
         n = g->n;
-        E0 = (uint32_t)g->m;
-    } else {
-        rc = build_layout(g, v.directedness, L);
-        if (rc) return rc;
-        n = L.n;
-        E0 = L.E;
-        max_row = L.max_row;
-        new2old = std::move(L.new2old);
-        pos_of_edge = std::move(L.pos_of_edge);
+        E0 = (uint32_t)stored;
     }
     if (tiled) {
         if (v.tile_size == 0) {
@@ -173,31 +164,27 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = stripes_x.alloc(kStripeBytes)) ||
         (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))))
         return rc;
-    if (dev_layout) {
+    {
+        const uint32_t m = (uint32_t)g->m;
+        const size_t Mb = (size_t)std::max<uint32_t>(m, 1) * sizeof(uint32_t);
         DBuf da, db, ddeg, dn2o, dpos, dmax;
-        if ((rc = da.alloc(Eb)) || (rc = db.alloc(Eb)) || (rc = ddeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
-            (rc = dn2o.alloc(std::max<size_t>(Pb - 4, 4))) || (rc = dpos.alloc(Eb)) || (rc = dmax.alloc(4)))
+        if ((rc = da.alloc(Mb)) || (rc = db.alloc(Mb)) || (rc = ddeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
+            (rc = dn2o.alloc(std::max<size_t>(Pb - 4, 4))) || (rc = dpos.alloc(Mb)) || (rc = dmax.alloc(4)))
             return rc;
-        if (E0) {
-            KT_HIP_TRY(hipMemcpyAsync(da.p, g->ua.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-            KT_HIP_TRY(hipMemcpyAsync(db.p, g->ub.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
+        if (m) {
+            KT_HIP_TRY(hipMemcpyAsync(da.p, g->ua.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
+            KT_HIP_TRY(hipMemcpyAsync(db.p, g->ub.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
         }
         if (n) KT_HIP_TRY(hipMemcpyAsync(ddeg.p, g->degree.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(dev::build_degree_layout(n, E0, da.as<uint32_t>(), db.as<uint32_t>(), ddeg.as<uint32_t>(),
-                                            g->max_degree, ptr0.as<uint32_t>(), src0.as<uint32_t>(),
-                                            dst0.as<uint32_t>(), dn2o.as<uint32_t>(), dpos.as<uint32_t>(),
-                                            dmax.as<uint32_t>(), stream));
+        KT_HIP_TRY(dev::build_layout_device(n, m, da.as<uint32_t>(), db.as<uint32_t>(), v.directedness,
+                                            ddeg.as<uint32_t>(), g->max_degree, ptr0.as<uint32_t>(),
+                                            src0.as<uint32_t>(), dst0.as<uint32_t>(), dn2o.as<uint32_t>(),
+                                            dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream));
         new2old.resize(n);
-        pos_of_edge.resize(E0);
+        pos_of_edge.resize(m);
         if (n) KT_HIP_TRY(hipMemcpy(new2old.data(), dn2o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
-        if (E0) KT_HIP_TRY(hipMemcpy(pos_of_edge.data(), dpos.p, (size_t)E0 * 4, hipMemcpyDeviceToHost));
+        if (m) KT_HIP_TRY(hipMemcpy(pos_of_edge.data(), dpos.p, (size_t)m * 4, hipMemcpyDeviceToHost));
         KT_HIP_TRY(hipMemcpy(&max_row, dmax.p, 4, hipMemcpyDeviceToHost));
-    } else {
-        KT_HIP_TRY(hipMemcpyAsync(ptr0.p, L.ptr.data(), Pb, hipMemcpyHostToDevice, stream));
-        if (E0) {
-            KT_HIP_TRY(hipMemcpyAsync(src0.p, L.src.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-            KT_HIP_TRY(hipMemcpyAsync(dst0.p, L.dst.data(), (size_t)E0 * 4, hipMemcpyHostToDevice, stream));
-        }
     }
     KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(dev::fill_u32(stripes_x.as<uint32_t>(), 0, kStripeBytes / 4, stream));
