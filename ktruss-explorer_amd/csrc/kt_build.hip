@@ -182,7 +182,7 @@ hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const 
     KT_BUILD_TRY(old2new.alloc((size_t)n * 4));
     hipLaunchKernelGGL(k_iota, dim3(grid(n)), dim3(kBlock), 0, s, by_degree ? ids.as<uint32_t>() : new2old, n);
     KT_BUILD_TRY(hipGetLastError());
-    if (by_degree) {
+    if (by_degree && n) {
         size_t bytes = 0;
         const uint32_t bits = bits_for(max_degree) ? bits_for(max_degree) : 1u;
         Tmp dsorted;

@@ -161,3 +161,32 @@ def test_rmat_matches_golden_fixture(scale):
         assert hashlib.sha256(np.ascontiguousarray(tau, "<u4").tobytes()).hexdigest() == \
             gold["tau_sha256"], v.describe()
         assert st["k_max"] == gold["k_max"]
+
+
+TINY = {"edgeless": (6, []), "no_vertices": (0, []), "one_edge": (10, [(3, 7)]),
+        "triangle_high_ids": (50, [(40, 45), (45, 49), (40, 49), (0, 49)])}
+
+
+@pytest.mark.parametrize("v", VARIANTS + [("directed-by-degree", "coocsr", "all", "delta")],
+                         ids=lambda v: "/".join(v))
+def test_empty_and_tiny_graphs_match_oracle(v):
+    """Graphs with no entries, no vertices, one edge, and a triangle on the
+    highest ids: every variant (and DELTA) gives the oracle's truss numbers,
+    k_max and k-truss edge sets."""
+    d, f, r = v[:3]
+    for name, (n, e) in TINY.items():
+        s = np.array([a for a, b in e] + [b for a, b in e], np.uint32)
+        t = np.array([b for a, b in e] + [a for a, b in e], np.uint32)
+        g = K.Graph(n, s, t)
+        gv = gpu_variant(d, f, r)
+        if len(v) > 3:
+            gv = K.make_variant(d, f, r, count_kernel=K.COUNT_KERNELS["delta"])
+        p = K.Plan(g, gv)
+        st = p.decompose()
+        tau_o, st_o, _ = O.decompose(n, s, t, oracle_variant(d, f, r), O.SCHED_JUMP | O.SCHED_REUSE)
+        assert np.array_equal(p.truss_numbers(), tau_o), name
+        assert st["k_max"] == st_o["k_max"], name
+        for k in (2, 3, 4):
+            os_, od_, _, _ = O.ktruss(n, s, t, k, oracle_variant(d, f, r))
+            gs, gd, _ = p.ktruss(k)
+            assert edge_set(gs, gd) == edge_set(os_, od_), (name, k)
