@@ -150,6 +150,28 @@ __device__ __forceinline__ uint32_t next_class_tile(uint32_t t, uint32_t y, uint
     return m == y ? t : t + (y + Y - m) % Y;
 }
 
+// Stream mode of a tiled item, class y: walking v's segments of class y costs
+// two binary searches to find the first and two per further segment, plus a
+// wave step per segment; scanning v's row once with the class as a lane
+// predicate costs a step per 64 entries.  Counted in dependent loads (a step:
+// one load and a search in u's row, about 16), from what is in registers:
+// the segments are at most the class-y tiles of u's candidate range
+// [lo, hi], the entries scanned at most vlen.  The scan is taken only when it
+// is at most half the walk; the same hits either way.  Rows spread over many
+// tiles of the class (segments of a few entries) scan; dense tiles, and
+// candidate ranges that few tiles of the class meet, walk.
+__device__ __forceinline__ bool tiled_scan_row(uint32_t vlen, uint32_t lo, uint32_t hi, uint32_t S, uint32_t y,
+                                               uint32_t Y) {
+    if (hi < lo) return false;
+    const uint32_t L = 32u - (uint32_t)__clz(vlen);
+    const uint32_t t0 = lo / S, t1 = hi / S;
+    // tiles t in [t0, t1] with t % Y == y
+    const uint64_t nseg = (t1 + 1 + Y - 1 - y) / Y - (t0 + Y - 1 - y) / Y;
+    const uint64_t walk = 2ull * L + nseg * (2ull * L + 16);
+    const uint64_t scan = (uint64_t)((vlen + 63) / 64) * 16 + (vlen > 128 ? L : 0);
+    return 2 * scan <= walk;
+}
+
 // Directed entry e = (u, v): credits of every triangle u < v < w.
 // u's candidates are row positions [lo, ulen) (the entries after v).
 template <bool TILED>
@@ -164,15 +186,17 @@ __device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, u
     const uint32_t umin = urow[lo] & KT_VMASK;
     uint32_t c = 0;
 
-    if (!TILED && (ulen - lo) * a.query_factor < vlen) {
-        // Query mode: u's candidates searched in v's row (HBM).
+    if ((ulen - lo) * a.query_factor < vlen) {
+        // Query mode: u's candidates searched in v's row (HBM); tiled, only
+        // the candidates of x-tile class y (the same triangles the segments
+        // of class y in v's row find)
         const uint32_t vmax = dst[vb + vlen - 1] & KT_VMASK;
         for (uint32_t p0 = lo; p0 < ulen; p0 += kWave) {
             const uint32_t p = p0 + lane;
             const uint32_t x = p < ulen ? urow[p] : KT_DEAD;
             bool hit = false;
             uint32_t j = 0;
-            if (!(x & KT_DEAD) && x <= vmax) {
+            if (!(x & KT_DEAD) && x <= vmax && (!TILED || ((x & KT_VMASK) / a.tile_size) % a.ytiles == y)) {
                 j = lower_bound_masked(dst + vb, 0, vlen, x);
                 hit = j < vlen && dst[vb + j] == x;
             }
@@ -192,7 +216,8 @@ __device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, u
     uint32_t jend = vlen;
     uint32_t t = 0;
     const uint32_t S = a.tile_size, Y = a.ytiles;
-    if (TILED) {
+    const bool scan = TILED && tiled_scan_row(vlen, umin, r.umax, S, y, Y);
+    if (TILED && !scan) {
         // first segment of class y at or after the tile of umin
         t = next_class_tile(umin / S, y, Y);
         j0 = wave_lower_bound(dst + vb, 0, vlen, t * S, lane);
@@ -209,7 +234,7 @@ __device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, u
             const uint32_t wm = w & KT_VMASK;
             bool hit = false;
             uint32_t p = 0;
-            if (inr && !(w & KT_DEAD) && wm <= r.umax) {
+            if (inr && !(w & KT_DEAD) && wm <= r.umax && (!scan || (wm / S) % Y == y)) {
                 p = lower_bound_masked(urow, lo, ulen, wm);
                 hit = p < ulen && urow[p] == wm;
             }
@@ -221,7 +246,7 @@ __device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, u
             c += (uint32_t)__popcll(ballot(hit));
             if (ballot(inr && wm > r.umax)) { done = true; break; }
         }
-        if (!TILED || done) break;
+        if (!TILED || done || scan) break;
         // next segment of class y
         if (jend >= vlen) break;
         const uint32_t tn = (dst[vb + jend] & KT_VMASK) / S;
@@ -245,13 +270,14 @@ __device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r,
     const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
     uint32_t c = 0;
 
-    if (!TILED && ulen * a.query_factor < vlen) {
+    if (ulen * a.query_factor < vlen) {
+        // query mode (tiled: the candidates of x-tile class y)
         const uint32_t vmax = dst[vb + vlen - 1] & KT_VMASK;
         for (uint32_t p0 = 0; p0 < ulen; p0 += kWave) {
             const uint32_t p = p0 + lane;
             const uint32_t x = p < ulen ? urow[p] : KT_DEAD;
             bool hit = false;
-            if (!(x & KT_DEAD) && x <= vmax) {
+            if (!(x & KT_DEAD) && x <= vmax && (!TILED || ((x & KT_VMASK) / a.tile_size) % a.ytiles == y)) {
                 const uint32_t j = lower_bound_masked(dst + vb, 0, vlen, x);
                 hit = j < vlen && dst[vb + j] == x;
             }
@@ -265,7 +291,8 @@ __device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r,
     const uint32_t umin = urow[0] & KT_VMASK;
     uint32_t j0 = 0, jend = vlen, t = 0;
     const uint32_t S = a.tile_size, Y = a.ytiles;
-    if (TILED) {
+    const bool scan = TILED && tiled_scan_row(vlen, umin, r.umax, S, y, Y);
+    if (TILED && !scan) {
         t = next_class_tile(umin / S, y, Y);
         j0 = wave_lower_bound(dst + vb, 0, vlen, t * S, lane);
         jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
@@ -280,7 +307,7 @@ __device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r,
             const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
             const uint32_t wm = w & KT_VMASK;
             bool hit = false;
-            if (inr && !(w & KT_DEAD) && wm <= r.umax) {
+            if (inr && !(w & KT_DEAD) && wm <= r.umax && (!scan || (wm / S) % Y == y)) {
                 const uint32_t p = lower_bound_masked(urow, 0, ulen, wm);
                 hit = p < ulen && urow[p] == wm;
             }
@@ -288,7 +315,7 @@ __device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r,
             if (c >= cap) return cap;
             if (ballot(inr && wm > r.umax)) { done = true; break; }
         }
-        if (!TILED || done) break;
+        if (!TILED || done || scan) break;
         if (jend >= vlen) break;
         const uint32_t tn = (dst[vb + jend] & KT_VMASK) / S;
         t = next_class_tile(tn, y, Y);
