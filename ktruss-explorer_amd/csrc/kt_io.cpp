@@ -198,8 +198,8 @@ int write_lines(const char* path, uint64_t m, const uint32_t* src, const uint32_
     if (!fp) return fail(KT_ERR_IO, std::string("cannot open ") + path);
     constexpr uint64_t kSlice = 1 << 18;
     constexpr size_t kLineMax = 3 * 10 + 3;
-    const size_t T = num_threads();
-    std::vector<std::vector<char>> out(T, std::vector<char>(kSlice * kLineMax));
+    const size_t T = (size_t)std::max<uint64_t>(1, std::min<uint64_t>(num_threads(), (m + kSlice - 1) / kSlice));
+    std::vector<std::vector<char>> out(T, std::vector<char>((size_t)std::min<uint64_t>(kSlice, m) * kLineMax));
     std::vector<size_t> len(T, 0);
     bool ok = true;
     for (uint64_t base = 0; base < m && ok; base += kSlice * T) {
