@@ -190,3 +190,21 @@ def test_empty_and_tiny_graphs_match_oracle(v):
             os_, od_, _, _ = O.ktruss(n, s, t, k, oracle_variant(d, f, r))
             gs, gd, _ = p.ktruss(k)
             assert edge_set(gs, gd) == edge_set(os_, od_), (name, k)
+
+
+@pytest.mark.parametrize("v", [x for x in VARIANTS if x[1] == "tiled-coocsr"], ids=vid)
+def test_many_small_tiles_match_oracle(small_graphs, v):
+    """Tiles of a few vertices (tiles_per_dim = n / 4) and 8 parallel tile
+    classes: a row meets many tiles of each class, so items take the class
+    query, the predicated row scan and the segment walk; truss numbers,
+    traces and the byte counters equal the oracle's."""
+    for name in ("plc300", "gnp80", "mixed"):
+        n, s, d = small_graphs[name]
+        tiles = max(1, n // 4)
+        tau_o, st_o, tr_o = O.decompose(n, s, d, oracle_variant(*v, tiles=tiles), O.SCHED_JUMP | O.SCHED_REUSE)
+        p = K.Plan(K.Graph(n, s, d), gpu_variant(*v, tiles=tiles, y=8))
+        st = p.decompose()
+        assert np.array_equal(p.truss_numbers(), tau_o), name
+        assert [tuple(x) for x in tr_o] == [(k, it, dl, rc) for k, it, dl, rc in p.trace()], name
+        for key in ("count_passes", "recount_edges", "merge_elems", "live_in_passes"):
+            assert st[key] == st_o[key], (name, key)
