@@ -223,9 +223,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X, and the k-truss path "
-              "is built and measured on a single device (see DESIGN.md for how it would shard).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the k-truss path runs on a single MI355X, so only "
+              "--gpus 1 is accepted.", file=sys.stderr)
         sys.exit(2)
 
     import ktruss_amd as K
