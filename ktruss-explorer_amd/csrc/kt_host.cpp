@@ -139,8 +139,9 @@ int graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
         return fail(KT_ERR_ARG, "unknown input flags");
     if (m >= (1ull << 32)) return fail(KT_ERR_ARG, "too many entries");
     // large inputs are validated on the device once it is initialised
-    // (kt_build.hip: the same checks and the same edges, ids and degrees)
-    if (m >= kDeviceValidateMin && device_initialized()) {
+    // (kt_build.hip: the same checks and the same edges, ids and degrees);
+    // ensure_device makes it current on this thread
+    if (m >= kDeviceValidateMin && device_initialized() && ensure_device() == KT_OK) {
         auto* g = new (std::nothrow) kt_graph();
         if (!g) return fail(KT_ERR_NOMEM, "graph");
         dev::InputCheck chk;

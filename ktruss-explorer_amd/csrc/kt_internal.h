@@ -281,6 +281,9 @@ hipError_t build_adjacency_device(uint32_t n, uint32_t E, const uint32_t* src, c
 
 // the gfx950 device has been initialised (kt_init or a plan)
 bool device_initialized();
+// makes the engine's device current on the calling thread (initialising it
+// on first use)
+int ensure_device();
 
 }  // namespace kt
 

@@ -46,7 +46,7 @@ void DBuf::release() {
 
 namespace {
 
-bool g_device_checked = false;
+std::atomic<bool> g_device_checked{false};  // read by graph creation on any thread
 int g_device = 0;
 
 int next_pow2(uint32_t x) {
@@ -72,11 +72,11 @@ int init_device(int device) {
         return fail(KT_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
                                           ", this build targets gfx950 (MI355X)");
     g_device = device;
-    g_device_checked = true;
+    g_device_checked.store(true);
     return KT_OK;
 }
 
-bool device_initialized() { return g_device_checked; }
+bool device_initialized() { return g_device_checked.load(); }
 
 int ensure_device() {
     if (g_device_checked) return hipSetDevice(g_device) == hipSuccess ? KT_OK
