@@ -97,6 +97,8 @@ int main(int argc, char** argv) {
     if (rc) return report(1, "reading the graph");
     if (c.binCacheName && (rc = kt_write_edges_bin(c.binCacheName, n, m, src, dst, flags)))
         return report(1, "writing the binary edge cache");
+    // the device first: large inputs are then checked and numbered on it
+    if ((rc = kt_init(0))) return report(1, "kt_init");
     kt_graph* graph = nullptr;
     if ((rc = kt_graph_create(n, m, src, dst, flags, &graph))) return report(1, "kt_graph_create");
     uint64_t m_und = 0, m_in = 0;
