@@ -3,7 +3,8 @@
 // Same option letters, meanings and defaults as the reference's Config /
 // parseArgs (config.h:163-224 there); additions: -D (full decomposition),
 // -s (synthetic input instead of -g), -c none (skip verification), -B (write
-// the input as a binary edge cache; -g reads such a cache by its magic).
+// the input as a binary edge cache; -g reads such a cache by its magic), -K
+// (support-pass kernel, KT_COUNT_*; default auto, the reference's count).
 // Unlike the reference (which prints and exits with status 0 on a bad
 // option), a bad option exits with status 2.
 #pragma once
@@ -41,6 +42,7 @@ struct Config {
     bool decompose = false;
     const char* synthetic = nullptr;  // rmat:SCALE:EF:SEED | er:N:DEG:SEED | cl:N:GAMMA:DEG:SEED
     const char* binCacheName = nullptr;  // -B: binary edge cache written from the input
+    int countKernel = KT_COUNT_AUTO;     // -K
 };
 
 inline void usage() {
@@ -69,6 +71,9 @@ inline void usage() {
         "    -D                       full decomposition: truss number of every edge (-o writes\n"
         "                             \"dst src truss\" lines; the k-truss verified is the\n"
         "                             entries with truss number >= k)\n"
+        "    -K <countKernel>         support passes: auto (default) | row | search | merge |\n"
+        "                             delta (maintained supports; directed-by-degree or\n"
+        "                             directed-by-index, coocsr, recount all)\n"
         "    -s <spec>                synthetic input instead of -g: rmat:SCALE:EF:SEED,\n"
         "                             er:N:AVGDEG:SEED or cl:N:GAMMA:AVGDEG:SEED\n"
         "    -h                       help\n\n");
@@ -86,11 +91,13 @@ inline bool parse_enum(const char* s, const char* const* names, int count, int* 
 static const char* const kDirectedness[] = {"undirected", "directed-by-index", "directed-by-degree"};
 static const char* const kFormats[] = {"coocsr", "tiled-coocsr"};
 static const char* const kRecounts[] = {"all", "affected"};
+// indexed by KT_COUNT_* (0..4)
+static const char* const kCountKernels[] = {"auto", "row", "search", "merge", "delta"};
 static const char* const kProcessors[] = {"cpu", "gpu"};
 
 inline int parseArgs(int argc, char** argv, Config& c) {
     int opt;
-    while ((opt = getopt(argc, argv, "g:k:o:w:x:c:p:d:f:e:r:b:t:y:v:hDs:B:")) >= 0) {
+    while ((opt = getopt(argc, argv, "g:k:o:w:x:c:p:d:f:e:r:b:t:y:v:hDs:B:K:")) >= 0) {
         int e = 0;
         switch (opt) {
             case 'g': c.graphFileName = optarg; break;
@@ -132,6 +139,12 @@ inline int parseArgs(int argc, char** argv, Config& c) {
             case 'D': c.decompose = true; break;
             case 's': c.synthetic = optarg; break;
             case 'B': c.binCacheName = optarg; break;
+            case 'K':
+                if (!parse_enum(optarg, kCountKernels, 5, &c.countKernel)) {
+                    std::fprintf(stderr, "Unrecognized -K option: %s\n", optarg);
+                    return 2;
+                }
+                break;
             case 'h': usage(); return 1;
             default:
                 std::fprintf(stderr, "\nUnrecognized option!\n");
@@ -159,6 +172,7 @@ inline void printConfig(const Config& c) {
     }
     std::printf("    Number of initial iterations to remove edges: %u\n", c.numEdgeRemoveIter);
     std::printf("    Recount strategy: %s\n", kRecounts[c.recount]);
+    if (c.countKernel != KT_COUNT_AUTO) std::printf("    Count kernel: %s\n", kCountKernels[c.countKernel]);
 }
 
 inline void printConfigAsCSV(const Config& c) {

@@ -120,6 +120,7 @@ int main(int argc, char** argv) {
     v.block_size = c.blockSize;
     v.tiles_per_dim = c.numTiles;
     v.num_parallel_tiles = c.numParallelTiles;
+    v.count_kernel = (uint32_t)c.countKernel;
     Timer t(c.verbosity >= 1);
     t.start();
     kt_plan* plan = nullptr;

@@ -87,3 +87,20 @@ def test_cli_binary_edge_cache(small_graphs, tmp_path):
     tau_o, _, _ = O.decompose(n, s, d)
     _, ts, td, tt = K.read_truss_tsv(str(out_bin))
     assert np.array_equal(ts, s) and np.array_equal(td, d) and np.array_equal(tt, tau_o)
+
+
+@pytest.mark.parametrize("kernel", ["delta", "search", "merge"])
+def test_cli_count_kernel_option(small_graphs, tmp_path, kernel):
+    """-K picks the support-pass kernel: the truss file equals the default
+    kernel's byte for byte; an unknown name exits with status 2."""
+    n, s, d = small_graphs["plc300"]
+    graph = tmp_path / "g.tsv"
+    K.write_tsv(str(graph), s, d)
+    ref, out = tmp_path / "ref.tsv", tmp_path / "out.tsv"
+    r = run_cli("-g", str(graph), "-D", "-p", "gpu", "-o", str(ref), "-c", "none", "-v", "0")
+    assert r.returncode == 0, r.stderr
+    r = run_cli("-g", str(graph), "-D", "-K", kernel, "-p", "gpu", "-o", str(out), "-c", "none", "-v", "1")
+    assert r.returncode == 0, r.stderr
+    assert f"Count kernel: {kernel}" in r.stdout
+    assert out.read_bytes() == ref.read_bytes()
+    assert run_cli("-K", "bogus").returncode == 2
