@@ -345,10 +345,11 @@ int read_edges_bin(const char* path, uint32_t* n, uint64_t* m, uint32_t** src, u
         std::free(d);
         return fail(KT_ERR_NOMEM, "edge arrays");
     }
-    if (hm) {
-        std::memcpy(s, f.p + kBinHeader, hm * 4);
-        std::memcpy(d, f.p + kBinHeader + hm * 4, hm * 4);
-    }
+    // slices copied by all threads (the mapped pages fault in in parallel)
+    parallel_for(hm, 1 << 20, [&](size_t b, size_t e) {
+        std::memcpy(s + b, f.p + kBinHeader + b * 4, (e - b) * 4);
+        std::memcpy(d + b, f.p + kBinHeader + (hm + b) * 4, (e - b) * 4);
+    });
     if (edge_checksum(hm, s, d) != sum) {
         std::free(s);
         std::free(d);
