@@ -4,7 +4,8 @@
 // parseArgs (config.h:163-224 there); additions: -D (full decomposition),
 // -s (synthetic input instead of -g), -c none (skip verification), -B (write
 // the input as a binary edge cache; -g reads such a cache by its magic), -K
-// (support-pass kernel, KT_COUNT_*; default auto, the reference's count).
+// (support-pass kernel, KT_COUNT_*; default auto, the reference's count), -M
+// (the learned cost model's per-phase pick among its candidates).
 // Unlike the reference (which prints and exits with status 0 on a bad
 // option), a bad option exits with status 2.
 #pragma once
@@ -43,6 +44,7 @@ struct Config {
     const char* synthetic = nullptr;  // rmat:SCALE:EF:SEED | er:N:DEG:SEED | cl:N:GAMMA:DEG:SEED
     const char* binCacheName = nullptr;  // -B: binary edge cache written from the input
     int countKernel = KT_COUNT_AUTO;     // -K
+    const char* modelFileName = nullptr;  // -M: cost model (picker/count_picker.ktm)
 };
 
 inline void usage() {
@@ -74,6 +76,8 @@ inline void usage() {
         "    -K <countKernel>         support passes: auto (default) | row | search | merge |\n"
         "                             delta (maintained supports; directed-by-degree or\n"
         "                             directed-by-index, coocsr, recount all)\n"
+        "    -M <modelFile>           per-phase pick of the support-pass kernel by the learned\n"
+        "                             cost model (picker/count_picker.ktm)\n"
         "    -s <spec>                synthetic input instead of -g: rmat:SCALE:EF:SEED,\n"
         "                             er:N:AVGDEG:SEED or cl:N:GAMMA:AVGDEG:SEED\n"
         "    -h                       help\n\n");
@@ -97,7 +101,7 @@ static const char* const kProcessors[] = {"cpu", "gpu"};
 
 inline int parseArgs(int argc, char** argv, Config& c) {
     int opt;
-    while ((opt = getopt(argc, argv, "g:k:o:w:x:c:p:d:f:e:r:b:t:y:v:hDs:B:K:")) >= 0) {
+    while ((opt = getopt(argc, argv, "g:k:o:w:x:c:p:d:f:e:r:b:t:y:v:hDs:B:K:M:")) >= 0) {
         int e = 0;
         switch (opt) {
             case 'g': c.graphFileName = optarg; break;
@@ -139,6 +143,7 @@ inline int parseArgs(int argc, char** argv, Config& c) {
             case 'D': c.decompose = true; break;
             case 's': c.synthetic = optarg; break;
             case 'B': c.binCacheName = optarg; break;
+            case 'M': c.modelFileName = optarg; break;
             case 'K':
                 if (!parse_enum(optarg, kCountKernels, 5, &c.countKernel)) {
                     std::fprintf(stderr, "Unrecognized -K option: %s\n", optarg);
@@ -173,6 +178,7 @@ inline void printConfig(const Config& c) {
     std::printf("    Number of initial iterations to remove edges: %u\n", c.numEdgeRemoveIter);
     std::printf("    Recount strategy: %s\n", kRecounts[c.recount]);
     if (c.countKernel != KT_COUNT_AUTO) std::printf("    Count kernel: %s\n", kCountKernels[c.countKernel]);
+    if (c.modelFileName) std::printf("    Count kernel per phase: cost model %s\n", c.modelFileName);
 }
 
 inline void printConfigAsCSV(const Config& c) {

@@ -68,6 +68,43 @@ int make_synthetic(const char* spec, uint32_t* n, uint64_t* m, uint32_t** src, u
 
 }  // namespace
 
+// Cost model written by learn.export_model_bin, little-endian: "KTMODEL1",
+// u32 candidates C, u32 parameter count P, C x (kernel, block, cap, chunk)
+// u32, P float32 parameters, KT_MLP_IN lower then upper input bounds.  Sets
+// the plan's candidates and its per-phase pick; the model is returned for
+// kt_mlp_destroy.
+static int load_model(const char* path, kt_plan* plan, kt_mlp** out) {
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) {
+        std::fprintf(stderr, "cannot open %s\n", path);
+        return KT_ERR_IO;
+    }
+    char magic[8];
+    uint32_t hdr[2] = {0, 0};
+    bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, "KTMODEL1", 8) == 0 &&
+              std::fread(hdr, 4, 2, fp) == 2;
+    const uint32_t C = hdr[0], P = hdr[1];
+    ok = ok && C >= 1 && C <= KT_MAX_CANDIDATES && C <= KT_MLP_MAX_OUT && P < (1u << 24);
+    std::vector<uint32_t> raw(ok ? 4 * (size_t)C : 0);
+    std::vector<float> params(ok ? P : 0), lo(KT_MLP_IN), hi(KT_MLP_IN);
+    ok = ok && std::fread(raw.data(), 4, raw.size(), fp) == raw.size() &&
+         std::fread(params.data(), 4, P, fp) == P && std::fread(lo.data(), 4, KT_MLP_IN, fp) == KT_MLP_IN &&
+         std::fread(hi.data(), 4, KT_MLP_IN, fp) == KT_MLP_IN;
+    std::fclose(fp);
+    if (!ok) {
+        std::fprintf(stderr, "%s is not a cost model file\n", path);
+        return KT_ERR_IO;
+    }
+    std::vector<kt_count_config> cands(C);
+    for (uint32_t i = 0; i < C; ++i) cands[i] = {raw[4 * i], raw[4 * i + 1], raw[4 * i + 2], raw[4 * i + 3]};
+    int rc;
+    if ((rc = kt_mlp_create(C, 1, out))) return rc;
+    if ((rc = kt_mlp_set_params(*out, params.data(), P))) return rc;
+    if ((rc = kt_mlp_set_input_range(*out, lo.data(), hi.data()))) return rc;
+    if ((rc = kt_plan_set_candidates(plan, cands.data(), C))) return rc;
+    return kt_plan_set_pick(plan, KT_PICK_MODEL, *out);
+}
+
 int main(int argc, char** argv) {
     Config c;
     const int prc = parseArgs(argc, argv, c);
@@ -125,6 +162,8 @@ int main(int argc, char** argv) {
     t.start();
     kt_plan* plan = nullptr;
     if ((rc = kt_plan_create(graph, &v, &plan))) return report(1, "kt_plan_create");
+    kt_mlp* model = nullptr;
+    if (c.modelFileName && (rc = load_model(c.modelFileName, plan, &model))) return report(1, "loading the cost model");
     t.stopAndPrint("    Relabel, orient, build COO/CSR and copy to GPU");
 
     std::vector<uint32_t> out_src(2 * m_und + 1), out_dst(2 * m_und + 1), tau;
@@ -212,6 +251,7 @@ int main(int argc, char** argv) {
     // Free memory
     timer.printAndStart("Cleaning up\n");
     kt_plan_destroy(plan);
+    if (model) kt_mlp_destroy(model);
     kt_graph_destroy(graph);
     kt_free(src);
     kt_free(dst);

@@ -34,6 +34,7 @@ sys.path.insert(0, HERE)
 import ktruss_amd as K  # noqa: E402
 
 MODEL_PATH = os.path.join(HERE, "picker", "count_picker.npz")
+MODEL_BIN_PATH = os.path.join(HERE, "picker", "count_picker.ktm")
 
 # Candidate support configurations; index 0 is the default variant's own.
 # "delta" maintains the supports across deletions instead of recounting
@@ -175,6 +176,27 @@ def save_model(model, path=MODEL_PATH, info=None):
     lo, hi = model.get_input_range()
     np.savez(path, params=model.get_params(), candidates=cands, lo=lo, hi=hi,
              info=np.array(json.dumps(info or {})))
+
+
+def export_model_bin(npz_path=MODEL_PATH, out_path=MODEL_BIN_PATH):
+    """The model as the ktruss command line reads it (-M), little-endian:
+    "KTMODEL1", u32 candidates C, u32 parameter count P, C x (kernel, block,
+    cap, chunk) u32, P float32 parameters, then the KT_MLP_IN lower and upper
+    input bounds as float32."""
+    with np.load(npz_path, allow_pickle=False) as z:
+        params = z["params"].astype("<f4")
+        cands = np.asarray(z["candidates"]).astype("<u4")
+        lo = z["lo"].astype("<f4")
+        hi = z["hi"].astype("<f4")
+    c4 = np.zeros((len(cands), 4), "<u4")
+    c4[:, :cands.shape[1]] = cands
+    with open(out_path, "wb") as f:
+        f.write(b"KTMODEL1")
+        f.write(np.array([len(cands), len(params)], "<u4").tobytes())
+        f.write(c4.tobytes())
+        f.write(params.tobytes())
+        f.write(lo.tobytes())
+        f.write(hi.tobytes())
 
 
 def load_model(path=MODEL_PATH):

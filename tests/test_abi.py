@@ -302,3 +302,26 @@ def test_binary_edge_cache_format_and_errors(tmp_path):
         K.read_edges_bin(p)
     assert "not a binary edge file" in str(e.value)
     assert not K.is_edges_bin(str(tmp_path / "missing.ktb"))
+
+
+def test_cost_model_export_for_the_command_line(tmp_path):
+    """learn.export_model_bin writes the picker's model in the layout the
+    ktruss command line reads (-M); the committed file is the current model."""
+    import learn
+    p = str(tmp_path / "m.ktm")
+    learn.export_model_bin(learn.MODEL_PATH, p)
+    raw = open(p, "rb").read()
+    assert raw == open(learn.MODEL_BIN_PATH, "rb").read()
+    with np.load(learn.MODEL_PATH, allow_pickle=False) as z:
+        params, cands, lo, hi = z["params"], z["candidates"], z["lo"], z["hi"]
+    C, P = struct.unpack("<II", raw[8:16])
+    assert raw[:8] == b"KTMODEL1" and (C, P) == (len(cands), len(params))
+    o = 16
+    c4 = np.frombuffer(raw[o:o + 16 * C], "<u4").reshape(C, 4)
+    o += 16 * C
+    assert np.array_equal(c4[:, :3], cands) and not c4[:, 3].any()
+    assert np.array_equal(np.frombuffer(raw[o:o + 4 * P], "<f4"), params)
+    o += 4 * P
+    assert np.array_equal(np.frombuffer(raw[o:o + 64], "<f4"), lo)
+    assert np.array_equal(np.frombuffer(raw[o + 64:o + 128], "<f4"), hi)
+    assert len(raw) == o + 128

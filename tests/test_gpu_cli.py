@@ -104,3 +104,22 @@ def test_cli_count_kernel_option(small_graphs, tmp_path, kernel):
     assert f"Count kernel: {kernel}" in r.stdout
     assert out.read_bytes() == ref.read_bytes()
     assert run_cli("-K", "bogus").returncode == 2
+
+
+def test_cli_cost_model_pick(small_graphs, tmp_path):
+    """-M: the learned cost model picks each phase's support-pass kernel; the
+    truss file equals the default kernel's; a file that is not a model fails."""
+    import learn
+    n, s, d = small_graphs["plc300"]
+    graph = tmp_path / "g.tsv"
+    K.write_tsv(str(graph), s, d)
+    ref, out = tmp_path / "ref.tsv", tmp_path / "out.tsv"
+    r = run_cli("-g", str(graph), "-D", "-p", "gpu", "-o", str(ref), "-c", "none", "-v", "0")
+    assert r.returncode == 0, r.stderr
+    r = run_cli("-g", str(graph), "-D", "-M", learn.MODEL_BIN_PATH, "-p", "gpu", "-o", str(out), "-c", "none",
+                "-v", "1")
+    assert r.returncode == 0, r.stderr
+    assert "cost model" in r.stdout
+    assert out.read_bytes() == ref.read_bytes()
+    r = run_cli("-g", str(graph), "-D", "-M", str(graph), "-p", "gpu", "-c", "none", "-v", "0")
+    assert r.returncode != 0 and "not a cost model" in r.stderr
