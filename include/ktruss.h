@@ -148,7 +148,8 @@ void kt_free(void* p);
  * symmetric input, kernel_h.cu:24), numbers the undirected edges and keeps the
  * map from every input entry to its edge.  Inputs of 2^16 entries or more
  * are checked and numbered on the device once kt_init (or a plan) has
- * initialised it; the error then names the lowest offending entry. */
+ * initialised it, others on the host; either way an error names the lowest
+ * offending entry. */
 int kt_graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                     uint32_t flags, kt_graph** out);
 void kt_graph_destroy(kt_graph* g);

@@ -169,27 +169,34 @@ int graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
         *out = g;
         return KT_OK;
     }
-    // range and self-loop checks (graph.cu:54)
-    std::atomic<int> bad{0};
-    std::atomic<uint64_t> bad_at{0};
+    // range and self-loop checks (graph.cu:54); like the device checks, the
+    // lowest offending entry is reported (a range error before a self-loop at
+    // the same entry)
+    std::atomic<uint64_t> bad_range{~0ull}, bad_loop{~0ull};
+    auto atomic_min = [](std::atomic<uint64_t>& a, uint64_t v) {
+        uint64_t cur = a.load();
+        while (v < cur && !a.compare_exchange_weak(cur, v)) {
+        }
+    };
     parallel_for(m, 1 << 16, [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) {
-            int code = 0;
-            if (src[i] >= n || dst[i] >= n) code = KT_ERR_RANGE;
-            else if (src[i] == dst[i]) code = KT_ERR_SELF_LOOP;
-            if (code) {
-                int expect = 0;
-                if (bad.compare_exchange_strong(expect, code)) bad_at = i;
+            if (src[i] >= n || dst[i] >= n) {
+                atomic_min(bad_range, i);
+                return;
+            }
+            if (src[i] == dst[i]) {
+                atomic_min(bad_loop, i);
                 return;
             }
         }
     });
-    if (bad.load()) {
+    if (bad_range.load() != ~0ull || bad_loop.load() != ~0ull) {
+        const bool range = bad_range.load() <= bad_loop.load();
+        const uint64_t i = range ? bad_range.load() : bad_loop.load();
         char buf[160];
-        const uint64_t i = bad_at.load();
         std::snprintf(buf, sizeof buf, "entry %llu = (%u, %u) with n = %u",
                       (unsigned long long)i, src[i], dst[i], n);
-        return fail(bad.load(), buf);
+        return fail(range ? KT_ERR_RANGE : KT_ERR_SELF_LOOP, buf);
     }
     Bucketed B;
     bucket_by_min(n, m, src, dst, B);
@@ -201,49 +208,43 @@ int graph_create(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* ds
     // per bucket: validate and count unique edges
     const bool sym = flags == KT_INPUT_SYMMETRIC;
     std::vector<uint64_t> nuniq((size_t)n + 1, 0);
-    std::atomic<int> err{0};
-    std::atomic<uint64_t> err_at{0};
+    // every pair group (same (min, max)) is checked; the reported entry is
+    // the lowest over the offending groups, each group naming its first entry
+    // in (direction, entry) order (a repeated pair of a pair input: its lowest
+    // entry), as the device checks do
+    std::atomic<uint64_t> bad_asym{~0ull}, bad_dup{~0ull};
     parallel_for(n, 4096, [&](size_t v0, size_t v1) {
         for (size_t v = v0; v < v1; ++v) {
             const uint64_t b = B.off[v], e = B.off[v + 1];
             uint64_t c = 0;
-            if (sym) {
-                for (uint64_t p = b; p < e;) {
-                    const uint64_t hi = B.key[p] >> 1;
-                    uint64_t q = p;
-                    while (q < e && (B.key[q] >> 1) == hi) ++q;
-                    const uint64_t cnt = q - p;
-                    int code = 0;
+            for (uint64_t p = b; p < e;) {
+                const uint64_t hi = B.key[p] >> 1;
+                uint64_t q = p;
+                while (q < e && (B.key[q] >> 1) == hi) ++q;
+                const uint64_t cnt = q - p;
+                if (sym) {
                     if (cnt == 2 && (B.key[p] & 1) == 0 && (B.key[p + 1] & 1) == 1) ++c;
-                    else if (cnt == 1) code = KT_ERR_ASYMMETRIC;
-                    else code = KT_ERR_DUPLICATE;
-                    if (code) {
-                        int expect = 0;
-                        if (err.compare_exchange_strong(expect, code)) err_at = B.idx[p];
-                        return;
-                    }
-                    p = q;
-                }
-            } else {
-                for (uint64_t p = b; p < e; ++p) {
-                    if (p + 1 < e && (B.key[p] >> 1) == (B.key[p + 1] >> 1)) {
-                        int expect = 0;
-                        if (err.compare_exchange_strong(expect, KT_ERR_DUPLICATE)) err_at = B.idx[p];
-                        return;
-                    }
+                    else atomic_min(cnt == 1 ? bad_asym : bad_dup, B.idx[p]);
+                } else if (cnt == 1) {
                     ++c;
+                } else {
+                    uint64_t lo = B.idx[p];
+                    for (uint64_t r = p + 1; r < q; ++r) lo = std::min<uint64_t>(lo, B.idx[r]);
+                    atomic_min(bad_dup, lo);
                 }
+                p = q;
             }
             nuniq[v + 1] = c;
         }
     });
-    if (err.load()) {
-        const uint64_t i = err_at.load();
+    if (bad_asym.load() != ~0ull || bad_dup.load() != ~0ull) {
+        const bool asym = bad_asym.load() <= bad_dup.load();
+        const uint64_t i = asym ? bad_asym.load() : bad_dup.load();
         char buf[160];
         std::snprintf(buf, sizeof buf, "entry %llu = (%u, %u)%s", (unsigned long long)i, src[i],
-                      dst[i], err.load() == KT_ERR_ASYMMETRIC ? " has no reverse entry" : " repeats");
+                      dst[i], asym ? " has no reverse entry" : " repeats");
         delete g;
-        return fail(err.load(), buf);
+        return fail(asym ? KT_ERR_ASYMMETRIC : KT_ERR_DUPLICATE, buf);
     }
     for (uint32_t v = 0; v < n; ++v) nuniq[v + 1] += nuniq[v];
     g->m = nuniq[n];

@@ -325,3 +325,36 @@ def test_cost_model_export_for_the_command_line(tmp_path):
     assert np.array_equal(np.frombuffer(raw[o:o + 64], "<f4"), lo)
     assert np.array_equal(np.frombuffer(raw[o + 64:o + 128], "<f4"), hi)
     assert len(raw) == o + 128
+
+
+def test_host_validation_reports_lowest_bad_entry():
+    """Inputs checked on the host (under 2^16 entries, or before the device is
+    initialised) name the lowest offending entry, as the device checks do,
+    whichever thread finds which error first."""
+    n = 60000
+    rng = np.random.default_rng(11)
+    key = np.unique(rng.integers(0, n, 40000) * n + rng.integers(0, n, 40000))
+    a, b = (key // n).astype(np.uint32), (key % n).astype(np.uint32)
+    keep = a != b
+    a, b = np.minimum(a, b)[keep][:15000], np.maximum(a, b)[keep][:15000]
+    s = np.concatenate([a, b])
+    d = np.concatenate([b, a])
+    # an entry without its reverse late in the input on a low vertex, a
+    # repeated pair early in the input on a high vertex
+    lo_v = int(np.argmin(a))
+    hi_v = int(np.argmax(a))
+    s2 = np.concatenate([[a[hi_v], b[hi_v]], s, [a[lo_v] if a[lo_v] != 0 else 1], ]).astype(np.uint32)
+    d2 = np.concatenate([[b[hi_v], a[hi_v]], d, [n - 1]]).astype(np.uint32)
+    if (s2[-1], d2[-1]) in set(zip(s.tolist(), d.tolist())) or s2[-1] == d2[-1]:
+        pytest.skip("planted entry collides")
+    with pytest.raises(K.KtError) as e:
+        K.Graph(n, s2, d2)
+    assert e.value.status == K.ERR_DUPLICATE
+    assert re.search(r"entry (\d+) = ", str(e.value)).group(1) == "0"
+    # a range error before a self-loop: the range error is reported
+    s3, d3 = s.copy(), d.copy()
+    s3[100], d3[100] = 7, 7
+    s3[50] = n
+    with pytest.raises(K.KtError) as e:
+        K.Graph(n, s3, d3)
+    assert e.value.status == K.ERR_RANGE and "entry 50 " in str(e.value)
