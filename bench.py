@@ -4,8 +4,9 @@
 A step is one full decomposition (k++ loop over the peel) of a synthetic
 R-MAT graph whose oriented COO+CSR layout is already resident in HBM; the
 step ends with the truss number of every edge resident in HBM.  Workload:
-R-MAT scale 20, edge factor 16, (a, b, c, d) = (.57, .19, .19, .05), seed 1,
-symmetrised, self-loops and duplicates removed.
+R-MAT scale 22 (the held-out graph the cost model is evaluated on), edge
+factor 16, (a, b, c, d) = (.57, .19, .19, .05), seed 1, symmetrised,
+self-loops and duplicates removed.
 
 The measured steps use the learned per-phase pick (the cost model in
 ktruss-explorer_amd/picker/ chooses, at every phase, among recount
@@ -198,7 +199,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["learned", "fixed"], default="learned",
@@ -297,15 +298,20 @@ def bench_rmat(args, K):
     # between dependent kernels); one more decomposition with them gives the
     # kernels' own durations for the roofline
     plan.set_timing(False)
+    setup_ms = 0.0  # DELTA adjacency lists, built by the plan's first run (outside ms_total)
     for i in range(args.warmup):
         st = plan.decompose()
-        log(f"warmup {i}: {st['ms_total']:.1f} ms device, k_max {st['k_max']}")
+        setup_ms += st["ms_setup"]
+        log(f"warmup {i}: {st['ms_total']:.1f} ms device, k_max {st['k_max']}"
+            + (f", set-up {st['ms_setup']:.1f} ms" if st["ms_setup"] else ""))
     K.device_synchronize()
     per_step = []
+    setup_in_timed = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = plan.decompose()
         per_step.append(st["ms_total"])
+        setup_in_timed += st["ms_setup"]
         log(f"step {i}: {st['ms_total']:.1f} ms device")
     K.device_synchronize()
     wall = time.perf_counter() - t0
@@ -363,6 +369,8 @@ def bench_rmat(args, K):
             "k_max": st["k_max"],
             "iterations": st["iterations"],
             "median_device_ms": statistics.median(per_step) if per_step else None,
+            "delta_setup_ms": round(setup_ms + setup_in_timed, 2),
+            "delta_setup_in_timed_steps": setup_in_timed > 0,
             "phase_picks": picks,
             "default_variant": default,
             "speedup_vs_default": (default["ms_per_step"] / ms_per_step) if default else None,

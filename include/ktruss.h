@@ -126,6 +126,9 @@ typedef struct kt_stats {
     double balg_delta_bytes;  /* DELTA algorithmic bytes: 4 (candidates + window
                                  entries read) + 32 per destroyed triangle          */
     uint64_t delta_live;      /* DELTA candidates whose edge was still alive       */
+    double ms_setup;          /* wall time of the one-time set-up this run did before
+                                 ms_total started (the DELTA adjacency lists, built on
+                                 a plan's first DELTA run); 0 when there was none  */
 } kt_stats;
 
 typedef struct kt_graph kt_graph; /* validated host-side edge set            */

@@ -120,7 +120,7 @@ __device__ __forceinline__ void finalize_list(const uint32_t* __restrict__ fin_d
 
 __device__ __forceinline__ void mark_flush(const uint32_t* s_out, uint32_t* s_cnt, uint32_t* s_base,
                                            uint32_t* dl, unsigned int* ndel) {
-    const uint32_t c = *s_cnt;  // read by every thread after a barrier: uniform
+    const uint32_t c = *s_cnt;  // every thread reads it after a barrier and before anyone changes it
     if (!c) return;
     if (threadIdx.x == 0) *s_base = atomicAdd(ndel, c);
     __syncthreads();
@@ -187,8 +187,13 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                 if (ldeg) atomicSub(ldeg + u, 1u);
             }
         }
+        // every thread reads the count between two barriers: a wave that
+        // went on to the next tile could otherwise add to it before a slower
+        // wave has read it, and the two would disagree on flushing
         __syncthreads();
-        if (s_cnt > kMarkBuf - kMarkTile) mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
+        const uint32_t c = s_cnt;
+        __syncthreads();
+        if (c > kMarkBuf - kMarkTile) mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
     }
     __syncthreads();
     mark_flush(s_out, &s_cnt, &s_base, dl, ndel);

@@ -969,7 +969,8 @@ uint32_t list_grid(uint32_t items) {
 }
 
 // Copies the workgroup's LDS list to the global list at a reserved offset
-// and empties it.  Every thread calls it after a barrier.
+// and empties it.  Every thread calls it after a barrier, with no addition to
+// the count pending.
 __device__ __forceinline__ void list_flush(const uint32_t* s_out, uint32_t* s_cnt, uint32_t* s_base,
                                            uint32_t* out, unsigned int* total) {
     const uint32_t c = *s_cnt;
@@ -1089,8 +1090,12 @@ __global__ __launch_bounds__(kBlock) void k_build_worklist(const uint32_t* __res
                 sup[e] = 0;
             }
         }
+        // the count is read between two barriers, so no wave's next
+        // additions can reach it before every wave has read it
         __syncthreads();
-        if (s_cnt > kListCap - kBlock) list_flush(s_out, &s_cnt, &s_base, wl, &ctr->wl_size);
+        const uint32_t c = s_cnt;
+        __syncthreads();
+        if (c > kListCap - kBlock) list_flush(s_out, &s_cnt, &s_base, wl, &ctr->wl_size);
     }
     __syncthreads();
     list_flush(s_out, &s_cnt, &s_base, wl, &ctr->wl_size);

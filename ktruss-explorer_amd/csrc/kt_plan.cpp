@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <deque>
 #include <new>
@@ -665,6 +666,7 @@ bool Plan::delta_capable() const {
 int Plan::ensure_delta() {
     if (delta_ready) return KT_OK;
     if (n >= KT_DYING) return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^30 vertices");
+    const auto t0 = std::chrono::steady_clock::now();
     int rc;
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4, Pb = ((size_t)n + 1) * 4;
     const uint32_t ct = dev::compact_tiles_for(2 * E0) + 2;
@@ -686,6 +688,7 @@ int Plan::ensure_delta() {
     if ((rc = restore_delta())) return rc;
     KT_HIP_TRY(hipStreamSynchronize(stream));
     delta_ready = true;
+    stats.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return KT_OK;
 }
 

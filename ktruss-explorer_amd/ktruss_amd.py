@@ -103,7 +103,7 @@ class Stats(ctypes.Structure):
                 ("delta_passes", ctypes.c_uint64), ("delta_edges", ctypes.c_uint64),
                 ("delta_elems", ctypes.c_uint64), ("delta_triangles", ctypes.c_uint64),
                 ("ms_delta", ctypes.c_double), ("balg_delta_bytes", ctypes.c_double),
-                ("delta_live", ctypes.c_uint64)]
+                ("delta_live", ctypes.c_uint64), ("ms_setup", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
