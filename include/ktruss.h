@@ -226,6 +226,34 @@ int kt_plan_info(const kt_plan* p, uint64_t* entries, uint32_t* max_row);
 int kt_plan_ktruss(kt_plan* p, uint32_t k, uint32_t* out_src, uint32_t* out_dst, uint64_t cap,
                    uint64_t* out_m, kt_stats* stats);
 
+/* One k of the directed COO+CSR peel, recount all, on a layout the caller
+ * holds in device memory, allocating nothing (no hipMalloc, no hipFree, no
+ * allocating library call).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- serving loops and harnesses
+ * that forbid hipMalloc.  It is kt_plan_ktruss without the plan: the layout
+ * is the caller's COOCSRGraph (srcIdx, dstIdx, srcPtr of graph.h:40-47) and
+ * the peel state (supports, counters) lives in a fixed device array of the
+ * library, about 32 MiB per device, held until the process ends.
+ *   ptr[n+1], src[E], dst[E]: device arrays of the current device, oriented by
+ *     id (src[e] < dst[e] < n, each undirected edge once), rows by source,
+ *     destinations strictly ascending per row (a degree-ordered relabelling
+ *     makes it the reference's directed-by-degree layout);
+ *   dst is also the output: entries outside the k-truss get bit 31 set (the
+ *     vertex id stays below it), the others are unchanged;
+ *   stream: a hipStream_t of the current device (NULL: the null stream); the
+ *     call returns after the stream has run its kernels (it reads a
+ *     deletion count back every iteration, so it cannot be captured);
+ *   out_live (optional): entries left in the k-truss.
+ * E above KT_NOALLOC_MAX_ENTRIES is KT_ERR_CAPACITY; a layout that breaks the
+ * rules above is KT_ERR_ARG naming the first offending entry or vertex,
+ * found on the device before any entry changes.  Every call on one device
+ * shares that device's array (each call clears what must start at zero), so
+ * calls that overlap on one device -- from two host threads or on two
+ * streams -- race and must not be made. */
+#define KT_NOALLOC_MAX_ENTRIES (1u << 23)
+int kt_ktruss_noalloc(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t* src, uint32_t* dst,
+                      uint32_t k, void* stream, uint64_t* out_live);
+
 /* Full decomposition (k++ loop over the same peel) on the device-resident
  * layout; truss numbers stay on the device.  stats->ms_total is the hipEvent
  * time from the resident oriented layout to the resident truss numbers. */

@@ -188,6 +188,11 @@ int kt_plan_ktruss(kt_plan* p, uint32_t k, uint32_t* out_src, uint32_t* out_dst,
     })
 }
 
+int kt_ktruss_noalloc(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t* src, uint32_t* dst,
+                      uint32_t k, void* stream, uint64_t* out_live) {
+    KT_GUARD(return kt::ktruss_noalloc(n, E, ptr, src, dst, k, (hipStream_t)stream, out_live);)
+}
+
 int kt_plan_decompose(kt_plan* p, kt_stats* stats) {
     if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
     KT_GUARD({

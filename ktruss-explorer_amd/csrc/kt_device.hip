@@ -985,8 +985,8 @@ __device__ __forceinline__ void list_flush(const uint32_t* s_out, uint32_t* s_cn
 }
 
 // mark_deleted_edges_kernel (kernel_ddc.cu:56-77 and its three twins): an
-// entry whose support is below k-2 dies; its truss number is k-1.  The
-// deletions are counted per workgroup in LDS and added once.
+// entry whose support is below k-2 dies; its truss number is k-1 (when tau is
+// given).  The deletions are counted per workgroup in LDS and added once.
 __global__ __launch_bounds__(kBlock) void k_mark_deleted(uint32_t* __restrict__ dst,
                                                          const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ eid,
@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_deleted(uint32_t* __restrict__ 
         if (!(v & KT_DEAD) && (!affected || state[e] == KT_DIRECTLY_AFFECTED) && sup[e] < k - 2) {
             ++mine;
             dst[e] = v | KT_DEAD;
-            tau[eid[e]] = k - 1;
+            if (tau) tau[eid[e]] = k - 1;  // no truss numbers kept: tau and eid are null
             if (affected) {
                 nflag[src[e]] = KT_DIRECTLY_AFFECTED;
                 nflag[v] = KT_DIRECTLY_AFFECTED;

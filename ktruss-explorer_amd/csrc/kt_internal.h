@@ -279,6 +279,11 @@ hipError_t build_adjacency_device(uint32_t n, uint32_t E, const uint32_t* src, c
                                   uint32_t* snbr, uint32_t* seid, uint32_t* sa, uint32_t* sb, hipStream_t s);
 }  // namespace dev
 
+// kt_ktruss_noalloc (kt_noalloc.hip): one k of the directed COO+CSR peel on
+// the caller's device layout, peel state in a fixed device array.
+int ktruss_noalloc(uint32_t n, uint32_t E, const uint32_t* ptr, const uint32_t* src, uint32_t* dst,
+                   uint32_t k, hipStream_t s, uint64_t* out_live);
+
 // the gfx950 device has been initialised (kt_init or a plan)
 bool device_initialized();
 // makes the engine's device current on the calling thread (initialising it

@@ -42,7 +42,7 @@ EXPORTS = [
     "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
     "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
-    "kt_plan_set_timing",
+    "kt_plan_set_timing", "kt_ktruss_noalloc",
     "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
     "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
     "kt_mlp_set_input_range", "kt_mlp_get_input_range",
@@ -222,6 +222,8 @@ def lib():
         L.kt_batch_set_timing.argtypes = [_vp, ctypes.c_int]
         L.kt_batch_truss_numbers.argtypes = [_vp, ctypes.c_uint32, _u32p, ctypes.c_uint64]
         L.kt_batch_graph_info.argtypes = [_vp, ctypes.c_uint32, _u32p]
+        L.kt_ktruss_noalloc.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_uint32,
+                                        _vp, _u64p]
         _lib = L
     return _lib
 
@@ -247,6 +249,20 @@ def _take_array(p, count):
 
 def init(device=0):
     _check(lib().kt_init(device), "kt_init")
+
+
+NOALLOC_MAX_ENTRIES = 1 << 23
+
+
+def ktruss_noalloc(n, num_entries, ptr, src, dst, k, stream=0):
+    """kt_ktruss_noalloc: one k of the directed COO+CSR peel on a device
+    layout the caller holds (device addresses ptr, src, dst as integers, dst
+    rewritten in place: entries outside the k-truss get bit 31), with no
+    device allocation.  Returns the number of entries left."""
+    live = ctypes.c_uint64(0)
+    _check(lib().kt_ktruss_noalloc(n, num_entries, ptr, src, dst, k, stream, ctypes.byref(live)),
+           "kt_ktruss_noalloc")
+    return live.value
 
 
 def device_synchronize():
