@@ -331,11 +331,12 @@ def bench_rmat(args, K):
     delta_rl = roofline_obj(acc["balg_delta_bytes"], acc["ms_delta"], acc["delta_passes"],
                             "k_delta (DELTA support update)")
     if os.path.exists(TRAFFIC_FILE):
+        # HBM bytes per launch measured by rocprof counters on this workload
         with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
+            t = json.load(f).get("by_scale", {}).get(str(args.scale), {})
         for rl in (count_rl, delta_rl):
             key = rl["kernel"].split()[0]
-            if t.get("workload_scale") == args.scale and key in t.get("hbm_bytes_per_launch", {}):
+            if args.edge_factor == 16 and args.seed == 1 and key in t.get("hbm_bytes_per_launch", {}):
                 rl["traffic"] = t["hbm_bytes_per_launch"][key]
     dominant = delta_rl if acc["ms_delta"] > acc["ms_count"] else count_rl
     other = count_rl if dominant is delta_rl else delta_rl
