@@ -273,6 +273,56 @@ int kt_plan_truss_numbers(kt_plan* p, uint32_t* tau, uint64_t count);
  * smaller). */
 int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len);
 
+/* Per-iteration log of the last run, KT_ITER_REC uint64 per iteration -- the
+ * quantities the reference prints per iteration at -v 2 (kernel_ddc.cu:126-198
+ * and its three twins):
+ *   [0] k, [1] iteration within the phase, [2] undirected edges deleted,
+ *   [3] entries recounted, [4] support-pass ns ("Triangle counting time"),
+ *   [5] mark ns ("Mark deleted edges"), [6] removal ns ("Remove deleted
+ *   edges", the first -e iterations), [7] stored entries after that removal
+ *   ("# edges remaining"), [8] affected-marking ns ("Mark affected edges"),
+ *   [9] removal ns after the phase's loop (kernel_ddc.cu:207-212; last
+ *   iteration of a phase only), [10] KT_ITER_* flags saying which of them
+ *   happened, [11] 0.
+ * Times are device times (hipEvents) when plan timing is on (the default),
+ * removal times host wall times around the removal (it reads the kept count
+ * back); with timing off, or for steps fused into another kernel (DELTA
+ * marks), the KT_ITER_*_TIMED flag is clear and the time is 0. */
+#define KT_ITER_REC 12
+enum {
+    KT_ITER_COUNTED = 1u,        /* a support pass ran (not an iteration 0 handed exact supports) */
+    KT_ITER_MARK_TIMED = 2u,     /* [5] holds the mark's time                           */
+    KT_ITER_REMOVED = 4u,        /* deleted entries were removed in the iteration ([6], [7]) */
+    KT_ITER_AFFECTED = 8u,       /* affected marking ran ([8])                          */
+    KT_ITER_FINAL_REMOVAL = 16u, /* deleted entries were removed after the loop ([9])   */
+    KT_ITER_COUNT_TIMED = 32u,   /* [4] holds the support pass's time                   */
+    KT_ITER_DELTA = 64u          /* the support pass was a DELTA update                 */
+};
+int kt_plan_iteration_log(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len);
+
+/* Wall times (ms, host clock around device work that is synchronised) of the
+ * stages of kt_plan_create and of the last kt_plan_ktruss, named after the
+ * reference's -v 1 lines for the same work (kernel.cu:19, 36;
+ * kernel_d.cu:20-84; kernel_ddc.cu:223, 249):
+ *   [0] relabel by degree (directed-by-degree; 0 otherwise)
+ *   [1] copy of the edge list to the GPU
+ *   [2] allocation of the peel state
+ *   [3] orientation (directed variants; 0 for undirected)
+ *   [4] build of the stored COO+CSR rows
+ *   [5] free of the edge list's device copy
+ *   [6] gathering the surviving entries (COO+CSR -> COO)
+ *   [7] adding the reverse entries (directed -> undirected COO; directed only)
+ *   [8] copy of the stored entries from the GPU
+ *   [9] restoring the caller's vertex labels */
+#define KT_PLAN_STAGES 10
+int kt_plan_stage_times(const kt_plan* p, double* ms, uint32_t n);
+/* kt_plan_destroy that times what it frees, in the reference's order
+ * (kernel_ddc.cu:254, kernel_d.cu:66, 72): ms[0] the working layout
+ * ("Deallocate COOCSR"), ms[1] the pristine copy of the layout ("Deallocate
+ * directed COO"), ms[2] the peel state and the rest ("Deallocate intermediate
+ * data structures").  ms may be NULL. */
+void kt_plan_destroy_timed(kt_plan* p, double* ms);
+
 /* ---- learned count-kernel picker -----------------------------------------
  * No counterpart in the reference (its variant is fixed per run).  A count
  * configuration is the part of a variant that can change between phases

@@ -68,6 +68,53 @@ int make_synthetic(const char* spec, uint32_t* n, uint64_t* m, uint32_t** src, u
 
 }  // namespace
 
+// The -v 2 lines of one run (kernel_ddc.cu:126-198, kernel_ddt.cu:135-207,
+// kernel_duc.cu:105-174, kernel_dut.cu:116-186; the undirected COO+CSR
+// variant says "Mark deleted edges time" and prints no "# edges remaining"),
+// from the plan's iteration log.  A decomposition heads each phase with its
+// k.  A DELTA iteration's support pass is its update; its mark is fused with
+// the next kernels and has no line.
+static void print_iterations(const kt_plan* plan, const Config& c) {
+    uint64_t len = 0;
+    kt_plan_iteration_log(plan, nullptr, 0, &len);
+    std::vector<uint64_t> lg((size_t)KT_ITER_REC * len);
+    kt_plan_iteration_log(plan, lg.data(), len, &len);
+    const bool duc = c.directedness == KT_UNDIRECTED && c.inputFormat == KT_COOCSR;
+    for (uint64_t r = 0; r < len; ++r) {
+        const uint64_t* x = &lg[(size_t)KT_ITER_REC * r];
+        const uint64_t fl = x[10];
+        if (c.decompose && x[1] == 0) std::printf("        k = %llu\n", (unsigned long long)x[0]);
+        std::printf("        Iteration %llu\n", (unsigned long long)x[1]);
+        if (fl & KT_ITER_COUNTED) Timer::print(x[4] * 1e-6, "            Triangle counting time");
+        if (fl & KT_ITER_MARK_TIMED)
+            Timer::print(x[5] * 1e-6, duc ? "            Mark deleted edges time" : "            Mark deleted edges");
+        if (fl & KT_ITER_REMOVED) {
+            Timer::print(x[6] * 1e-6, "            Remove deleted edges");
+            if (!duc) std::printf("                # edges remaining = %llu\n", (unsigned long long)x[7]);
+        }
+        if (fl & KT_ITER_AFFECTED) Timer::print(x[8] * 1e-6, "            Mark affected edges");
+        if (fl & KT_ITER_FINAL_REMOVAL) Timer::print(x[9] * 1e-6, "        Remove deleted edges");
+    }
+}
+
+// The -v 1 lines after the runs (kernel_ddc.cu:249-254 and twins,
+// kernel_d.cu:60-84, kernel.cu:36): plan stage times sm, frees fm.
+static void print_conversion_stages(const double* sm, const double* fm, const Config& c) {
+    const bool directed = c.directedness != KT_UNDIRECTED;
+    const bool tiled = c.inputFormat == KT_TILED_COOCSR;
+    if (!c.decompose) Timer::print(sm[6], tiled ? "    Convert tiled COOCSR to COO" : "    Convert COO/CSR to COO");
+    Timer::print(fm[0], tiled ? "    Deallocate tiled COOCSR" : "    Deallocate COOCSR");
+    if (directed) {
+        if (!c.decompose) Timer::print(sm[7], "    Convert directed COO to undirected COO");
+        Timer::print(fm[1], "    Deallocate directed COO");
+    }
+    Timer::print(directed ? fm[2] : fm[1] + fm[2], "    Deallocate intermediate data structures");
+    if (!c.decompose) Timer::print(sm[8], "    Copy COO truss from GPU");
+    Timer::print(sm[5], "    Deallocate COO graph and truss from GPU");
+    if (c.directedness == KT_DIRECTED_BY_DEGREE && !c.decompose)
+        Timer::print(sm[9], "    Restore original vertex labels (not optimized)");
+}
+
 // Cost model written by learn.export_model_bin, little-endian: "KTMODEL1",
 // u32 candidates C, u32 parameter count P, C x (kernel, block, cap, chunk)
 // u32, P float32 parameters, KT_MLP_IN lower then upper input bounds.  Sets
@@ -145,7 +192,9 @@ int main(int argc, char** argv) {
     timer.stopAndPrint("    Set up time");
     if (c.verbosity >= 1) printConfig(c);
 
-    // Compute
+    // Compute: the flow and -v 1 labels of kernel.cu:9-39, kernel_d.cu:11-86
+    // and the variant wrappers (kernel_ddc.cu:216-256 and twins), each label
+    // printed with the time of the plan stage that does that work
     Timer total(c.verbosity >= 1);
     total.printAndStart("Computing\n");
     kt_variant v;
@@ -158,36 +207,38 @@ int main(int argc, char** argv) {
     v.tiles_per_dim = c.numTiles;
     v.num_parallel_tiles = c.numParallelTiles;
     v.count_kernel = (uint32_t)c.countKernel;
-    Timer t(c.verbosity >= 1);
-    t.start();
     kt_plan* plan = nullptr;
     if ((rc = kt_plan_create(graph, &v, &plan))) return report(1, "kt_plan_create");
     kt_mlp* model = nullptr;
     if (c.modelFileName && (rc = load_model(c.modelFileName, plan, &model))) return report(1, "loading the cost model");
-    t.stopAndPrint("    Relabel, orient, build COO/CSR and copy to GPU");
+    const bool directed = c.directedness != KT_UNDIRECTED;
+    const bool tiled = c.inputFormat == KT_TILED_COOCSR;
+    const bool by_degree = c.directedness == KT_DIRECTED_BY_DEGREE;
+    double sm[KT_PLAN_STAGES] = {};
+    kt_plan_stage_times(plan, sm, KT_PLAN_STAGES);
+    if (c.verbosity >= 1) {
+        if (by_degree) Timer::print(sm[0], "    Relabel vertices (not optimized)");          // kernel.cu:19
+        Timer::print(sm[1], "    Copy COO graph to GPU");                                      // kernel_d.cu:20
+        Timer::print(sm[2], "    Allocate intermediate data structures");                      // kernel_d.cu:25
+        if (directed) Timer::print(sm[3], "    Convert undirected COO to directed COO");      // kernel_d.cu:33
+        // kernel_ddc.cu:223, kernel_ddt.cu:234, kernel_duc.cu:201, kernel_dut.cu:213
+        const char* build = directed ? (tiled ? "    Convert directed COO to tiled COOCSR (not optimized)"
+                                              : "    Convert directed COO to COOCSR (not optimized)")
+                                     : (tiled ? "    Convert undirected COO to tiled COOCSR (not optimized)"
+                                              : "    Convert undirected COO to COOCSR (not optimized)");
+        Timer::print(sm[4], build);
+    }
 
     std::vector<uint32_t> out_src(2 * m_und + 1), out_dst(2 * m_und + 1), tau;
     uint64_t out_m = 0;
     kt_stats st{};
+    Timer t(c.verbosity >= 1);
     for (unsigned i = 0; i < c.numWarmupRuns + c.numTimedRuns; ++i) {
         t.printAndStart("    Performing K-truss\n");
         if (c.decompose) rc = kt_plan_decompose(plan, &st);
         else rc = kt_plan_ktruss(plan, c.k, out_src.data(), out_dst.data(), out_src.size(), &out_m, &st);
         if (rc) return report(1, c.decompose ? "kt_plan_decompose" : "kt_plan_ktruss");
-        if (c.verbosity >= 2) {
-            uint64_t len = 0;
-            kt_plan_trace(plan, nullptr, 0, &len);
-            std::vector<uint64_t> tr(5 * len);
-            kt_plan_trace(plan, tr.data(), len, &len);
-            for (uint64_t r = 0; r < len; ++r) {
-                std::printf("        k = %llu, iteration %llu\n", (unsigned long long)tr[5 * r],
-                            (unsigned long long)tr[5 * r + 1]);
-                Timer::print(tr[5 * r + 4] * 1e-6, "            Triangle counting time");
-                std::printf("            # edges recounted = %llu, # edges deleted = %llu\n",
-                            (unsigned long long)tr[5 * r + 3], (unsigned long long)tr[5 * r + 2]);
-            }
-            Timer::print(st.ms_count, "            Triangle counting time (all passes)");
-        }
+        if (c.verbosity >= 2) print_iterations(plan, c);
         if (c.verbosity == 0 && i >= c.numWarmupRuns) {
             printConfigAsCSV(c);
             std::printf("%f ms\n", st.ms_total);
@@ -199,8 +250,11 @@ int main(int argc, char** argv) {
         }
     }
     if (c.decompose) {
+        Timer tt(c.verbosity >= 1);
+        tt.start();
         tau.resize(m_in);
         if ((rc = kt_plan_truss_numbers(plan, tau.data(), m_in))) return report(1, "kt_plan_truss_numbers");
+        tt.stopAndPrint("    Copy truss numbers from GPU");
         // the k-truss for verification: entries with truss number >= k, both directions
         out_m = 0;
         for (uint64_t e = 0; e < m_in; ++e) {
@@ -214,6 +268,14 @@ int main(int argc, char** argv) {
                 ++out_m;
             }
         }
+    }
+    // the results are on the host: the plan's device arrays go now, in the
+    // reference's order of frees
+    double fm[3] = {0, 0, 0};
+    kt_plan_destroy_timed(plan, fm);
+    plan = nullptr;
+    if (c.verbosity >= 1) {
+        print_conversion_stages(sm, fm, c);
     }
     total.stopAndPrint("    Total time");
 
@@ -250,7 +312,6 @@ int main(int argc, char** argv) {
 
     // Free memory
     timer.printAndStart("Cleaning up\n");
-    kt_plan_destroy(plan);
     if (model) kt_mlp_destroy(model);
     kt_graph_destroy(graph);
     kt_free(src);

@@ -17,6 +17,8 @@
 // Row starts are lower bounds of each vertex in the sorted keys.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "kt_internal.h"
@@ -170,7 +172,18 @@ namespace dev {
 hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b, int directedness,
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
-                               hipStream_t s) {
+                               hipStream_t s, double* stage_ms) {
+    using clk = std::chrono::steady_clock;
+    clk::time_point t0 = clk::now();
+    // stage boundary: with stage_ms, wait for the stream and book the wall time
+    auto stage = [&](int i) -> hipError_t {
+        if (!stage_ms) return hipSuccess;
+        const hipError_t e = hipStreamSynchronize(s);
+        const clk::time_point t1 = clk::now();
+        stage_ms[i] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t0 = t1;
+        return e;
+    };
     const bool by_degree = directedness == KT_DIRECTED_BY_DEGREE, both = directedness == KT_UNDIRECTED;
     const uint64_t E64 = both ? 2ull * m : (uint64_t)m;
     if (E64 >= 0xFFFFFFFFull) return hipErrorInvalidValue;
@@ -199,6 +212,7 @@ hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const 
         hipLaunchKernelGGL(k_invert, dim3(grid(n)), dim3(kBlock), 0, s, new2old, old2new.as<uint32_t>(), n);
         KT_BUILD_TRY(hipGetLastError());
     }
+    KT_BUILD_TRY(stage(0));
     // entries sorted by (source, destination)
     Tmp k0, k1, v0, v1;
     KT_BUILD_TRY(k0.alloc((size_t)E * 8));
@@ -210,6 +224,9 @@ hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const 
                            by_degree ? old2new.as<uint32_t>() : (const uint32_t*)nullptr, m, both ? 1 : 0,
                            k0.as<uint64_t>(), v0.as<uint32_t>());
         KT_BUILD_TRY(hipGetLastError());
+    }
+    KT_BUILD_TRY(stage(1));
+    if (m) {
         KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), E,
                                 32 + (bits_for(n) ? bits_for(n) : 1u), s));
         hipLaunchKernelGGL(k_unpack_layout, dim3(grid(E)), dim3(kBlock), 0, s, k1.as<uint64_t>(),
@@ -222,6 +239,7 @@ hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const 
     KT_BUILD_TRY(hipMemsetAsync(max_row, 0, 4, s));
     hipLaunchKernelGGL(k_max_row, dim3(grid(n)), dim3(kBlock), 0, s, ptr, n, max_row);
     KT_BUILD_TRY(hipGetLastError());
+    KT_BUILD_TRY(stage(2));
     return hipStreamSynchronize(s);
 }
 

@@ -162,6 +162,12 @@ int kt_plan_create(const kt_graph* g, const kt_variant* v, kt_plan** out) {
 
 void kt_plan_destroy(kt_plan* p) { delete p; }
 
+void kt_plan_destroy_timed(kt_plan* p, double* ms) {
+    if (!p) return;
+    p->impl.release_timed(ms);
+    delete p;
+}
+
 int kt_plan_info(const kt_plan* p, uint64_t* entries, uint32_t* max_row) {
     if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
     if (entries) *entries = p->impl.E0;
@@ -217,6 +223,24 @@ int kt_plan_trace(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len)
         for (uint64_t i = 0; i < c; ++i)
             for (int j = 0; j < 5; ++j) recs[5 * i + j] = tr[i][j];
     }
+    return KT_OK;
+}
+
+int kt_plan_iteration_log(const kt_plan* p, uint64_t* recs, uint64_t cap, uint64_t* len) {
+    if (!p || !len) return kt::fail(KT_ERR_ARG, "NULL argument");
+    const auto& tr = p->impl.trace;
+    *len = tr.size();
+    if (recs) {
+        const uint64_t c = std::min<uint64_t>(cap, tr.size());
+        for (uint64_t i = 0; i < c; ++i)
+            for (int j = 0; j < KT_ITER_REC; ++j) recs[(size_t)KT_ITER_REC * i + j] = tr[i][j];
+    }
+    return KT_OK;
+}
+
+int kt_plan_stage_times(const kt_plan* p, double* ms, uint32_t n) {
+    if (!p || (n && !ms)) return kt::fail(KT_ERR_ARG, "NULL argument");
+    for (uint32_t i = 0; i < n; ++i) ms[i] = i < KT_PLAN_STAGES ? p->impl.stage_ms[i] : 0.0;
     return KT_OK;
 }
 

@@ -257,11 +257,12 @@ hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_
 // degrees, at most max_degree; the others keep ids): ptr (n + 1), src / dst
 // (m entries, or 2m for KT_UNDIRECTED; rows by source, destinations
 // ascending), new2old (n), pos (m: pair -> its entry in the higher row) and
-// *max_row (the longest row).
+// *max_row (the longest row).  stage_ms (optional, 3 entries): wall times of
+// the relabel, the orientation and the row build, each synchronised.
 hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b, int directedness,
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
-                               hipStream_t s);
+                               hipStream_t s, double* stage_ms = nullptr);
 // Input validation of m entries over n vertices (kt_graph_create's checks):
 // res.status is KT_OK or the error of the lowest offending entry res.at;
 // for a valid input, the unique edges (ua < ub, sorted), every entry's edge

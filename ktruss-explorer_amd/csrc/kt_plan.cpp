@@ -93,10 +93,34 @@ Plan::~Plan() {
             if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : ev_sw)
         if (ev) (void)hipEventDestroy(ev);
-    for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1})
+    for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1, &ev_m0, &ev_m1, &ev_a0, &ev_a1})
         if (*ev) (void)hipEventDestroy(*ev);
     if (stream) (void)hipStreamDestroy(stream);
     if (graph) graph_release(graph);
+}
+
+void Plan::release_timed(double* ms) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    auto lap = [&](int i) {
+        const auto t1 = clk::now();
+        if (ms) ms[i] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t0 = t1;
+    };
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int b = 0; b < 2; ++b)
+        for (DBuf* x : {&ptr[b], &src[b], &dst[b], &eid[b], &sup[b]}) x->release();
+    lap(0);
+    for (DBuf* x : {&ptr0, &src0, &dst0}) x->release();
+    lap(1);
+    for (DBuf* x : {&state, &nflag, &wl, &tau, &tiles, &xs, &ctr, &feat_acc, &ctr_scratch, &feat_log, &stripes,
+                    &stripes_x, &model, &sa, &sb, &sptr0, &snbr0, &seid0, &sa0, &sb0, &dl, &dl_alt, &dl3,
+                    &sup_snap, &dscratch, &ctsum, &cxs, &ldeg})
+        x->release();
+    for (int b = 0; b < 2; ++b)
+        for (DBuf* x : {&sptr_b[b], &snbr_b[b], &seid_b[b]}) x->release();
+    delta_ready = false;
+    lap(2);
 }
 
 int Plan::create(const kt_graph* g, const kt_variant* var) {
@@ -139,7 +163,8 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     query_factor = 8;
 
     KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1}) KT_HIP_TRY(hipEventCreate(ev));
+    for (auto* ev : {&ev_start, &ev_stop, &ev_c0, &ev_c1, &ev_m0, &ev_m1, &ev_a0, &ev_a1})
+        KT_HIP_TRY(hipEventCreate(ev));
     KT_HIP_TRY(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocDefault));
     std::memset(h_ctr, 0, sizeof(DevCounters));
     for (auto& ev : ev_sw) KT_HIP_TRY(hipEventCreate(&ev));
@@ -150,6 +175,10 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
                              hipHostMallocMapped | hipHostMallocCoherent));
     KT_HIP_TRY(hipHostGetDevicePointer((void**)&d_ra, h_ra, 0));
 
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
+    std::fill(stage_ms, stage_ms + KT_PLAN_STAGES, 0.0);
+    auto t0 = clk::now();
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
     const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
     if ((rc = ptr0.alloc(Pb)) || (rc = src0.alloc(Eb)) || (rc = dst0.alloc(Eb))) return rc;
@@ -165,10 +194,12 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = stripes_x.alloc(kStripeBytes)) ||
         (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))))
         return rc;
+    stage_ms[2] = ms_since(t0);  // the peel state (Info, info.cu:22-35) and the layouts
     {
         const uint32_t m = (uint32_t)g->m;
         const size_t Mb = (size_t)std::max<uint32_t>(m, 1) * sizeof(uint32_t);
         DBuf da, db, ddeg, dn2o, dpos, dmax;
+        t0 = clk::now();
         if ((rc = da.alloc(Mb)) || (rc = db.alloc(Mb)) || (rc = ddeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
             (rc = dn2o.alloc(std::max<size_t>(Pb - 4, 4))) || (rc = dpos.alloc(Mb)) || (rc = dmax.alloc(4)))
             return rc;
@@ -177,15 +208,26 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
             KT_HIP_TRY(hipMemcpyAsync(db.p, g->ub.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
         }
         if (n) KT_HIP_TRY(hipMemcpyAsync(ddeg.p, g->degree.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipStreamSynchronize(stream));
+        stage_ms[1] = ms_since(t0);
+        double build_ms[3] = {0, 0, 0};  // relabel, orientation, rows
         KT_HIP_TRY(dev::build_layout_device(n, m, da.as<uint32_t>(), db.as<uint32_t>(), v.directedness,
                                             ddeg.as<uint32_t>(), g->max_degree, ptr0.as<uint32_t>(),
                                             src0.as<uint32_t>(), dst0.as<uint32_t>(), dn2o.as<uint32_t>(),
-                                            dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream));
+                                            dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream, build_ms));
+        stage_ms[0] = build_ms[0];
+        stage_ms[3] = build_ms[1];
+        stage_ms[4] = build_ms[2];
+        t0 = clk::now();
         new2old.resize(n);
         pos_of_edge.resize(m);
         if (n) KT_HIP_TRY(hipMemcpy(new2old.data(), dn2o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
         if (m) KT_HIP_TRY(hipMemcpy(pos_of_edge.data(), dpos.p, (size_t)m * 4, hipMemcpyDeviceToHost));
         KT_HIP_TRY(hipMemcpy(&max_row, dmax.p, 4, hipMemcpyDeviceToHost));
+        stage_ms[4] += ms_since(t0);  // the rows' maps back to the edges
+        t0 = clk::now();
+        for (DBuf* b : {&da, &db, &ddeg, &dn2o, &dpos, &dmax}) b->release();
+        stage_ms[5] = ms_since(t0);
     }
     KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(dev::fill_u32(stripes_x.as<uint32_t>(), 0, kStripeBytes / 4, stream));
@@ -453,12 +495,21 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
             const uint32_t s = h_ctr->min_sup;
             if (s != 0xFFFFFFFFu && s + 3 > k) k = s + 3;
         }
+        if (timing) KT_HIP_TRY(hipEventRecord(ev_m0, stream));
         KT_HIP_TRY(dev::mark_deleted(dst[cur].as<uint32_t>(), src[cur].as<uint32_t>(),
                                      eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                      state.as<uint8_t>(), nflag.as<uint8_t>(), tau.as<uint32_t>(),
                                      use_wl ? wl.as<uint32_t>() : nullptr, use_wl ? nitems : E, k, affected,
                                      ctr.as<DevCounters>(), stream));
+        if (timing) KT_HIP_TRY(hipEventRecord(ev_m1, stream));
         if ((rc = sync_counters())) return rc;
+        std::array<uint64_t, KT_ITER_REC> irec{};
+        if (timing) {
+            float mark_ms = 0;
+            KT_HIP_TRY(hipEventElapsedTime(&mark_ms, ev_m0, ev_m1));
+            irec[5] = (uint64_t)(mark_ms * 1e6);
+            irec[10] |= KT_ITER_MARK_TIMED;
+        }
         float count_ms = 0;
         if (counted && pick == KT_PICK_SWEEP) {
             float best = 0;
@@ -480,22 +531,33 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
         const uint32_t ndel = h_ctr->ndel;
         live -= ndel;
         stats.iterations++;
-        trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)(directed ? ndel : ndel / 2),
-                         h_ctr->recount_edges - rec_before, (uint64_t)(count_ms * 1e6)});
+        irec[0] = k;
+        irec[1] = iter;
+        irec[2] = directed ? ndel : ndel / 2;
+        irec[3] = h_ctr->recount_edges - rec_before;
+        irec[4] = (uint64_t)(count_ms * 1e6);
+        if (counted) irec[10] |= KT_ITER_COUNTED | KT_ITER_COUNT_TIMED;  // count events are always recorded here
         if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
         changed = ndel > 0;
         if (changed) {
             if (iter < v.num_edge_remove_iter) {
+                const auto t0 = std::chrono::steady_clock::now();
                 if ((rc = compact_now())) return rc;
+                irec[6] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                              std::chrono::steady_clock::now() - t0).count();
+                irec[7] = E;
+                irec[10] |= KT_ITER_REMOVED;
                 has_del = false;
             } else {
                 has_del = true;
             }
             if (k == 3) {  // kernel_ddc.cu:183-186
+                trace.push_back(irec);
                 broke = true;
                 break;
             }
             if (affected) {
+                if (timing) KT_HIP_TRY(hipEventRecord(ev_a0, stream));
                 KT_HIP_TRY(dev::mark_directly(src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
                                               state.as<uint8_t>(), nflag.as<uint8_t>(), E, directed,
                                               stream));
@@ -504,14 +566,30 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
                                                state.as<uint8_t>(), nflag.as<uint8_t>(),
                                                sup[cur].as<uint32_t>(), wl.as<uint32_t>(), E,
                                                directed, ctr.as<DevCounters>(), stream));
+                if (timing) KT_HIP_TRY(hipEventRecord(ev_a1, stream));
                 if ((rc = sync_counters())) return rc;
+                if (timing) {
+                    float aff_ms = 0;
+                    KT_HIP_TRY(hipEventElapsedTime(&aff_ms, ev_a0, ev_a1));
+                    irec[8] = (uint64_t)(aff_ms * 1e6);
+                }
+                irec[10] |= KT_ITER_AFFECTED;
                 nitems = h_ctr->wl_size;
                 use_wl = true;
             }
         }
+        trace.push_back(irec);
         ++iter;
     } while (changed);
-    if (has_del && (rc = compact_now())) return rc;  // kernel_ddc.cu:206-212
+    if (has_del) {  // kernel_ddc.cu:206-212
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = compact_now())) return rc;
+        if (!trace.empty()) {
+            trace.back()[9] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                  std::chrono::steady_clock::now() - t0).count();
+            trace.back()[10] |= KT_ITER_FINAL_REMOVAL;
+        }
+    }
     exact_out = !broke && !changed;
     // phase record
     if (pick == KT_PICK_SWEEP && nc) {
@@ -618,15 +696,20 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
     if ((rc = flush_phase_features())) return rc;
     stats.ms_total = ms;
     finish_stats();
+    // copy back, coocsr2coo, directed2undirectedCOO, unsort (kernel_d.cu:75-79,
+    // kernel_ddc.cu:246-250, kernel_d.cu:57-61, kernel.cu:31-37), each timed
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
+    auto t0 = clk::now();
     std::vector<uint32_t> s(E), d(E);
     if (E) {
         KT_HIP_TRY(hipMemcpy(s.data(), src[cur].p, (size_t)E * 4, hipMemcpyDeviceToHost));
         KT_HIP_TRY(hipMemcpy(d.data(), dst[cur].p, (size_t)E * 4, hipMemcpyDeviceToHost));
     }
-    // coocsr2coo + directed2undirectedCOO + unsort (kernel_ddc.cu:246-250,
-    // kernel_d.cu:57-61, kernel.cu:31-37)
+    stage_ms[8] = ms_since(t0);
     // DELTA plans keep tombstones in place (and KT_DYING entries of a last
     // k == 3 pass); the others compacted them away
+    t0 = clk::now();
     std::vector<uint32_t> keep;
     keep.reserve(E);
     for (uint32_t e = 0; e < E; ++e) {
@@ -641,14 +724,24 @@ int Plan::ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_
     out_src.resize(out);
     out_dst.resize(out);
     for (size_t i = 0; i < L; ++i) {
-        const uint32_t e = keep[i];
-        out_src[i] = new2old[s[e]];
-        out_dst[i] = new2old[d[e]];
-        if (directed) {
-            out_src[L + i] = new2old[d[e]];
-            out_dst[L + i] = new2old[s[e]];
+        out_src[i] = s[keep[i]];
+        out_dst[i] = d[keep[i]];
+    }
+    stage_ms[6] = ms_since(t0);
+    t0 = clk::now();
+    if (directed) {
+        for (size_t i = 0; i < L; ++i) {
+            out_src[L + i] = out_dst[i];
+            out_dst[L + i] = out_src[i];
         }
     }
+    stage_ms[7] = ms_since(t0);
+    t0 = clk::now();
+    for (size_t i = 0; i < out; ++i) {
+        out_src[i] = new2old[out_src[i]];
+        out_dst[i] = new2old[out_dst[i]];
+    }
+    stage_ms[9] = ms_since(t0);
     return KT_OK;
 }
 
@@ -837,10 +930,18 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
 }
 
 // Bookkeeping of one finished peel iteration (its deletions are read back).
-int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms) {
+int Plan::record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms,
+                           uint32_t flags) {
     live -= ndel;
     stats.iterations++;
-    trace.push_back({(uint64_t)k, (uint64_t)iter, (uint64_t)ndel, recount, (uint64_t)(count_ms * 1e6)});
+    std::array<uint64_t, KT_ITER_REC> r{};
+    r[0] = k;
+    r[1] = iter;
+    r[2] = ndel;
+    r[3] = recount;
+    r[4] = (uint64_t)(count_ms * 1e6);
+    r[10] = flags;
+    trace.push_back(r);
     if (ndel && k - 1 > stats.k_max) stats.k_max = k - 1;
     return KT_OK;
 }
@@ -1013,7 +1114,8 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         prev_nd = last_nd;
         last_nd = nd;
         iter = q.it + 1;
-        return record_iteration(k, q.it, nd, 0, ms);
+        const uint32_t fl = q.updated ? (KT_ITER_COUNTED | KT_ITER_DELTA | (timing ? (uint32_t)KT_ITER_COUNT_TIMED : 0u)) : 0u;
+        return record_iteration(k, q.it, nd, 0, ms, fl);
     };
     // the phase ended at the iteration just read: the younger queued ones are
     // void; the youngest's count is left for the next read-back to check.
@@ -1214,7 +1316,10 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         phase_ms += count_ms;
         const uint32_t ndel = h_ctr->ndel;
         pending = ndel;
-        if ((rc = record_iteration(k, iter, ndel, h_ctr->recount_edges - rec_before, count_ms))) return rc;
+        const uint32_t fl = counted ? (KT_ITER_COUNTED | KT_ITER_COUNT_TIMED |
+                                       (pick != KT_PICK_SWEEP && last_pass_delta ? (uint32_t)KT_ITER_DELTA : 0u))
+                                    : 0u;
+        if ((rc = record_iteration(k, iter, ndel, h_ctr->recount_edges - rec_before, count_ms, fl))) return rc;
         changed = ndel > 0;
         if (changed && k == 3) {  // kernel_ddc.cu:183-186
             broke = true;

@@ -71,7 +71,9 @@ struct Plan {
     // idle without it is an error
     int poll_pinned(uint32_t i, uint32_t& v);
 
-    std::vector<std::array<uint64_t, 5>> trace;
+    std::vector<std::array<uint64_t, KT_ITER_REC>> trace;  // kt_plan_iteration_log records
+    double stage_ms[KT_PLAN_STAGES] = {};                    // kt_plan_stage_times
+    hipEvent_t ev_m0 = nullptr, ev_m1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;  // mark, affected
     kt_stats stats{};
     bool have_tau = false;
 
@@ -121,6 +123,10 @@ struct Plan {
                  const float* range);
 
     ~Plan();
+    // Frees the working layout, then the pristine layout, then the peel state
+    // and the rest of the device arrays, timing each (ms[3], may be null); the
+    // plan can only be destroyed afterwards.
+    void release_timed(double* ms);
     int create(const kt_graph* g, const kt_variant* v);
     int reset();
     int decompose();
@@ -138,7 +144,9 @@ struct Plan {
     int run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& ended, bool head = false);
     // also clears ctr->ndel / resets ctr->min_sup in the same one-thread kernel
     int take_void(bool zero_ndel = false, bool reset_min = false);
-    int record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms);
+    // flags: KT_ITER_* of the iteration
+    int record_iteration(uint32_t k, uint32_t iter, uint32_t ndel, uint64_t recount, float count_ms,
+                         uint32_t flags = 0);
     int sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms);
     // defer: leave the model's choice on the device (read with the next
     // counter read-back) instead of synchronising for it now

@@ -42,7 +42,8 @@ EXPORTS = [
     "kt_generate_chung_lu", "kt_generate_erdos_renyi", "kt_plan_create", "kt_plan_destroy",
     "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
-    "kt_plan_set_timing", "kt_ktruss_noalloc",
+    "kt_plan_set_timing", "kt_ktruss_noalloc", "kt_plan_iteration_log", "kt_plan_stage_times",
+    "kt_plan_destroy_timed",
     "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
     "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
     "kt_mlp_set_input_range", "kt_mlp_get_input_range",
