@@ -66,14 +66,29 @@ __global__ void k_orient(const uint32_t* __restrict__ a, const uint32_t* __restr
     }
 }
 
-// an edge's position: its one entry in a directed layout, the (max, min)
-// copy (the one in the higher row) in an undirected layout
-__global__ void k_unpack_layout(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                uint32_t E, int both, uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                uint32_t* __restrict__ pos) {
+// Tiled formats: the sort key's high word becomes the entry's block row,
+// ((tile(s) * Y + class(tile(d))) * S + s mod S) with tiles of S vertices and
+// class(t) = t mod Y, so entries sort by (source tile, destination class,
+// source, destination).
+__global__ void k_tile_keys(uint64_t* __restrict__ keys, uint32_t E, uint32_t S, uint32_t Y) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < E; p += gridDim.x * blockDim.x) {
         const uint64_t k = keys[p];
         const uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)k;
+        const uint32_t row = ((s / S) * Y + (d / S) % Y) * S + s % S;
+        keys[p] = ((uint64_t)row << 32) | d;
+    }
+}
+
+// an edge's position: its one entry in a directed layout, the (max, min)
+// copy (the one in the higher row) in an undirected layout
+// (tiled: the source is recovered from the block row, S and Y as k_tile_keys)
+__global__ void k_unpack_layout(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                uint32_t E, int both, uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                uint32_t* __restrict__ pos, uint32_t S, uint32_t Y) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < E; p += gridDim.x * blockDim.x) {
+        const uint64_t k = keys[p];
+        const uint32_t hi = (uint32_t)(k >> 32), d = (uint32_t)k;
+        const uint32_t s = Y ? (hi / S / Y) * S + hi % S : hi;
         src[p] = s;
         dst[p] = d;
         if (!both || s > d) pos[vals[p]] = p;
@@ -93,6 +108,10 @@ __global__ void k_row_starts(const uint64_t* __restrict__ keys, uint32_t len, ui
         }
         starts[v] = lo;
     }
+}
+
+__global__ void k_zero_word(uint32_t* w) {
+    if (threadIdx.x == 0) *w = 0;
 }
 
 __global__ void k_max_row(const uint32_t* __restrict__ ptr, uint32_t n, uint32_t* max_row) {
@@ -172,7 +191,7 @@ namespace dev {
 hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b, int directedness,
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
-                               hipStream_t s, double* stage_ms) {
+                               hipStream_t s, double* stage_ms, uint32_t tile_size, uint32_t classes) {
     using clk = std::chrono::steady_clock;
     clk::time_point t0 = clk::now();
     // stage boundary: with stage_ms, wait for the stream and book the wall time
@@ -188,6 +207,11 @@ hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const 
     const uint64_t E64 = both ? 2ull * m : (uint64_t)m;
     if (E64 >= 0xFFFFFFFFull) return hipErrorInvalidValue;
     const uint32_t E = (uint32_t)E64;
+    // rows of the stored layout: vertices, or block rows of a tiled format
+    const uint32_t S = classes ? tile_size : 0, Y = classes;
+    const uint64_t rows64 = Y ? tiled_rows(n, S, Y) : (uint64_t)n;
+    if (rows64 >= 0xFFFFFFFFull || (Y && S == 0)) return hipErrorInvalidValue;
+    const uint32_t rows = (uint32_t)rows64;
     // relabel: vertices sorted by degree, stable in id (other orientations
     // keep ids)
     Tmp ids, old2new;
@@ -224,20 +248,25 @@ hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const 
                            by_degree ? old2new.as<uint32_t>() : (const uint32_t*)nullptr, m, both ? 1 : 0,
                            k0.as<uint64_t>(), v0.as<uint32_t>());
         KT_BUILD_TRY(hipGetLastError());
+        if (Y) {
+            hipLaunchKernelGGL(k_tile_keys, dim3(grid(E)), dim3(kBlock), 0, s, k0.as<uint64_t>(), E, S, Y);
+            KT_BUILD_TRY(hipGetLastError());
+        }
     }
     KT_BUILD_TRY(stage(1));
     if (m) {
         KT_BUILD_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), E,
-                                32 + (bits_for(n) ? bits_for(n) : 1u), s));
+                                32 + (bits_for(rows) ? bits_for(rows) : 1u), s));
         hipLaunchKernelGGL(k_unpack_layout, dim3(grid(E)), dim3(kBlock), 0, s, k1.as<uint64_t>(),
-                           v1.as<uint32_t>(), E, both ? 1 : 0, src, dst, pos);
+                           v1.as<uint32_t>(), E, both ? 1 : 0, src, dst, pos, S, Y);
         KT_BUILD_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_row_starts, dim3(grid((uint64_t)n + 1)), dim3(kBlock), 0, s, k1.as<uint64_t>(), E, n,
-                       ptr);
+    hipLaunchKernelGGL(k_row_starts, dim3(grid((uint64_t)rows + 1)), dim3(kBlock), 0, s, k1.as<uint64_t>(), E,
+                       rows, ptr);
     KT_BUILD_TRY(hipGetLastError());
-    KT_BUILD_TRY(hipMemsetAsync(max_row, 0, 4, s));
-    hipLaunchKernelGGL(k_max_row, dim3(grid(n)), dim3(kBlock), 0, s, ptr, n, max_row);
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, s, max_row);
+    KT_BUILD_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_max_row, dim3(grid(rows)), dim3(kBlock), 0, s, ptr, rows, max_row);
     KT_BUILD_TRY(hipGetLastError());
     KT_BUILD_TRY(stage(2));
     return hipStreamSynchronize(s);

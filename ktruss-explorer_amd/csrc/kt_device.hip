@@ -144,40 +144,12 @@ __device__ void stage_row(RowState& r, uint32_t u, const uint32_t* ptr, const ui
     }
 }
 
-// Next x-tile of class y (mod Y) at or after tile t.
-__device__ __forceinline__ uint32_t next_class_tile(uint32_t t, uint32_t y, uint32_t Y) {
-    const uint32_t m = t % Y;
-    return m == y ? t : t + (y + Y - m) % Y;
-}
-
-// Stream mode of a tiled item, class y: walking v's segments of class y costs
-// two binary searches to find the first and two per further segment, plus a
-// wave step per segment; scanning v's row once with the class as a lane
-// predicate costs a step per 64 entries.  Counted in dependent loads (a step:
-// one load and a search in u's row, about 16), from what is in registers:
-// the segments are at most the class-y tiles of u's candidate range
-// [lo, hi], the entries scanned at most vlen.  The scan is taken only when it
-// is at most half the walk; the same hits either way.  Rows spread over many
-// tiles of the class (segments of a few entries) scan; dense tiles, and
-// candidate ranges that few tiles of the class meet, walk.
-__device__ __forceinline__ bool tiled_scan_row(uint32_t vlen, uint32_t lo, uint32_t hi, uint32_t S, uint32_t y,
-                                               uint32_t Y) {
-    if (hi < lo) return false;
-    const uint32_t L = 32u - (uint32_t)__clz(vlen);
-    const uint32_t t0 = lo / S, t1 = hi / S;
-    // tiles t in [t0, t1] with t % Y == y
-    const uint64_t nseg = (t1 + 1 + Y - 1 - y) / Y - (t0 + Y - 1 - y) / Y;
-    const uint64_t walk = 2ull * L + nseg * (2ull * L + 16);
-    const uint64_t scan = (uint64_t)((vlen + 63) / 64) * 16 + (vlen > 128 ? L : 0);
-    return 2 * scan <= walk;
-}
-
 // Directed entry e = (u, v): credits of every triangle u < v < w.
-// u's candidates are row positions [lo, ulen) (the entries after v).
-template <bool TILED>
-__device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, uint32_t lo,
-                                        uint32_t vb, uint32_t vlen, uint32_t* s_val,
-                                        uint32_t* s_cnt, uint32_t y, uint32_t lane) {
+// u's candidates are row positions [lo, ulen) (the entries after v); v's row
+// is [vb, vb + vlen).  Tiled formats pass block rows (the class-y parts of
+// both rows), so the same code counts one x-tile class.
+__device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, uint32_t lo, uint32_t vb,
+                                        uint32_t vlen, uint32_t* s_val, uint32_t* s_cnt, uint32_t lane) {
     const uint32_t* dst = a.dst;
     uint32_t* sup = a.sup;
     const uint32_t ulen = r.ulen;
@@ -187,16 +159,14 @@ __device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, u
     uint32_t c = 0;
 
     if ((ulen - lo) * a.query_factor < vlen) {
-        // Query mode: u's candidates searched in v's row (HBM); tiled, only
-        // the candidates of x-tile class y (the same triangles the segments
-        // of class y in v's row find)
+        // Query mode: u's candidates searched in v's row (HBM)
         const uint32_t vmax = dst[vb + vlen - 1] & KT_VMASK;
         for (uint32_t p0 = lo; p0 < ulen; p0 += kWave) {
             const uint32_t p = p0 + lane;
             const uint32_t x = p < ulen ? urow[p] : KT_DEAD;
             bool hit = false;
             uint32_t j = 0;
-            if (!(x & KT_DEAD) && x <= vmax && (!TILED || ((x & KT_VMASK) / a.tile_size) % a.ytiles == y)) {
+            if (!(x & KT_DEAD) && x <= vmax) {
                 j = lower_bound_masked(dst + vb, 0, vlen, x);
                 hit = j < vlen && dst[vb + j] == x;
             }
@@ -212,72 +182,47 @@ __device__ uint32_t count_directed_item(const CountArgs& a, const RowState& r, u
     }
 
     // Stream mode: v's row streamed, each entry searched among u's candidates.
-    uint32_t j0 = 0;
-    uint32_t jend = vlen;
-    uint32_t t = 0;
-    const uint32_t S = a.tile_size, Y = a.ytiles;
-    const bool scan = TILED && tiled_scan_row(vlen, umin, r.umax, S, y, Y);
-    if (TILED && !scan) {
-        // first segment of class y at or after the tile of umin
-        t = next_class_tile(umin / S, y, Y);
-        j0 = wave_lower_bound(dst + vb, 0, vlen, t * S, lane);
-        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
-    } else if (vlen > 2 * kWave) {
-        j0 = wave_lower_bound(dst + vb, 0, vlen, umin, lane);
-    }
-    while (j0 < vlen) {
-        bool done = false;
-        for (; j0 < jend; j0 += kWave) {
-            const uint32_t j = j0 + lane;
-            const bool inr = j < jend;
-            const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
-            const uint32_t wm = w & KT_VMASK;
-            bool hit = false;
-            uint32_t p = 0;
-            if (inr && !(w & KT_DEAD) && wm <= r.umax && (!scan || (wm / S) % Y == y)) {
-                p = lower_bound_masked(urow, lo, ulen, wm);
-                hit = p < ulen && urow[p] == wm;
-            }
-            if (hit) {
-                if (r.in_lds) atomicAdd(s_cnt + p, 1u);
-                else atomicAdd(sup + r.ub + p, 1u);
-                atomicAdd(sup + vb + j, 1u);
-            }
-            c += (uint32_t)__popcll(ballot(hit));
-            if (ballot(inr && wm > r.umax)) { done = true; break; }
+    const uint32_t j0 = vlen > 2 * kWave ? wave_lower_bound(dst + vb, 0, vlen, umin, lane) : 0u;
+    for (uint32_t jb = j0; jb < vlen; jb += kWave) {
+        const uint32_t j = jb + lane;
+        const bool inr = j < vlen;
+        const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
+        const uint32_t wm = w & KT_VMASK;
+        bool hit = false;
+        uint32_t p = 0;
+        if (inr && !(w & KT_DEAD) && wm <= r.umax) {
+            p = lower_bound_masked(urow, lo, ulen, wm);
+            hit = p < ulen && urow[p] == wm;
         }
-        if (!TILED || done || scan) break;
-        // next segment of class y
-        if (jend >= vlen) break;
-        const uint32_t tn = (dst[vb + jend] & KT_VMASK) / S;
-        t = next_class_tile(tn, y, Y);
-        if (t * S > r.umax) break;
-        j0 = t == tn ? jend : wave_lower_bound(dst + vb, jend, vlen, t * S, lane);
-        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
+        if (hit) {
+            if (r.in_lds) atomicAdd(s_cnt + p, 1u);
+            else atomicAdd(sup + r.ub + p, 1u);
+            atomicAdd(sup + vb + j, 1u);
+        }
+        c += (uint32_t)__popcll(ballot(hit));
+        if (ballot(inr && wm > r.umax)) break;
     }
     return c;
 }
 
-// Undirected entry e = (u, v): |N(u) n N(v)| restricted to x-tile class y
-// (tiled), stopping at cap.
-template <bool TILED>
-__device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r, uint32_t vb,
-                                          uint32_t vlen, const uint32_t* s_val, uint32_t y,
-                                          uint32_t lane) {
+// Undirected entry e = (u, v): |N(u) n N(v)|, stopping at cap (tiled: the
+// class-y block rows of both, one x-tile class's partial, and cap what the
+// item still lacks of k-2).
+__device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r, uint32_t vb, uint32_t vlen,
+                                          const uint32_t* s_val, uint32_t lane, uint32_t cap) {
     const uint32_t* dst = a.dst;
-    const uint32_t ulen = r.ulen, cap = a.cap;
+    const uint32_t ulen = r.ulen;
     if (ulen == 0 || vlen == 0 || cap == 0) return 0;
     const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
     uint32_t c = 0;
 
     if (ulen * a.query_factor < vlen) {
-        // query mode (tiled: the candidates of x-tile class y)
         const uint32_t vmax = dst[vb + vlen - 1] & KT_VMASK;
         for (uint32_t p0 = 0; p0 < ulen; p0 += kWave) {
             const uint32_t p = p0 + lane;
             const uint32_t x = p < ulen ? urow[p] : KT_DEAD;
             bool hit = false;
-            if (!(x & KT_DEAD) && x <= vmax && (!TILED || ((x & KT_VMASK) / a.tile_size) % a.ytiles == y)) {
+            if (!(x & KT_DEAD) && x <= vmax) {
                 const uint32_t j = lower_bound_masked(dst + vb, 0, vlen, x);
                 hit = j < vlen && dst[vb + j] == x;
             }
@@ -289,39 +234,20 @@ __device__ uint32_t count_undirected_item(const CountArgs& a, const RowState& r,
     }
 
     const uint32_t umin = urow[0] & KT_VMASK;
-    uint32_t j0 = 0, jend = vlen, t = 0;
-    const uint32_t S = a.tile_size, Y = a.ytiles;
-    const bool scan = TILED && tiled_scan_row(vlen, umin, r.umax, S, y, Y);
-    if (TILED && !scan) {
-        t = next_class_tile(umin / S, y, Y);
-        j0 = wave_lower_bound(dst + vb, 0, vlen, t * S, lane);
-        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
-    } else if (vlen > 2 * kWave) {
-        j0 = wave_lower_bound(dst + vb, 0, vlen, umin, lane);
-    }
-    while (j0 < vlen) {
-        bool done = false;
-        for (; j0 < jend; j0 += kWave) {
-            const uint32_t j = j0 + lane;
-            const bool inr = j < jend;
-            const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
-            const uint32_t wm = w & KT_VMASK;
-            bool hit = false;
-            if (inr && !(w & KT_DEAD) && wm <= r.umax && (!scan || (wm / S) % Y == y)) {
-                const uint32_t p = lower_bound_masked(urow, 0, ulen, wm);
-                hit = p < ulen && urow[p] == wm;
-            }
-            c += (uint32_t)__popcll(ballot(hit));
-            if (c >= cap) return cap;
-            if (ballot(inr && wm > r.umax)) { done = true; break; }
+    const uint32_t j0 = vlen > 2 * kWave ? wave_lower_bound(dst + vb, 0, vlen, umin, lane) : 0u;
+    for (uint32_t jb = j0; jb < vlen; jb += kWave) {
+        const uint32_t j = jb + lane;
+        const bool inr = j < vlen;
+        const uint32_t w = inr ? dst[vb + j] : KT_VMASK;
+        const uint32_t wm = w & KT_VMASK;
+        bool hit = false;
+        if (inr && !(w & KT_DEAD) && wm <= r.umax) {
+            const uint32_t p = lower_bound_masked(urow, 0, ulen, wm);
+            hit = p < ulen && urow[p] == wm;
         }
-        if (!TILED || done || scan) break;
-        if (jend >= vlen) break;
-        const uint32_t tn = (dst[vb + jend] & KT_VMASK) / S;
-        t = next_class_tile(tn, y, Y);
-        if (t * S > r.umax) break;
-        j0 = t == tn ? jend : wave_lower_bound(dst + vb, jend, vlen, t * S, lane);
-        jend = wave_lower_bound(dst + vb, j0, vlen, (t + 1) * S, lane);
+        c += (uint32_t)__popcll(ballot(hit));
+        if (c >= cap) return cap;
+        if (ballot(inr && wm > r.umax)) break;
     }
     return c < cap ? c : cap;
 }
@@ -338,6 +264,95 @@ __device__ __forceinline__ void stripe_stats(const CountArgs& a, unsigned long l
     atomicAdd(q + 1, rec);
 }
 
+// Items of a tiled count whose two class-y block rows hold at most this many
+// entries together are merged by one lane (fine tiles leave most block rows
+// a few entries long, too little for a wave).
+constexpr uint32_t kLaneMerge = 48;
+
+// One lane's item e = (u, v) of a tiled count: u's block row [ub, ub + ulen)
+// merged with v's [vb, vb + vlen).  Directed: the triangles u < v < w of
+// this class -- u's entries after v (from e itself when e lies in the block
+// row: own), credits on (u,v), (u,w), (v,w) by atomics.  Undirected: the
+// common neighbours up to cap, added to e's support.
+template <bool DIR>
+__device__ void lane_merge_item(const CountArgs& a, uint32_t e, uint32_t v, uint32_t ub, uint32_t ulen, uint32_t vb,
+                                uint32_t vlen, bool own, uint32_t cap) {
+    const uint32_t* dst = a.dst;
+    uint32_t i = ub, iend = ub + ulen, j = vb, jend = vb + vlen, c = 0;
+    if (DIR) i = own ? e + 1 : ub + lower_bound_masked(dst + ub, 0, ulen, v + 1);
+    if (DIR) cap = 0xFFFFFFFFu;
+    while (i < iend && j < jend && c < cap) {
+        const uint32_t x = dst[i], w = dst[j];
+        if (x & KT_DEAD) { ++i; continue; }
+        if (w & KT_DEAD) { ++j; continue; }
+        if (x < w) {
+            ++i;
+        } else if (x > w) {
+            ++j;
+        } else {
+            if (DIR) {
+                atomicAdd(a.sup + i, 1u);
+                atomicAdd(a.sup + j, 1u);
+            }
+            ++c;
+            ++i;
+            ++j;
+        }
+    }
+    if (c) {
+        if (DIR || a.ytiles > 1) atomicAdd(a.sup + e, c);
+        else a.sup[e] = c;
+    }
+}
+
+// A lane's item whose block rows both hold at most kRegRow entries: both are
+// loaded into registers with independent loads (one memory round trip) and
+// compared all against all; the same credits as lane_merge_item.
+constexpr uint32_t kRegRow = 16;
+
+template <bool DIR>
+__device__ void lane_regs_item(const CountArgs& a, uint32_t e, uint32_t v, uint32_t ub, uint32_t ulen, uint32_t vb,
+                               uint32_t vlen, uint32_t cap) {
+    const uint32_t* dst = a.dst;
+    uint32_t uu[kRegRow], vv[kRegRow];
+#pragma unroll
+    for (uint32_t t = 0; t < kRegRow; ++t) uu[t] = t < ulen ? dst[ub + t] : KT_DEAD;
+#pragma unroll
+    for (uint32_t t = 0; t < kRegRow; ++t) vv[t] = t < vlen ? dst[vb + t] : KT_DEAD;
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kRegRow; ++i) {
+        const uint32_t x = uu[i];
+        // directed: u's candidates are the entries after v (rows ascend)
+        if (!(x & KT_DEAD) && (!DIR || x > v)) {
+#pragma unroll
+            for (uint32_t j = 0; j < kRegRow; ++j) {
+                if (vv[j] == x) {  // a dead entry of v's row has bit 31 set: never equal
+                    ++c;
+                    if (DIR) {
+                        atomicAdd(a.sup + ub + i, 1u);
+                        atomicAdd(a.sup + vb + j, 1u);
+                    }
+                }
+            }
+        }
+    }
+    if (!DIR && c > cap) c = cap;
+    if (c) {
+        if (DIR || a.ytiles > 1) atomicAdd(a.sup + e, c);
+        else a.sup[e] = c;
+    }
+}
+
+// Support count per item e = (u, v) with u's row staged in the wave's LDS
+// slice (undirected COO+CSR; both tiled formats).  Tiled formats: y-block y
+// (blockIdx.y) counts the triangles whose third vertex w lies in a tile of
+// class y -- the x-tile segments of kernel_ddt.cu:25-34 / kernel_dut.cu:25-34,
+// which the block layout stores contiguously as u's and v's block rows of
+// class y.  u's block row is staged (in stored order an item's row changes
+// once per block); summed over the classes (directed: credits added by
+// atomics; undirected: per-class partials capped at k-2 and summed
+// atomically, kernel_dut.cu:57-61) the supports equal the untiled count's.
 template <bool DIR, bool TILED>
 __global__ __launch_bounds__(1024) void k_count(CountArgs a) {
     extern __shared__ uint32_t lds[];
@@ -346,15 +361,19 @@ __global__ __launch_bounds__(1024) void k_count(CountArgs a) {
     const uint32_t nw = blockDim.x / kWave;
     uint32_t* s_val = lds + wid * (DIR ? 2u : 1u) * a.cap_lds;
     uint32_t* s_cnt = s_val + a.cap_lds;
-    const uint32_t task = xcd_block(blockIdx.x, gridDim.x) * nw + wid;
+    const uint32_t S = a.tile_size, Y = TILED ? a.ytiles : 1u;
+    // tiled: blockIdx.y is the x-tile class (the y-blocks of kernel_ddt.cu:28
+    // and kernel_dut.cu:28); a class's workgroups run together and the classes
+    // one after another, so no two workgroups credit one item's support at once
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t y0 = TILED ? blockIdx.y : 0u;
+    const uint32_t task = lb * nw + wid;
     const uint64_t first = (uint64_t)task * a.chunk;
     if (first >= a.nitems) return;
     const uint32_t i0 = (uint32_t)first;
     const uint32_t i1 = a.nitems - i0 < a.chunk ? a.nitems : i0 + a.chunk;
-    const uint32_t y = TILED ? blockIdx.y : 0;
     const uint32_t* ptr = a.ptr;
     const uint32_t* dst = a.dst;
-    const uint32_t S = a.tile_size;
 
     RowState r;
     r.u = 0xFFFFFFFFu;
@@ -364,64 +383,119 @@ __global__ __launch_bounds__(1024) void k_count(CountArgs a) {
 
     for (uint32_t base = i0; base < i1; base += kWave) {
         const uint32_t my = base + lane;
-        uint32_t e = 0, u = 0, v = KT_DEAD, vb = 0, vlen = 0;
+        uint32_t e = 0, u = 0, v = KT_DEAD;
         if (my < i1) {
             e = a.wl ? a.wl[my] : my;
             v = dst[e];
-            if (!(v & KT_DEAD)) {
-                u = a.src[e];
-                vb = ptr[v];
-                vlen = ptr[v + 1] - vb;
-            }
+            if (!(v & KT_DEAD)) u = a.src[e];
         }
-        uint64_t todo = ballot(my < i1 && !(v & KT_DEAD));
-        while (todo) {
-            const uint32_t l = (uint32_t)__ffsll((long long)todo) - 1;
-            todo &= todo - 1;
-            const uint32_t eu = readlane(u, l), ee = readlane(e, l), ev = readlane(v, l);
-            const uint32_t evb = readlane(vb, l), evlen = readlane(vlen, l);
-            if (eu != r.u) {
-                flush_row<DIR>(r, s_cnt, a.sup, lane);
-                stage_row<DIR>(r, eu, ptr, dst, s_val, s_cnt, a.cap_lds, lane);
-            }
-            const uint32_t q = ee - r.ub;  // position of v in u's row
-            if (y == 0) {
-                acc_rec += 1;
-                uint32_t lu = r.ulen;
-                if (TILED && DIR) {
-                    // u's segments from tile(v) on (kernel_ddt.cu:28)
-                    const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
-                    const uint32_t tv = ev / S;
-                    const uint32_t cut = r.in_lds
-                                             ? lower_bound_masked(urow, 0, r.ulen, tv * S)
-                                             : wave_lower_bound(urow, 0, r.ulen, tv * S, lane);
-                    lu = r.ulen - cut;
+        const bool live = my < i1 && !(v & KT_DEAD);
+        if (live && y0 == 0) acc_rec += 1;
+        // tiled: this workgroup's x-tile class, a y-block's share of
+        // kernel_ddt.cu:28 / kernel_dut.cu:28; untiled: the one pass
+        for (uint32_t y = y0; y < y0 + 1; ++y) {
+            uint32_t vb = 0, vlen = 0, icap = a.cap;
+            bool work = live;
+            if (live) {
+                if (TILED) {
+                    const uint32_t vr = tiled_row(v, y, S, Y), ur = tiled_row(u, y, S, Y);
+                    vb = ptr[vr];
+                    vlen = ptr[vr + 1] - vb;
+                    const uint32_t ub = ptr[ur], ulen = ptr[ur + 1] - ub;
+                    // class-y block rows are short: the lane settles the item's
+                    // statistics and drops it when a block row is empty
+                    // (directed: or holds nothing after v; undirected: or a
+                    // whole stored row is shorter than k-2, the exact
+                    // pre-filter of kernel_duc.cu:27)
+                    uint32_t lu = ulen;
+                    if (DIR && ulen) lu -= lower_bound_masked(dst + ub, 0, ulen, (v / S) * S);
+                    acc_merge += (unsigned long long)lu + vlen;
+                    work = ulen && vlen && (!DIR || (dst[ub + ulen - 1] & KT_VMASK) > v);
+                    if (!DIR && work) {
+                        if (a.rowlen) work = a.rowlen[u] >= a.cap && a.rowlen[v] >= a.cap;
+                        // Only "support >= k-2" is decided: a class counts what
+                        // the item still lacks after the classes that added to
+                        // its support already.  If one class reaches its cap,
+                        // the sum reaches k-2 (the others add what it saw); if
+                        // none does, every partial is whole: the decision is
+                        // the full count's either way.
+                        const uint32_t have = a.sup[e];
+                        icap = have < a.cap ? a.cap - have : 0u;
+                        work = work && icap;
+                    }
+                    if (work && ulen <= kRegRow && vlen <= kRegRow) {
+                        lane_regs_item<DIR>(a, e, v, ub, ulen, vb, vlen, icap);
+                        work = false;
+                    } else if (work && ulen + vlen <= kLaneMerge) {
+                        // both block rows short: this lane merges them alone
+                        lane_merge_item<DIR>(a, e, v, ub, ulen, vb, vlen, (v / S) % Y == y, icap);
+                        work = false;
+                    }
+                } else {
+                    vb = ptr[v];
+                    vlen = ptr[v + 1] - vb;
                 }
-                acc_merge += (unsigned long long)lu + evlen;
             }
-            if (DIR) {
-                const uint32_t c = count_directed_item<TILED>(a, r, q + 1, evb, evlen, s_val,
-                                                               s_cnt, y, lane);
-                if (c && lane == 0) {
-                    if (r.in_lds) atomicAdd(s_cnt + q, c);
-                    else atomicAdd(a.sup + ee, c);
+            uint64_t todo = ballot(work);
+            while (todo) {
+                const uint32_t l = (uint32_t)__ffsll((long long)todo) - 1;
+                todo &= todo - 1;
+                const uint32_t eu = readlane(u, l), ee = readlane(e, l), ev = readlane(v, l);
+                const uint32_t evb = readlane(vb, l), evlen = readlane(vlen, l), ecap = readlane(icap, l);
+                const uint32_t ur = TILED ? tiled_row(eu, y, S, Y) : eu;  // u's row (of class y)
+                if (ur != r.u) {
+                    flush_row<DIR>(r, s_cnt, a.sup, lane);
+                    stage_row<DIR>(r, ur, ptr, dst, s_val, s_cnt, a.cap_lds, lane);
                 }
-            } else {
-                uint32_t c = 0;
-                if (!a.prefilter || (r.ulen >= a.cap && evlen >= a.cap))
-                    c = count_undirected_item<TILED>(a, r, evb, evlen, s_val, y, lane);
-                if (lane == 0) {
-                    if (TILED && a.ytiles > 1) {
-                        if (c) atomicAdd(a.sup + ee, c);
-                    } else {
-                        a.sup[ee] = c;
+                const uint32_t* urow = r.in_lds ? s_val : dst + r.ub;
+                // e itself lies in the staged row (always untiled; tiled when
+                // v's tile is of class y)
+                const bool own = !TILED || (ev / S) % Y == y;
+                if (!TILED && lane == 0) acc_merge += (unsigned long long)r.ulen + evlen;  // summed over lanes below
+                if (DIR) {
+                    // u's candidates: the entries after v
+                    uint32_t lo = ee - r.ub + 1;
+                    if (!own)
+                        lo = r.in_lds ? lower_bound_masked(urow, 0, r.ulen, ev + 1)
+                                      : wave_lower_bound(urow, 0, r.ulen, ev + 1, lane);
+                    const uint32_t c = count_directed_item(a, r, lo, evb, evlen, s_val, s_cnt, lane);
+                    if (c && lane == 0) {
+                        if (own && r.in_lds) atomicAdd(s_cnt + (ee - r.ub), c);
+                        else atomicAdd(a.sup + ee, c);
+                    }
+                } else {
+                    uint32_t c = 0;
+                    if (!a.prefilter || (r.ulen >= a.cap && evlen >= a.cap))
+                        c = count_undirected_item(a, r, evb, evlen, s_val, lane, ecap);
+                    if (lane == 0) {
+                        if (TILED && Y > 1) {
+                            if (c) atomicAdd(a.sup + ee, c);
+                        } else {
+                            a.sup[ee] = c;
+                        }
                     }
                 }
             }
         }
     }
     flush_row<DIR>(r, s_cnt, a.sup, lane);
-    if (lane == 0 && acc_rec) stripe_stats(a, acc_merge, acc_rec);
+    for (int off = 32; off > 0; off >>= 1) {  // statistics kept per lane
+        acc_merge += __shfl_xor(acc_merge, off);
+        acc_rec += __shfl_xor(acc_rec, off);
+    }
+    if (lane == 0 && (acc_rec || acc_merge)) stripe_stats(a, acc_merge, acc_rec);
+}
+
+__global__ void k_tiled_row_totals(const uint32_t* __restrict__ ptr, uint32_t n, uint32_t S, uint32_t Y,
+                                   uint32_t* __restrict__ out) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+        uint32_t t = 0;
+        for (uint32_t y = 0; y < Y; ++y) {
+            const uint32_t row = tiled_row(u, y, S, Y);
+            t += ptr[row + 1] - ptr[row];
+        }
+        out[u] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,6 +1369,7 @@ hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_
         hipLaunchKernelGGL(k_count_dir<false>, dim3(pb ? pb : 1), dim3(block), lds, s, a, TaskList{});
         return hipGetLastError();
     }
+    // tiled: grid (task groups, x-tile classes)
     const size_t lds = (size_t)nw * (directed ? 2 : 1) * a.cap_lds * sizeof(uint32_t);
     if (directed) {
         hipLaunchKernelGGL((k_count<true, true>), grid, dim3(block), lds, s, a);
@@ -1302,6 +1377,12 @@ hipError_t count_supports(const CountArgs& a, bool directed, bool tiled, uint32_
         if (tiled) hipLaunchKernelGGL((k_count<false, true>), grid, dim3(block), lds, s, a);
         else hipLaunchKernelGGL((k_count<false, false>), grid, dim3(block), lds, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t tiled_row_totals(const uint32_t* ptr, uint32_t n, uint32_t S, uint32_t Y, uint32_t* out, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_tiled_row_totals, dim3(grid_for(n)), dim3(kBlock), 0, s, ptr, n, S, Y, out);
     return hipGetLastError();
 }
 

@@ -102,6 +102,7 @@ struct CountArgs {
     uint32_t prefilter;   // undirected COO+CSR stored-length pre-filter (huc:20)
     DevCounters* ctr;
     unsigned long long* stripes;  // kStripes lines of the count statistics
+    const uint32_t* rowlen;  // tiled undirected: stored row length per vertex (pre-filter); else null
 };
 
 // Device-resident task list of a batched decomposition: task t covers the
@@ -113,9 +114,30 @@ struct TaskList {
     const uint32_t* ntasks;
 };
 
+// Tiled COO+CSR format (TiledCOOCSRGraph, graph.h:67-76): tiles of S
+// vertices, destination tiles grouped into Y classes (class(t) = t mod Y, the
+// x-tile groups the reference's 2-D grid counts in parallel, kernel_ddt.cu:28,
+// kernel_dut.cu:28).  Entries are stored block by block -- block (i, y) holds
+// the entries from source tile i into tiles of class y -- and within a block
+// by source, then destination: a CSR per block, whose rows are the block rows
+// ((i * Y + y) * S + u mod S) of one pointer array of tiled_rows(n, S, Y) + 1
+// entries.  With Y equal to the tiles per dimension this is the reference's
+// tileSrcPtr ((tile(u) * T + tile(v)) * S + u mod S, graph.cu:674-676); with
+// fewer classes it has T * Y * S + 1 pointers instead of T^2 * S + 1, so fine
+// tiles (T = 1024 at 2^22 vertices) need no 2^32-entry array.
+__host__ __device__ __forceinline__ uint32_t tiled_row(uint32_t u, uint32_t cls, uint32_t S, uint32_t Y) {
+    return ((u / S) * Y + cls) * S + u % S;
+}
+inline uint64_t tiled_rows(uint32_t n, uint32_t S, uint32_t Y) {
+    return (((uint64_t)n + S - 1) / S) * Y * S;
+}
+
 // Kernel launchers (kt_device.hip).  All asynchronous on `stream`.
 namespace dev {
 hipError_t fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
+// Stored row length of every vertex of a tiled layout (the sum of its Y block
+// rows): out[u] for u < n.
+hipError_t tiled_row_totals(const uint32_t* ptr, uint32_t n, uint32_t S, uint32_t Y, uint32_t* out, hipStream_t s);
 hipError_t stripes_fold(unsigned long long* stripes, DevCounters* ctr, hipStream_t s);
 hipError_t fill_u8(uint8_t* p, uint8_t value, uint64_t count, hipStream_t s);
 hipError_t iota_u32(uint32_t* p, uint64_t count, hipStream_t s);
@@ -259,10 +281,14 @@ hipError_t compact(uint32_t n, uint32_t E, const uint32_t* ptr_in, const uint32_
 // ascending), new2old (n), pos (m: pair -> its entry in the higher row) and
 // *max_row (the longest row).  stage_ms (optional, 3 entries): wall times of
 // the relabel, the orientation and the row build, each synchronised.
+// classes > 0: the tiled format -- entries in block-row order (tiled_row),
+// ptr with tiled_rows(n, tile_size, classes) + 1 entries, *max_row the
+// longest block row.
 hipError_t build_layout_device(uint32_t n, uint32_t m, const uint32_t* a, const uint32_t* b, int directedness,
                                const uint32_t* degree, uint32_t max_degree, uint32_t* ptr, uint32_t* src,
                                uint32_t* dst, uint32_t* new2old, uint32_t* pos, uint32_t* max_row,
-                               hipStream_t s, double* stage_ms = nullptr);
+                               hipStream_t s, double* stage_ms = nullptr, uint32_t tile_size = 0,
+                               uint32_t classes = 0);
 // Input validation of m entries over n vertices (kt_graph_create's checks):
 // res.status is KT_OK or the error of the lowest offending entry res.at;
 // for a valid input, the unique edges (ua < ub, sorted), every entry's edge

@@ -113,7 +113,7 @@ void Plan::release_timed(double* ms) {
     lap(0);
     for (DBuf* x : {&ptr0, &src0, &dst0}) x->release();
     lap(1);
-    for (DBuf* x : {&state, &nflag, &wl, &tau, &tiles, &xs, &ctr, &feat_acc, &ctr_scratch, &feat_log, &stripes,
+    for (DBuf* x : {&state, &nflag, &wl, &tau, &tiles, &xs, &ctr, &rowlen, &feat_acc, &ctr_scratch, &feat_log, &stripes,
                     &stripes_x, &model, &sa, &sb, &sptr0, &snbr0, &seid0, &sa0, &sb0, &dl, &dl_alt, &dl3,
                     &sup_snap, &dscratch, &ctsum, &cxs, &ldeg})
         x->release();
@@ -160,6 +160,13 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         tile_size = n ? n : 1;
         ytiles = 1;
     }
+    rows = n;
+    if (tiled) {
+        const uint64_t r = tiled_rows(n, tile_size, ytiles);
+        if (r >= 0xFFFFFFFFull)
+            return fail(KT_ERR_CAPACITY, "tiled format: tiles x parallel tiles x tile size must stay below 2^32");
+        rows = (uint32_t)r;
+    }
     query_factor = 8;
 
     KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -181,9 +188,10 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     auto t0 = clk::now();
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
     const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
-    if ((rc = ptr0.alloc(Pb)) || (rc = src0.alloc(Eb)) || (rc = dst0.alloc(Eb))) return rc;
+    const size_t Rb = ((size_t)rows + 1) * sizeof(uint32_t);  // row pointers of the stored layout
+    if ((rc = ptr0.alloc(Rb)) || (rc = src0.alloc(Eb)) || (rc = dst0.alloc(Eb))) return rc;
     for (int b = 0; b < 2; ++b)
-        if ((rc = ptr[b].alloc(Pb)) || (rc = src[b].alloc(Eb)) || (rc = dst[b].alloc(Eb)) ||
+        if ((rc = ptr[b].alloc(Rb)) || (rc = src[b].alloc(Eb)) || (rc = dst[b].alloc(Eb)) ||
             (rc = eid[b].alloc(Eb)) || (rc = sup[b].alloc(Eb)))
             return rc;
     if ((rc = state.alloc(std::max<uint32_t>(E0, 1))) || (rc = nflag.alloc(std::max<uint32_t>(n, 1))) ||
@@ -194,6 +202,7 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = stripes_x.alloc(kStripeBytes)) ||
         (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))))
         return rc;
+    if (tiled && !directed && (rc = rowlen.alloc(std::max<size_t>((size_t)n * 4, 4)))) return rc;
     stage_ms[2] = ms_since(t0);  // the peel state (Info, info.cu:22-35) and the layouts
     {
         const uint32_t m = (uint32_t)g->m;
@@ -214,7 +223,8 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         KT_HIP_TRY(dev::build_layout_device(n, m, da.as<uint32_t>(), db.as<uint32_t>(), v.directedness,
                                             ddeg.as<uint32_t>(), g->max_degree, ptr0.as<uint32_t>(),
                                             src0.as<uint32_t>(), dst0.as<uint32_t>(), dn2o.as<uint32_t>(),
-                                            dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream, build_ms));
+                                            dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream, build_ms,
+                                            tiled ? tile_size : 0, tiled ? ytiles : 0));
         stage_ms[0] = build_ms[0];
         stage_ms[3] = build_ms[1];
         stage_ms[4] = build_ms[2];
@@ -263,11 +273,13 @@ int Plan::set_counter(size_t offset, uint32_t value) {
 }
 
 int Plan::reset() {
-    const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
+    const size_t Rb = ((size_t)rows + 1) * sizeof(uint32_t);
     cur = 0;
     E = E0;
     live = E0;
-    KT_HIP_TRY(hipMemcpyAsync(ptr[0].p, ptr0.p, Pb, hipMemcpyDeviceToDevice, stream));
+    KT_HIP_TRY(hipMemcpyAsync(ptr[0].p, ptr0.p, Rb, hipMemcpyDeviceToDevice, stream));
+    if (tiled && !directed)
+        KT_HIP_TRY(dev::tiled_row_totals(ptr0.as<uint32_t>(), n, tile_size, ytiles, rowlen.as<uint32_t>(), stream));
     if (E0) {
         KT_HIP_TRY(hipMemcpyAsync(src[0].p, src0.p, (size_t)E0 * 4, hipMemcpyDeviceToDevice, stream));
         KT_HIP_TRY(hipMemcpyAsync(dst[0].p, dst0.p, (size_t)E0 * 4, hipMemcpyDeviceToDevice, stream));
@@ -297,7 +309,7 @@ int Plan::compact_now() {
     const int o = 1 - cur;
     stats.compactions++;
     stats.compact_elems += E;
-    KT_HIP_TRY(dev::compact(n, E, ptr[cur].as<uint32_t>(), src[cur].as<uint32_t>(),
+    KT_HIP_TRY(dev::compact(rows, E, ptr[cur].as<uint32_t>(), src[cur].as<uint32_t>(),
                             dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                             sup[cur].as<uint32_t>(), ptr[o].as<uint32_t>(), src[o].as<uint32_t>(),
                             dst[o].as<uint32_t>(), eid[o].as<uint32_t>(), sup[o].as<uint32_t>(),
@@ -309,6 +321,8 @@ int Plan::compact_now() {
                                       " entries, expected " + std::to_string(live));
     cur = o;
     E = h_ctr->live;
+    if (tiled && !directed)
+        KT_HIP_TRY(dev::tiled_row_totals(ptr[cur].as<uint32_t>(), n, tile_size, ytiles, rowlen.as<uint32_t>(), stream));
     return KT_OK;
 }
 
@@ -386,6 +400,9 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
     a.ytiles = ytiles;
     a.query_factor = query_factor;
     a.prefilter = (!directed && !tiled) ? 1u : 0u;  // kernel_duc.cu:27; kernel_dut.cu has none
+    // tiled undirected: the exact stored-length pre-filter of kernel_duc.cu:27
+    // (rows shorter than k-2 hold fewer than k-2 triangles), from stored row lengths
+    a.rowlen = tiled && !directed ? rowlen.as<uint32_t>() : nullptr;
     a.ctr = counters;
     if (counters == ctr.as<DevCounters>()) {
         a.stripes = stripes.as<unsigned long long>();
@@ -422,7 +439,7 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bo
     }
     const bool by_model = pick == KT_PICK_MODEL;
     KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), live_rows ? ldeg.as<uint32_t>() : nullptr,
-                                   dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), n, E, E0, k, exact,
+                                   dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), rows, E, E0, k, exact,
                                    feat_acc.as<unsigned long long>(), feat_log.as<float>() + ph * KT_MLP_IN,
                                    by_model ? model.as<float>() : nullptr, model_C,
                                    by_model ? model.as<float>() + model_n : nullptr,

@@ -40,6 +40,7 @@ struct Plan {
     const kt_graph* graph = nullptr;
     bool directed = true, tiled = false, affected = false;
     uint32_t n = 0, E0 = 0, E = 0;
+    uint32_t rows = 0;  // rows of the stored layout: n, or the tiled format's block rows
     uint64_t live = 0;
     uint32_t tile_size = 1, ytiles = 1;
     uint32_t cap_lds = 64, query_factor = 8;
@@ -50,6 +51,7 @@ struct Plan {
     DBuf ptr[2], src[2], dst[2], eid[2], sup[2];    // working layout, double-buffered
     int cur = 0;
     DBuf state, nflag, wl, tau, tiles, xs, ctr;
+    DBuf rowlen;  // tiled undirected: stored row length per vertex of the working layout
     DevCounters* h_ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;

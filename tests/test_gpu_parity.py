@@ -195,9 +195,9 @@ def test_empty_and_tiny_graphs_match_oracle(v):
 @pytest.mark.parametrize("v", [x for x in VARIANTS if x[1] == "tiled-coocsr"], ids=vid)
 def test_many_small_tiles_match_oracle(small_graphs, v):
     """Tiles of a few vertices (tiles_per_dim = n / 4) and 8 parallel tile
-    classes: a row meets many tiles of each class, so items take the class
-    query, the predicated row scan and the segment walk; truss numbers,
-    traces and the byte counters equal the oracle's."""
+    classes: a row meets many tiles of each class and its block rows are a
+    few entries long, so items take the register, lane-merge and wave paths;
+    truss numbers, traces and the byte counters equal the oracle's."""
     for name in ("plc300", "gnp80", "mixed"):
         n, s, d = small_graphs[name]
         tiles = max(1, n // 4)
@@ -208,3 +208,26 @@ def test_many_small_tiles_match_oracle(small_graphs, v):
         assert [tuple(x) for x in tr_o] == [(k, it, dl, rc) for k, it, dl, rc in p.trace()], name
         for key in ("count_passes", "recount_edges", "merge_elems", "live_in_passes"):
             assert st[key] == st_o[key], (name, key)
+
+
+@pytest.mark.parametrize("v", [x for x in VARIANTS if x[1] == "tiled-coocsr"], ids=vid)
+@pytest.mark.parametrize("tiles,y", [(3, 1), (4, 4), (6, 4)])
+def test_tile_classes_match_oracle(small_graphs, v, tiles, y):
+    """Block layouts with one destination class (each source tile one block),
+    with as many classes as tiles (the reference's own (tile, tile) blocks,
+    graph.h:67-76) and with tiles grouped into fewer classes: truss numbers,
+    traces and the byte counters equal the oracle's."""
+    for name in ("plc300", "mixed", "known_answer"):
+        n, s, d = small_graphs[name]
+        tau_o, st_o, tr_o = O.decompose(n, s, d, oracle_variant(*v, tiles=tiles), O.SCHED_JUMP | O.SCHED_REUSE)
+        p = K.Plan(K.Graph(n, s, d), gpu_variant(*v, tiles=tiles, y=y))
+        st = p.decompose()
+        assert np.array_equal(p.truss_numbers(), tau_o), name
+        assert [tuple(x) for x in tr_o] == [(k, it, dl, rc) for k, it, dl, rc in p.trace()], name
+        for key in ("count_passes", "recount_edges", "merge_elems", "live_in_passes", "compact_elems"):
+            assert st[key] == st_o[key], (name, key)
+        for k in (3, 5):
+            os_, od_, _, _ = O.ktruss(n, s, d, k, oracle_variant(*v, e=1, tiles=tiles))
+            q = K.Plan(K.Graph(n, s, d), gpu_variant(*v, e=1, tiles=tiles, y=y))
+            gs, gd, _ = q.ktruss(k)
+            assert edge_set(gs, gd) == edge_set(os_, od_), (name, k)
