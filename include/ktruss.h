@@ -323,6 +323,41 @@ int kt_plan_stage_times(const kt_plan* p, double* ms, uint32_t n);
  * data structures").  ms may be NULL. */
 void kt_plan_destroy_timed(kt_plan* p, double* ms);
 
+/* ---- autotuner of one plan ------------------------------------------------
+ * No counterpart in the reference (its shapes come from -b, -t, -y).
+ * Coordinate descent, one pass over each axis in turn:
+ *   tiled plans:   tile size (the plan's own, the reference's n/8, 65536,
+ *                  16384, 4096 vertices), destination classes (-y: 1, 2, 4,
+ *                  8, 16) and workgroup size (256, 512, 1024);
+ *   COO+CSR plans: count strategy (directed: ROW, SEARCH, MERGE; a DELTA plan
+ *                  keeps DELTA and tunes the shape of its opening count),
+ *                  workgroup size, LDS slice (automatic, 512, 1024, 2048) and
+ *                  task size (64, 128, 256).
+ * A COO+CSR plan first tries each of its per-phase candidates
+ * (kt_plan_set_candidates; a DELTA candidate as the count that opens its
+ * runs), so the log holds every candidate's cost on the whole graph: a
+ * training sample of the cost model (learn.py appends them).
+ * A trial is one full support count of the plan's pristine layout from zeroed
+ * supports at k, timed with hipEvents, best of `repeats`; a tiled plan's layout
+ * is rebuilt for each tile shape tried.  The fastest configuration is applied
+ * to the plan (and returned in *best when best is not NULL); every
+ * configuration counts the same supports, so results are unchanged.  Call it
+ * before a run: it clears the last run's results. */
+typedef struct kt_tune_trial {
+    uint32_t tile_size;          /* vertices per tile (tiled) or 0            */
+    uint32_t num_parallel_tiles; /* destination classes (tiled) or 1          */
+    uint32_t count_kernel;       /* KT_COUNT_*                                */
+    uint32_t block_size;
+    uint32_t cap_lds;            /* 0 = automatic                             */
+    uint32_t chunk;
+    float ms;                    /* best of the repeats                       */
+    uint32_t pad;
+} kt_tune_trial;
+int kt_plan_autotune(kt_plan* p, uint32_t k, uint32_t repeats, kt_variant* best);
+/* Trials of the last kt_plan_autotune; *len is their number (all of them
+ * even when cap is smaller). */
+int kt_plan_tune_log(const kt_plan* p, kt_tune_trial* recs, uint64_t cap, uint64_t* len);
+
 /* ---- learned count-kernel picker -----------------------------------------
  * No counterpart in the reference (its variant is fixed per run).  A count
  * configuration is the part of a variant that can change between phases
@@ -355,6 +390,11 @@ typedef struct kt_mlp kt_mlp;
 int kt_plan_set_candidates(kt_plan* p, const kt_count_config* cands, uint32_t n);
 /* model is read when mode is KT_PICK_MODEL (its parameters are copied). */
 int kt_plan_set_pick(kt_plan* p, uint32_t mode, const kt_mlp* model);
+/* Sampled SWEEP (training data from graphs too large to sweep whole): in
+ * KT_PICK_SWEEP mode every candidate is timed only in phases 0, every,
+ * 2*every, ...; the other phases run candidate `fill` alone and log no
+ * per-candidate times.  every = 1 (the default) sweeps every phase. */
+int kt_plan_set_sweep_sampling(kt_plan* p, uint32_t every, uint32_t fill);
 /* Phase log of the last decomposition, KT_PHASE_REC floats per phase:
  * [0] k, [1] iterations, [2] chosen candidate, [3] count ms of the chosen,
  * [4..19] features, [20..35] count ms per candidate (SWEEP; -1 otherwise). */

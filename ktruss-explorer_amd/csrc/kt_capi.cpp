@@ -244,6 +244,32 @@ int kt_plan_stage_times(const kt_plan* p, double* ms, uint32_t n) {
     return KT_OK;
 }
 
+int kt_plan_autotune(kt_plan* p, uint32_t k, uint32_t repeats, kt_variant* best) {
+    if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
+    KT_GUARD({
+        const int rc = p->impl.autotune(k, repeats);
+        if (rc) return rc;
+        if (best) *best = p->impl.v;
+        return KT_OK;
+    })
+}
+
+int kt_plan_tune_log(const kt_plan* p, kt_tune_trial* recs, uint64_t cap, uint64_t* len) {
+    if (!p || !len) return kt::fail(KT_ERR_ARG, "NULL argument");
+    const auto& lg = p->impl.tune_log;
+    *len = lg.size();
+    if (recs)
+        for (uint64_t i = 0; i < std::min<uint64_t>(cap, lg.size()); ++i) recs[i] = lg[i];
+    return KT_OK;
+}
+
+int kt_plan_set_sweep_sampling(kt_plan* p, uint32_t every, uint32_t fill) {
+    if (!p || every == 0) return kt::fail(KT_ERR_ARG, "plan is NULL or every is 0");
+    p->impl.sweep_every = every;
+    p->impl.sweep_fill = fill;
+    return KT_OK;
+}
+
 int kt_plan_set_candidates(kt_plan* p, const kt_count_config* cands, uint32_t n) {
     if (!p) return kt::fail(KT_ERR_ARG, "plan is NULL");
     KT_GUARD(return p->impl.set_candidates(cands, n);)

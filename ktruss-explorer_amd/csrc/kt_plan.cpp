@@ -123,6 +123,86 @@ void Plan::release_timed(double* ms) {
     lap(2);
 }
 
+// Tile size (tile_size_req, else ceil(n / tiles_per_dim), graph.cu:580) and
+// destination classes (at most the tiles) of a tiled plan, and the rows of
+// its stored layout.
+int Plan::set_tiles(uint32_t tile_size_req, uint32_t tiles_per_dim, uint32_t classes) {
+    if (tiled) {
+        uint32_t ts = tile_size_req;
+        if (ts == 0) {
+            if (tiles_per_dim == 0) return fail(KT_ERR_ARG, "tiles_per_dim must be positive");
+            ts = (n + tiles_per_dim - 1) / tiles_per_dim;
+        }
+        if (ts == 0) ts = 1;
+        const uint32_t ntiles = (n + ts - 1) / ts;
+        const uint32_t y = std::max<uint32_t>(1, std::min<uint32_t>(classes ? classes : 1, std::max<uint32_t>(ntiles, 1)));
+        const uint64_t r = tiled_rows(n, ts, y);
+        if (r >= 0xFFFFFFFFull)
+            return fail(KT_ERR_CAPACITY, "tiled format: tiles x parallel tiles x tile size must stay below 2^32");
+        tile_size = ts;
+        ytiles = y;
+        rows = (uint32_t)r;
+    } else {
+        tile_size = n ? n : 1;
+        ytiles = 1;
+        rows = n;
+    }
+    return KT_OK;
+}
+
+// The stored layout (pristine copy) of the plan's orientation and format,
+// built on the device from the graph's edges; its row-pointer arrays are
+// sized for the current rows.
+int Plan::build_layout(const kt_graph* g) {
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
+    int rc;
+    const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
+    const size_t Rb = ((size_t)rows + 1) * sizeof(uint32_t);  // row pointers of the stored layout
+    if (ptr0.size != Rb) {
+        for (DBuf* x : {&ptr0, &ptr[0], &ptr[1]}) {
+            x->release();
+            if ((rc = x->alloc(Rb))) return rc;
+        }
+    }
+    if (tiled && !directed && !rowlen.p && (rc = rowlen.alloc(std::max<size_t>((size_t)n * 4, 4)))) return rc;
+    const uint32_t m = (uint32_t)g->m;
+    const size_t Mb = (size_t)std::max<uint32_t>(m, 1) * sizeof(uint32_t);
+    DBuf da, db, ddeg, dn2o, dpos, dmax;
+    auto t0 = clk::now();
+    if ((rc = da.alloc(Mb)) || (rc = db.alloc(Mb)) || (rc = ddeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
+        (rc = dn2o.alloc(std::max<size_t>(Pb - 4, 4))) || (rc = dpos.alloc(Mb)) || (rc = dmax.alloc(4)))
+        return rc;
+    if (m) {
+        KT_HIP_TRY(hipMemcpyAsync(da.p, g->ua.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
+        KT_HIP_TRY(hipMemcpyAsync(db.p, g->ub.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
+    }
+    if (n) KT_HIP_TRY(hipMemcpyAsync(ddeg.p, g->degree.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream));
+    KT_HIP_TRY(hipStreamSynchronize(stream));
+    stage_ms[1] = ms_since(t0);
+    double build_ms[3] = {0, 0, 0};  // relabel, orientation, rows
+    KT_HIP_TRY(dev::build_layout_device(n, m, da.as<uint32_t>(), db.as<uint32_t>(), v.directedness,
+                                        ddeg.as<uint32_t>(), g->max_degree, ptr0.as<uint32_t>(),
+                                        src0.as<uint32_t>(), dst0.as<uint32_t>(), dn2o.as<uint32_t>(),
+                                        dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream, build_ms,
+                                        tiled ? tile_size : 0, tiled ? ytiles : 0));
+    stage_ms[0] = build_ms[0];
+    stage_ms[3] = build_ms[1];
+    stage_ms[4] = build_ms[2];
+    t0 = clk::now();
+    new2old.resize(n);
+    pos_of_edge.resize(m);
+    if (n) KT_HIP_TRY(hipMemcpy(new2old.data(), dn2o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (m) KT_HIP_TRY(hipMemcpy(pos_of_edge.data(), dpos.p, (size_t)m * 4, hipMemcpyDeviceToHost));
+    KT_HIP_TRY(hipMemcpy(&max_row, dmax.p, 4, hipMemcpyDeviceToHost));
+    stage_ms[4] += ms_since(t0);  // the rows' maps back to the edges
+    t0 = clk::now();
+    for (DBuf* b : {&da, &db, &ddeg, &dn2o, &dpos, &dmax}) b->release();
+    stage_ms[5] = ms_since(t0);
+    have_tau = false;
+    return KT_OK;
+}
+
 int Plan::create(const kt_graph* g, const kt_variant* var) {
     int rc = ensure_device();
     if (rc) return rc;
@@ -145,28 +225,7 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         n = g->n;
         E0 = (uint32_t)stored;
     }
-    if (tiled) {
-        if (v.tile_size == 0) {
-            if (v.tiles_per_dim == 0) return fail(KT_ERR_ARG, "tiles_per_dim must be positive");
-            tile_size = (n + v.tiles_per_dim - 1) / v.tiles_per_dim;  // graph.cu:580
-        } else {
-            tile_size = v.tile_size;
-        }
-        if (tile_size == 0) tile_size = 1;
-        const uint32_t ntiles = (n + tile_size - 1) / tile_size;
-        ytiles = std::max<uint32_t>(1, std::min<uint32_t>(v.num_parallel_tiles ? v.num_parallel_tiles : 1,
-                                                          std::max<uint32_t>(ntiles, 1)));
-    } else {
-        tile_size = n ? n : 1;
-        ytiles = 1;
-    }
-    rows = n;
-    if (tiled) {
-        const uint64_t r = tiled_rows(n, tile_size, ytiles);
-        if (r >= 0xFFFFFFFFull)
-            return fail(KT_ERR_CAPACITY, "tiled format: tiles x parallel tiles x tile size must stay below 2^32");
-        rows = (uint32_t)r;
-    }
+    if ((rc = set_tiles(v.tile_size, v.tiles_per_dim, v.num_parallel_tiles))) return rc;
     query_factor = 8;
 
     KT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -187,12 +246,9 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
     std::fill(stage_ms, stage_ms + KT_PLAN_STAGES, 0.0);
     auto t0 = clk::now();
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * sizeof(uint32_t);
-    const size_t Pb = ((size_t)n + 1) * sizeof(uint32_t);
-    const size_t Rb = ((size_t)rows + 1) * sizeof(uint32_t);  // row pointers of the stored layout
-    if ((rc = ptr0.alloc(Rb)) || (rc = src0.alloc(Eb)) || (rc = dst0.alloc(Eb))) return rc;
+    if ((rc = src0.alloc(Eb)) || (rc = dst0.alloc(Eb))) return rc;
     for (int b = 0; b < 2; ++b)
-        if ((rc = ptr[b].alloc(Rb)) || (rc = src[b].alloc(Eb)) || (rc = dst[b].alloc(Eb)) ||
-            (rc = eid[b].alloc(Eb)) || (rc = sup[b].alloc(Eb)))
+        if ((rc = src[b].alloc(Eb)) || (rc = dst[b].alloc(Eb)) || (rc = eid[b].alloc(Eb)) || (rc = sup[b].alloc(Eb)))
             return rc;
     if ((rc = state.alloc(std::max<uint32_t>(E0, 1))) || (rc = nflag.alloc(std::max<uint32_t>(n, 1))) ||
         (rc = wl.alloc(Eb)) || (rc = tau.alloc(Eb)) ||
@@ -202,43 +258,8 @@ int Plan::create(const kt_graph* g, const kt_variant* var) {
         (rc = stripes_x.alloc(kStripeBytes)) ||
         (rc = feat_acc.alloc((size_t)KT_FEAT_BLOCKS * KT_FEAT_ACC * sizeof(unsigned long long))))
         return rc;
-    if (tiled && !directed && (rc = rowlen.alloc(std::max<size_t>((size_t)n * 4, 4)))) return rc;
     stage_ms[2] = ms_since(t0);  // the peel state (Info, info.cu:22-35) and the layouts
-    {
-        const uint32_t m = (uint32_t)g->m;
-        const size_t Mb = (size_t)std::max<uint32_t>(m, 1) * sizeof(uint32_t);
-        DBuf da, db, ddeg, dn2o, dpos, dmax;
-        t0 = clk::now();
-        if ((rc = da.alloc(Mb)) || (rc = db.alloc(Mb)) || (rc = ddeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
-            (rc = dn2o.alloc(std::max<size_t>(Pb - 4, 4))) || (rc = dpos.alloc(Mb)) || (rc = dmax.alloc(4)))
-            return rc;
-        if (m) {
-            KT_HIP_TRY(hipMemcpyAsync(da.p, g->ua.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
-            KT_HIP_TRY(hipMemcpyAsync(db.p, g->ub.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
-        }
-        if (n) KT_HIP_TRY(hipMemcpyAsync(ddeg.p, g->degree.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream));
-        KT_HIP_TRY(hipStreamSynchronize(stream));
-        stage_ms[1] = ms_since(t0);
-        double build_ms[3] = {0, 0, 0};  // relabel, orientation, rows
-        KT_HIP_TRY(dev::build_layout_device(n, m, da.as<uint32_t>(), db.as<uint32_t>(), v.directedness,
-                                            ddeg.as<uint32_t>(), g->max_degree, ptr0.as<uint32_t>(),
-                                            src0.as<uint32_t>(), dst0.as<uint32_t>(), dn2o.as<uint32_t>(),
-                                            dpos.as<uint32_t>(), dmax.as<uint32_t>(), stream, build_ms,
-                                            tiled ? tile_size : 0, tiled ? ytiles : 0));
-        stage_ms[0] = build_ms[0];
-        stage_ms[3] = build_ms[1];
-        stage_ms[4] = build_ms[2];
-        t0 = clk::now();
-        new2old.resize(n);
-        pos_of_edge.resize(m);
-        if (n) KT_HIP_TRY(hipMemcpy(new2old.data(), dn2o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
-        if (m) KT_HIP_TRY(hipMemcpy(pos_of_edge.data(), dpos.p, (size_t)m * 4, hipMemcpyDeviceToHost));
-        KT_HIP_TRY(hipMemcpy(&max_row, dmax.p, 4, hipMemcpyDeviceToHost));
-        stage_ms[4] += ms_since(t0);  // the rows' maps back to the edges
-        t0 = clk::now();
-        for (DBuf* b : {&da, &db, &ddeg, &dn2o, &dpos, &dmax}) b->release();
-        stage_ms[5] = ms_since(t0);
-    }
+    if ((rc = build_layout(g))) return rc;
     KT_HIP_TRY(dev::fill_u32(stripes.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(dev::fill_u32(stripes_x.as<uint32_t>(), 0, kStripeBytes / 4, stream));
     KT_HIP_TRY(hipStreamSynchronize(stream));
@@ -417,10 +438,10 @@ CountArgs Plan::count_args(uint32_t k, bool use_wl, uint32_t nitems, const Count
 // and, in MODEL mode, the cost model's pick among the candidates.
 int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bool defer, bool live_rows,
                      uint32_t* host_choice) {
-    cfg = fixed_cfg;
+    cfg = phase_fixed;
     chosen = 0;
     phase_has_feat = false;
-    if (pick == KT_PICK_FIXED) return KT_OK;  // no pick to make: the phase log gets no features
+    if (pick_now == KT_PICK_FIXED) return KT_OK;  // no pick to make: the phase log gets no features
     // the phase's features stay on the device (feat_log, one row per phase)
     // until the run ends
     const size_t ph = phase_log.size();
@@ -437,7 +458,7 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bo
         std::swap(grown.size, feat_log.size);
         feat_cap = cap;
     }
-    const bool by_model = pick == KT_PICK_MODEL;
+    const bool by_model = pick_now == KT_PICK_MODEL;
     KT_HIP_TRY(dev::phase_features(ptr[cur].as<uint32_t>(), live_rows ? ldeg.as<uint32_t>() : nullptr,
                                    dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), rows, E, E0, k, exact,
                                    feat_acc.as<unsigned long long>(), feat_log.as<float>() + ph * KT_MLP_IN,
@@ -456,7 +477,20 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bo
 
 // One phase at k: the peel loop of kernel_ddc.cu:116-214 (and its tiled /
 // undirected twins) on the current graph.
+// The phase's pick mode and fixed configuration: a sampled SWEEP times every
+// candidate only in every sweep_every-th phase and runs candidate sweep_fill
+// alone in the others.
+void Plan::begin_phase() {
+    pick_now = pick;
+    phase_fixed = fixed_cfg;
+    if (pick == KT_PICK_SWEEP && sweep_every > 1 && phase_log.size() % sweep_every != 0 && !cands.empty()) {
+        pick_now = KT_PICK_FIXED;
+        phase_fixed = cands[std::min<size_t>(sweep_fill, cands.size() - 1)];
+    }
+}
+
 int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out) {
+    begin_phase();
     const bool jump = directed && (schedule & KT_SCHED_JUMP);
     uint32_t iter = 0;
     bool has_del = false, broke = false, changed = false;
@@ -470,7 +504,7 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     const uint32_t nc = (uint32_t)cands.size();
     std::array<float, KT_PHASE_REC> rec;
     rec.fill(-1.f);
-    CountCfg cfg = fixed_cfg;
+    CountCfg cfg = phase_fixed;
     uint32_t chosen = 0;
     if ((rc = phase_pick(k, reuse, cfg, chosen))) return rc;
     std::vector<double> cand_ms(nc, 0.0);
@@ -482,7 +516,7 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
         const uint64_t rec_before = h_ctr->recount_edges;
         if (!(iter == 0 && reuse)) {
             const uint32_t items = use_wl ? nitems : E;
-            if (pick == KT_PICK_SWEEP) {
+            if (pick_now == KT_PICK_SWEEP) {
                 // every candidate from zeroed supports; the last one's result is kept
                 for (uint32_t c = 0; c < nc; ++c) {
                     if (use_wl) KT_HIP_TRY(dev::zero_listed_supports(wl.as<uint32_t>(), nitems, sup[cur].as<uint32_t>(), stream));
@@ -528,7 +562,7 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
             irec[10] |= KT_ITER_MARK_TIMED;
         }
         float count_ms = 0;
-        if (counted && pick == KT_PICK_SWEEP) {
+        if (counted && pick_now == KT_PICK_SWEEP) {
             float best = 0;
             for (uint32_t c = 0; c < nc; ++c) {
                 float ms = 0;
@@ -609,7 +643,7 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     }
     exact_out = !broke && !changed;
     // phase record
-    if (pick == KT_PICK_SWEEP && nc) {
+    if (pick_now == KT_PICK_SWEEP && nc) {
         chosen = 0;
         for (uint32_t c = 1; c < nc; ++c)
             if (cand_ms[c] < cand_ms[chosen]) chosen = c;
@@ -618,7 +652,7 @@ int Plan::run_phase(uint32_t& k, bool reuse, uint32_t schedule, bool& exact_out)
     }
     rec[0] = (float)k;
     rec[1] = (float)iter + (broke ? 1.f : 0.f);
-    rec[2] = pick == KT_PICK_FIXED ? -1.f : (float)chosen;
+    rec[2] = pick_now == KT_PICK_FIXED ? -1.f : (float)chosen;
     rec[3] = (float)phase_ms;
     phase_log.push_back(rec);  // features: filled in by flush_phase_features
     feat_on.push_back(phase_has_feat ? 1 : 0);
@@ -1188,6 +1222,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
 // either removes their triangles (DELTA) or recounts.  The dead entries are
 // compacted away once they are a third of the stored ones.
 int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
+    begin_phase();
     const bool jump = (schedule & KT_SCHED_JUMP) != 0;
     uint32_t iter = 0;
     bool broke = false, changed = false;
@@ -1196,12 +1231,12 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     const uint32_t nc = (uint32_t)cands.size();
     std::array<float, KT_PHASE_REC> rec;
     rec.fill(-1.f);
-    CountCfg cfg = fixed_cfg;
+    CountCfg cfg = phase_fixed;
     uint32_t chosen = 0;
     // the model's choice is read from pinned memory at an early phase start
     // (below), else with the next counter read-back (the JUMP's or, when
     // deletions are pending, right before the support pass)
-    bool unresolved = pick == KT_PICK_MODEL;
+    bool unresolved = pick_now == KT_PICK_MODEL;
     // with nothing pending the deletion count is free to clear now; the JUMP
     // minimum is reset here too (one kernel instead of three)
     const bool start_clear = pending == 0;
@@ -1214,8 +1249,8 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     // its last workgroup, so the host queues the phase's first mark while the
     // features and the model's pick run; the choice arrives the same way.  The
     // phase then starts without a counter read-back.
-    const bool early = jump && start_clear && reuse && pick != KT_PICK_SWEEP && k > 3 &&
-                       (pick == KT_PICK_MODEL || fixed_cfg.kernel == KT_COUNT_DELTA);
+    const bool early = jump && start_clear && reuse && pick_now != KT_PICK_SWEEP && k > 3 &&
+                       (pick_now == KT_PICK_MODEL || phase_fixed.kernel == KT_COUNT_DELTA);
     if (early) {
         __atomic_store_n(h_ra + kSlotMin, kCountPending, __ATOMIC_RELEASE);
         __atomic_store_n(h_ra + kSlotChoice, kCountPending, __ATOMIC_RELEASE);
@@ -1273,7 +1308,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
                 if ((rc = sync_counters())) return rc;
                 resolve_choice();
             }
-            if (pick == KT_PICK_SWEEP) rc = sweep_pass(k, cand_ms, dummy);
+            if (pick_now == KT_PICK_SWEEP) rc = sweep_pass(k, cand_ms, dummy);
             else rc = support_pass(cfg, k, dummy);
             if (rc) return rc;
             counted = true;
@@ -1290,7 +1325,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             // with DELTA settled for the phase, iteration 0's mark (it deletes
             // the minimum) opens the run-ahead queue: its count comes back
             // with the next iteration's
-            if (s != 0xFFFFFFFFu && k > 3 && !counted && pick != KT_PICK_SWEEP && !unresolved &&
+            if (s != 0xFFFFFFFFu && k > 3 && !counted && pick_now != KT_PICK_SWEEP && !unresolved &&
                 cfg.kernel == KT_COUNT_DELTA && !compaction_due()) {
                 KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                            sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E,
@@ -1312,7 +1347,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;
-        if (counted && pick == KT_PICK_SWEEP) {
+        if (counted && pick_now == KT_PICK_SWEEP) {
             float best = 0;
             for (uint32_t c = 0; c < nc; ++c) {
                 float ms = 0;
@@ -1324,7 +1359,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         } else if (counted) {
             KT_HIP_TRY(hipEventElapsedTime(&count_ms, ev_c0, ev_c1));
         }
-        if (counted && pick != KT_PICK_SWEEP && last_pass_delta) {
+        if (counted && pick_now != KT_PICK_SWEEP && last_pass_delta) {
             stats.ms_delta += count_ms;
         } else if (counted) {
             stats.ms_count += count_ms;
@@ -1334,7 +1369,7 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         const uint32_t ndel = h_ctr->ndel;
         pending = ndel;
         const uint32_t fl = counted ? (KT_ITER_COUNTED | KT_ITER_COUNT_TIMED |
-                                       (pick != KT_PICK_SWEEP && last_pass_delta ? (uint32_t)KT_ITER_DELTA : 0u))
+                                       (pick_now != KT_PICK_SWEEP && last_pass_delta ? (uint32_t)KT_ITER_DELTA : 0u))
                                     : 0u;
         if ((rc = record_iteration(k, iter, ndel, h_ctr->recount_edges - rec_before, count_ms, fl))) return rc;
         changed = ndel > 0;
@@ -1345,14 +1380,14 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         ++iter;
         // with DELTA settled for the phase, the next iterations are queued
         // ahead of their read-backs until the phase ends or a compaction is due
-        if (changed && pick != KT_PICK_SWEEP && !unresolved && cfg.kernel == KT_COUNT_DELTA &&
+        if (changed && pick_now != KT_PICK_SWEEP && !unresolved && cfg.kernel == KT_COUNT_DELTA &&
             !compaction_due()) {
             bool ended = false;
             if ((rc = run_ahead_delta(k, iter, phase_ms, ended))) return rc;
             if (ended) changed = false;
         }
     } while (changed);
-    if (pick == KT_PICK_SWEEP && nc) {
+    if (pick_now == KT_PICK_SWEEP && nc) {
         chosen = 0;
         for (uint32_t c = 1; c < nc; ++c)
             if (cand_ms[c] < cand_ms[chosen]) chosen = c;
@@ -1361,12 +1396,123 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     }
     rec[0] = (float)k;
     rec[1] = (float)iter + (broke ? 1.f : 0.f);
-    rec[2] = pick == KT_PICK_FIXED ? -1.f : (float)chosen;
+    rec[2] = pick_now == KT_PICK_FIXED ? -1.f : (float)chosen;
     rec[3] = (float)phase_ms;
     phase_log.push_back(rec);  // features: filled in by flush_phase_features
     feat_on.push_back(phase_has_feat ? 1 : 0);
     stats.k_last = k;
     return KT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner
+
+// Times trial t (its ms filled in): the layout is rebuilt when t's tile shape
+// is not the current one; one support count of the pristine layout at k,
+// best of `repeats`.
+int Plan::tune_trial(kt_tune_trial& t, uint32_t k, uint32_t repeats) {
+    int rc;
+    if (tiled && (t.tile_size != tile_size || t.num_parallel_tiles != ytiles)) {
+        if ((rc = set_tiles(t.tile_size, 0, t.num_parallel_tiles))) return rc;
+        if ((rc = build_layout(graph))) return rc;
+        t.tile_size = tile_size;
+        t.num_parallel_tiles = ytiles;
+    }
+    CountCfg c;
+    const kt_count_config in{t.count_kernel == KT_COUNT_DELTA ? (uint32_t)KT_COUNT_ROW : t.count_kernel,
+                             t.block_size, t.cap_lds, t.chunk};
+    if ((rc = resolve(in, c))) return rc;
+    if ((rc = reset())) return rc;
+    float best = -1.f;
+    for (uint32_t r = 0; r < std::max<uint32_t>(repeats, 1); ++r) {
+        KT_HIP_TRY(dev::zero_live_supports(dst[cur].as<uint32_t>(), sup[cur].as<uint32_t>(), E, stream));
+        CountArgs a = count_args(k, false, E, c, ctr_scratch.as<DevCounters>());
+        KT_HIP_TRY(hipEventRecord(ev_c0, stream));
+        KT_HIP_TRY(dev::count_supports(a, directed, tiled, c.kernel, c.block, stream));
+        KT_HIP_TRY(hipEventRecord(ev_c1, stream));
+        KT_HIP_TRY(hipEventSynchronize(ev_c1));
+        float ms = 0;
+        KT_HIP_TRY(hipEventElapsedTime(&ms, ev_c0, ev_c1));
+        if (best < 0 || ms < best) best = ms;
+    }
+    t.ms = best;
+    tune_log.push_back(t);
+    return KT_OK;
+}
+
+int Plan::autotune(uint32_t k, uint32_t repeats) {
+    if (k < 2) return fail(KT_ERR_ARG, "k must be at least 2");
+    tune_log.clear();
+    int rc;
+    kt_tune_trial best{};
+    best.tile_size = tiled ? tile_size : 0;
+    best.num_parallel_tiles = ytiles;
+    best.count_kernel = fixed_cfg.kernel;
+    best.block_size = fixed_cfg.block;
+    best.cap_lds = v.cap_lds;
+    best.chunk = fixed_cfg.chunk;
+    if ((rc = tune_trial(best, k, repeats))) return rc;
+    // the plan's per-phase candidates first (a DELTA candidate as the count
+    // that opens its runs): the log then holds every candidate's cost on the
+    // whole graph, a sample for the cost model
+    if (!tiled) {
+        for (const CountCfg& c : cands) {
+            kt_tune_trial t = best;
+            t.count_kernel = c.kernel;
+            t.block_size = c.block;
+            t.cap_lds = c.cap;
+            t.chunk = c.chunk;
+            if ((rc = tune_trial(t, k, repeats))) return rc;
+            if (t.ms < best.ms && (c.kernel != KT_COUNT_DELTA || fixed_cfg.kernel == KT_COUNT_DELTA)) best = t;
+        }
+    }
+    // one axis at a time: each value tried against the best so far
+    auto axis = [&](uint32_t kt_tune_trial::*field, std::vector<uint32_t> values) -> int {
+        for (uint32_t x : values) {
+            if (best.*field == x) continue;
+            kt_tune_trial t = best;
+            t.*field = x;
+            int r = tune_trial(t, k, repeats);
+            if (r == KT_ERR_ARG || r == KT_ERR_CAPACITY) continue;  // not a valid shape for this plan
+            if (r) return r;
+            if (t.ms < best.ms) best = t;
+        }
+        return KT_OK;
+    };
+    if (tiled) {
+        std::vector<uint32_t> ts;
+        for (uint32_t x : {tile_size, (n + 7) / 8, 65536u, 16384u, 4096u})
+            if (x > 0 && x <= std::max<uint32_t>(n, 1) && std::find(ts.begin(), ts.end(), x) == ts.end()) ts.push_back(x);
+        if ((rc = axis(&kt_tune_trial::tile_size, ts))) return rc;
+        if ((rc = axis(&kt_tune_trial::num_parallel_tiles, {1u, 2u, 4u, 8u, 16u}))) return rc;
+        if ((rc = axis(&kt_tune_trial::block_size, {256u, 512u, 1024u}))) return rc;
+    } else {
+        if (directed && fixed_cfg.kernel != KT_COUNT_DELTA &&
+            (rc = axis(&kt_tune_trial::count_kernel, {(uint32_t)KT_COUNT_ROW, (uint32_t)KT_COUNT_SEARCH,
+                                                      (uint32_t)KT_COUNT_MERGE})))
+            return rc;
+        if ((rc = axis(&kt_tune_trial::block_size, {256u, 512u, 1024u}))) return rc;
+        if ((rc = axis(&kt_tune_trial::cap_lds, {0u, 512u, 1024u, 2048u}))) return rc;
+        if ((rc = axis(&kt_tune_trial::chunk, {64u, 128u, 256u}))) return rc;
+    }
+    // apply the best: its tile shape (layout rebuilt if the last trial left
+    // another) and its count configuration
+    if (tiled && (best.tile_size != tile_size || best.num_parallel_tiles != ytiles)) {
+        if ((rc = set_tiles(best.tile_size, 0, best.num_parallel_tiles))) return rc;
+        if ((rc = build_layout(graph))) return rc;
+    }
+    const kt_count_config in{best.count_kernel, best.block_size, best.cap_lds, best.chunk};
+    if ((rc = resolve(in, fixed_cfg))) return rc;
+    v.count_kernel = fixed_cfg.kernel;
+    v.block_size = fixed_cfg.block;
+    v.cap_lds = best.cap_lds;
+    v.chunk = fixed_cfg.chunk;
+    cap_lds = fixed_cfg.cap;
+    if (tiled) {
+        v.tile_size = tile_size;
+        v.num_parallel_tiles = ytiles;
+    }
+    return reset();
 }
 
 int Plan::truss_numbers(uint32_t* out, uint64_t count) {

@@ -83,6 +83,11 @@ struct Plan {
     CountCfg fixed_cfg;
     std::vector<CountCfg> cands;
     uint32_t pick = KT_PICK_FIXED;
+    // sampled SWEEP (kt_plan_set_sweep_sampling) and the current phase's mode
+    uint32_t sweep_every = 1, sweep_fill = 0;
+    uint32_t pick_now = KT_PICK_FIXED;
+    CountCfg phase_fixed;
+    void begin_phase();
     DBuf model;          // cost-model parameters (KT_PICK_MODEL)
     uint32_t model_C = 0, model_n = 0;
     DBuf feat_acc, ctr_scratch;
@@ -130,6 +135,12 @@ struct Plan {
     // plan can only be destroyed afterwards.
     void release_timed(double* ms);
     int create(const kt_graph* g, const kt_variant* v);
+    int set_tiles(uint32_t tile_size_req, uint32_t tiles_per_dim, uint32_t classes);
+    int build_layout(const kt_graph* g);
+    // kt_plan_autotune
+    std::vector<kt_tune_trial> tune_log;
+    int autotune(uint32_t k, uint32_t repeats);
+    int tune_trial(kt_tune_trial& t, uint32_t k, uint32_t repeats);
     int reset();
     int decompose();
     int ktruss(uint32_t k, std::vector<uint32_t>& out_src, std::vector<uint32_t>& out_dst);

@@ -43,7 +43,7 @@ EXPORTS = [
     "kt_plan_info", "kt_plan_ktruss", "kt_plan_decompose", "kt_plan_truss_numbers",
     "kt_plan_trace", "kt_plan_set_candidates", "kt_plan_set_pick", "kt_plan_phase_log",
     "kt_plan_set_timing", "kt_ktruss_noalloc", "kt_plan_iteration_log", "kt_plan_stage_times",
-    "kt_plan_destroy_timed",
+    "kt_plan_destroy_timed", "kt_plan_set_sweep_sampling", "kt_plan_autotune", "kt_plan_tune_log",
     "kt_mlp_create", "kt_mlp_destroy", "kt_mlp_num_params", "kt_mlp_get_params",
     "kt_mlp_set_params", "kt_mlp_forward", "kt_mlp_loss_grad", "kt_mlp_train",
     "kt_mlp_set_input_range", "kt_mlp_get_input_range",
@@ -91,6 +91,16 @@ class CountConfig(ctypes.Structure):
         names = {0: "auto", 1: "row", 2: "search", 3: "merge", 4: "delta"}
         chunk = self.chunk or (64 if self.count_kernel == COUNT_KERNELS["delta"] else 128)
         return f"{names[self.count_kernel]}/b{self.block_size}/c{self.cap_lds or 'auto'}/k{chunk}"
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("tile_size", ctypes.c_uint32), ("num_parallel_tiles", ctypes.c_uint32),
+                ("count_kernel", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
+                ("cap_lds", ctypes.c_uint32), ("chunk", ctypes.c_uint32), ("ms", ctypes.c_float),
+                ("pad", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
 class Stats(ctypes.Structure):
@@ -223,6 +233,9 @@ def lib():
         L.kt_batch_set_timing.argtypes = [_vp, ctypes.c_int]
         L.kt_batch_truss_numbers.argtypes = [_vp, ctypes.c_uint32, _u32p, ctypes.c_uint64]
         L.kt_batch_graph_info.argtypes = [_vp, ctypes.c_uint32, _u32p]
+        L.kt_plan_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Variant)]
+        L.kt_plan_tune_log.argtypes = [ctypes.c_void_p, ctypes.POINTER(TuneTrial), ctypes.c_uint64, _u64p]
+        L.kt_plan_set_sweep_sampling.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
         L.kt_ktruss_noalloc.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_uint32,
                                         _vp, _u64p]
         _lib = L
@@ -413,6 +426,24 @@ class Plan:
     def set_pick(self, mode, model=None):
         _check(lib().kt_plan_set_pick(self._h, mode, model._h if model is not None else None),
                "kt_plan_set_pick")
+
+    def autotune(self, k=3, repeats=1):
+        """kt_plan_autotune: applies and returns the fastest variant found."""
+        best = Variant()
+        _check(lib().kt_plan_autotune(self._h, k, repeats, ctypes.byref(best)), "kt_plan_autotune")
+        return best
+
+    def tune_log(self):
+        n = ctypes.c_uint64()
+        _check(lib().kt_plan_tune_log(self._h, None, 0, ctypes.byref(n)), "kt_plan_tune_log")
+        recs = (TuneTrial * max(n.value, 1))()
+        _check(lib().kt_plan_tune_log(self._h, recs, n.value, ctypes.byref(n)), "kt_plan_tune_log")
+        return [recs[i].as_dict() for i in range(n.value)]
+
+    def set_sweep_sampling(self, every, fill=0):
+        """SWEEP times every candidate only in every `every`-th phase; the
+        others run candidate `fill` alone."""
+        _check(lib().kt_plan_set_sweep_sampling(self._h, every, fill), "kt_plan_set_sweep_sampling")
 
     def phase_log(self):
         """(phases, 36) float32: k, iterations, chosen, phase count ms, 16 features,

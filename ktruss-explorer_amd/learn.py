@@ -34,6 +34,9 @@ sys.path.insert(0, HERE)
 import ktruss_amd as K  # noqa: E402
 
 MODEL_PATH = os.path.join(HERE, "picker", "count_picker.npz")
+HELDOUT_PATH = os.path.join(HERE, "picker", "heldout_phases.npz")
+# graphs held out of training: index % HOLDOUT_MOD == HOLDOUT_REM in the seeded list
+HOLDOUT_MOD, HOLDOUT_REM = 7, 3
 MODEL_BIN_PATH = os.path.join(HERE, "picker", "count_picker.ktm")
 
 # Candidate support configurations; index 0 is the default variant's own.
@@ -85,6 +88,33 @@ def graph_specs(count, seed, max_scale=17, min_scale=10):
     return specs
 
 
+def graph_specs_c5(count, seed, min_scale=12, max_scale=20):
+    """The training set of the learned pick: seeded R-MAT and Chung-Lu graphs
+    (with planted cliques), alternating, scales uniform in [min_scale,
+    max_scale]."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        scale = int(rng.integers(min_scale, max_scale + 1))
+        gseed = int(rng.integers(1, 1 << 30))
+        if i % 2 == 0:
+            a = float(rng.uniform(0.45, 0.6))
+            bc = (1 - a) * float(rng.uniform(0.35, 0.45))
+            specs.append({"kind": "rmat", "scale": scale, "ef": int(rng.choice([8, 16, 24])),
+                          "a": a, "b": bc, "c": bc, "seed": gseed})
+        else:
+            specs.append({"kind": "cl", "n": 1 << scale, "gamma": float(rng.uniform(2.0, 2.8)),
+                          "deg": float(rng.uniform(8, 32)), "maxw": float(2 ** rng.uniform(8, 14)),
+                          "cliques": int(rng.integers(0, 17)), "seed": gseed, "scale": scale})
+    return specs
+
+
+def sweep_every_for(scale):
+    """Phases between timed sweeps: every phase up to scale 15, then every
+    2nd, 4th, 8th as a full sweep of the larger graphs grows too costly."""
+    return 1 if scale <= 15 else 2 if scale <= 17 else 4 if scale <= 18 else 8
+
+
 def make_graph(spec):
     if spec["kind"] == "rmat":
         n, s, d = K.generate_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"],
@@ -100,22 +130,60 @@ def make_graph(spec):
 # ---------------------------------------------------------------------------
 # data
 
-def sweep(graph, variant=None):
-    """SWEEP decomposition: (truss numbers, phase log)."""
+def sweep(graph, variant=None, every=1):
+    """SWEEP decomposition (every candidate timed in every `every`-th phase,
+    the others run by DELTA): (truss numbers, phase log)."""
     p = K.Plan(graph, variant or K.default_variant())
     p.set_candidates(candidate_configs())
     p.set_pick(K.PICK_SWEEP)
+    if every > 1:
+        p.set_sweep_sampling(every, len(CANDIDATES) - 1)
     p.decompose()
     return p.truss_numbers(), p.phase_log()
 
 
-def collect(count, seed, max_scale=17, min_scale=10):
+def tuner_sample(graph, k=3):
+    """The autotuner's log as a training sample: kt_plan_autotune first times
+    every candidate's count on the whole graph (the phase that opens the
+    decomposition); their costs, or None when one is missing."""
+    p = K.Plan(graph, K.default_variant())
+    p.set_candidates(candidate_configs())
+    p.autotune(k=k, repeats=1)
+    lg = p.tune_log()[1:1 + len(CANDIDATES)]
+    ms = np.array([t["ms"] for t in lg], np.float32)
+    return ms if len(ms) == len(CANDIDATES) and np.all(ms > 0) else None
+
+
+def tuner_samples(count, seed, min_scale, max_scale, start, stop):
+    """Autotuner samples (features of the opening phase from a sweep of it
+    alone) of graphs [start, stop) of the seeded C5 list."""
     feats, costs, meta = [], [], []
-    specs = graph_specs(count, seed, max_scale, min_scale)
+    specs = graph_specs_c5(count, seed, min_scale, max_scale)
     t0 = time.time()
-    for i, spec in enumerate(specs):
+    for i in range(start, min(stop, len(specs))):
+        g = make_graph(specs[i])
+        _, ph = sweep(g, every=1 << 30)
+        ts = tuner_sample(g)
+        if len(ph) and ts is not None:
+            feats.append(ph[0, 4:20])
+            costs.append(ts)
+            meta.append((i, -1, 0))
+        log(f"[{i + 1}/{len(specs)}] tuner sample, m={g.m}, elapsed={time.time() - t0:.0f}s")
+    return np.array(feats, np.float32), np.array(costs, np.float32), np.array(meta, np.float32)
+
+
+def collect(count, seed, max_scale=17, min_scale=10, c5=False, start=0, stop=None):
+    """Samples of graphs [start, stop) of the seeded list; meta holds (graph
+    index, k, iterations) per sample."""
+    feats, costs, meta = [], [], []
+    specs = (graph_specs_c5(count, seed, min_scale, max_scale) if c5
+             else graph_specs(count, seed, max_scale, min_scale))
+    t0 = time.time()
+    for i in range(start, len(specs) if stop is None else min(stop, len(specs))):
+        spec = specs[i]
         g = make_graph(spec)
-        _, ph = sweep(g)
+        every = sweep_every_for(spec.get("scale", 0)) if c5 else 1
+        _, ph = sweep(g, every=every)
         C = len(CANDIDATES)
         for r in ph:
             ms = r[20:20 + C]
@@ -123,7 +191,13 @@ def collect(count, seed, max_scale=17, min_scale=10):
                 feats.append(r[4:20])
                 costs.append(ms)
                 meta.append((i, r[0], r[1]))
-        log(f"[{i + 1}/{len(specs)}] {spec['kind']} m={g.m} phases={len(ph)} "
+        if c5 and len(ph):
+            ts = tuner_sample(g)
+            if ts is not None:
+                feats.append(ph[0, 4:20])
+                costs.append(ts)
+                meta.append((i, -1, 0))  # k = -1: the autotuner's sample
+        log(f"[{i + 1}/{len(specs)}] {spec['kind']} scale={spec.get('scale', '-')} m={g.m} phases={len(ph)} "
             f"samples={len(feats)} elapsed={time.time() - t0:.0f}s")
     return (np.array(feats, np.float32), np.array(costs, np.float32),
             np.array(meta, np.float32), specs)
@@ -288,10 +362,26 @@ def main():
     c.add_argument("--seed", type=int, default=7)
     c.add_argument("--max-scale", type=int, default=17)
     c.add_argument("--min-scale", type=int, default=10)
+    c.add_argument("--c5", action="store_true",
+                   help="R-MAT / Chung-Lu graphs, scales uniform in [min, max], sampled sweeps above scale 15")
+    c.add_argument("--start", type=int, default=0)
+    c.add_argument("--stop", type=int, default=None)
+    u = sub.add_parser("tune", help="autotuner samples of the C5 graphs [start, stop)")
+    u.add_argument("out")
+    u.add_argument("--graphs", type=int, default=2000)
+    u.add_argument("--seed", type=int, default=2026)
+    u.add_argument("--min-scale", type=int, default=12)
+    u.add_argument("--max-scale", type=int, default=20)
+    u.add_argument("--start", type=int, default=0)
+    u.add_argument("--stop", type=int, default=2000)
     t = sub.add_parser("train")
     t.add_argument("data", nargs="+")
     t.add_argument("--out", default=MODEL_PATH)
     t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--holdout-by-graph", action="store_true",
+                   help="validate on whole held-out graphs instead of a random 15%% of the phases")
+    t.add_argument("--heldout-out", default=None, help="write the held-out phases here (npz)")
+    t.add_argument("--data-out", default=None, help="write all samples here (npz)")
     e = sub.add_parser("eval")
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--scale", type=int, default=20)
@@ -302,25 +392,45 @@ def main():
     args = ap.parse_args()
     K.init(0)
     if args.cmd == "collect":
-        f, c_, meta, specs = collect(args.graphs, args.seed, args.max_scale, args.min_scale)
+        f, c_, meta, specs = collect(args.graphs, args.seed, args.max_scale, args.min_scale, args.c5,
+                                     args.start, args.stop)
         np.savez(args.out, feats=f, costs=c_, meta=meta, specs=np.array(json.dumps(specs)))
         log(f"wrote {len(f)} samples to {args.out}")
+    elif args.cmd == "tune":
+        f, c_, meta = tuner_samples(args.graphs, args.seed, args.min_scale, args.max_scale, args.start, args.stop)
+        np.savez(args.out, feats=f, costs=c_, meta=meta)
+        log(f"wrote {len(f)} tuner samples to {args.out}")
     elif args.cmd == "train":
-        fs, cs = [], []
+        fs, cs, ms = [], [], []
         for path in args.data:
             with np.load(path, allow_pickle=False) as z:
                 fs.append(z["feats"])
                 cs.append(z["costs"])
-        f, c_ = np.concatenate(fs), np.concatenate(cs)
-        rng = np.random.default_rng(0)
-        idx = rng.permutation(len(f))
-        cut = int(0.85 * len(f))
-        tr, va = idx[:cut], idx[cut:]
+                ms.append(z["meta"] if "meta" in z.files else np.full((len(z["feats"]), 3), -2, np.float32))
+        f, c_, meta = np.concatenate(fs), np.concatenate(cs), np.concatenate(ms)
+        if args.holdout_by_graph:
+            # whole graphs held out (every HOLDOUT_MOD-th of the seeded list):
+            # the validation phases come from graphs the model never saw
+            held = (meta[:, 0].astype(np.int64) % HOLDOUT_MOD) == HOLDOUT_REM
+            tr = np.flatnonzero(~held)
+            va = np.flatnonzero(held & (meta[:, 1] >= 0))  # the sweeps' phases
+        else:
+            rng = np.random.default_rng(0)
+            idx = rng.permutation(len(f))
+            cut = int(0.85 * len(f))
+            tr, va = idx[:cut], idx[cut:]
         model, loss = train(f[tr], c_[tr], epochs=args.epochs)
-        info = {"train_samples": int(len(tr)), "loss": loss,
+        info = {"train_samples": int(len(tr)), "train_graphs": int(len(np.unique(meta[tr, 0]))),
+                "tuner_samples": int(np.sum(meta[tr, 1] == -1)), "loss": loss,
                 "train": picks_regret(model, f[tr], c_[tr]),
-                "validation": picks_regret(model, f[va], c_[va])}
+                "validation": picks_regret(model, f[va], c_[va]),
+                "validation_graphs": int(len(np.unique(meta[va, 0]))),
+                "candidates": [c.describe() for c in candidate_configs()]}
         save_model(model, args.out, info)
+        if args.heldout_out:
+            np.savez_compressed(args.heldout_out, feats=f[va], costs=c_[va], meta=meta[va])
+        if args.data_out:
+            np.savez_compressed(args.data_out, feats=f, costs=c_, meta=meta)
         print(json.dumps(info))
     else:
         model, configs = load_model(args.model)

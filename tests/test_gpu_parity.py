@@ -231,3 +231,26 @@ def test_tile_classes_match_oracle(small_graphs, v, tiles, y):
             q = K.Plan(K.Graph(n, s, d), gpu_variant(*v, e=1, tiles=tiles, y=y))
             gs, gd, _ = q.ktruss(k)
             assert edge_set(gs, gd) == edge_set(os_, od_), (name, k)
+
+
+@pytest.mark.parametrize("v", [("directed-by-degree", "coocsr", "all"), ("directed-by-index", "tiled-coocsr", "all"),
+                               ("undirected", "tiled-coocsr", "affected"), ("undirected", "coocsr", "all")], ids=vid)
+def test_autotune_keeps_results(small_graphs, v):
+    """kt_plan_autotune tries tile shapes (tiled: tile size, classes, workgroup
+    size, the layout rebuilt per shape) or count shapes (COO+CSR), applies the
+    fastest, and the decomposition still equals the oracle's."""
+    n, s, d = small_graphs["plc300"]
+    tau_o, st_o, _ = O.decompose(n, s, d, oracle_variant(*v), O.SCHED_JUMP | O.SCHED_REUSE)
+    p = K.Plan(K.Graph(n, s, d), gpu_variant(*v))
+    best = p.autotune(k=3, repeats=1)
+    log = p.tune_log()
+    assert len(log) >= 5 and all(t["ms"] > 0 for t in log)
+    if v[1] == "tiled-coocsr":
+        assert len({(t["tile_size"], t["num_parallel_tiles"]) for t in log}) >= 3
+        assert (best.tile_size, best.num_parallel_tiles) in {(t["tile_size"], t["num_parallel_tiles"]) for t in log}
+    assert min(t["ms"] for t in log) <= log[0]["ms"]
+    p.decompose()
+    assert np.array_equal(p.truss_numbers(), tau_o)
+    es, ed, _ = p.ktruss(4)
+    os_, od_, _, _ = O.ktruss(n, s, d, 4, oracle_variant(*v))
+    assert edge_set(es, ed) == edge_set(os_, od_)
