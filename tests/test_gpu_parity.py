@@ -140,7 +140,7 @@ def _golden(scale):
         return json.load(f), hashlib
 
 
-@pytest.mark.parametrize("scale", [14, 16, 18])
+@pytest.mark.parametrize("scale", [14, 16, 18, 20])
 def test_rmat_matches_golden_fixture(scale):
     gold, hashlib = _golden(scale)
     n, s, d = K.generate_rmat(scale, 16, seed=1)
@@ -151,6 +151,8 @@ def test_rmat_matches_golden_fixture(scale):
     variants = [K.default_variant(), K.make_variant("directed-by-degree", "coocsr", "affected"),
                 K.make_variant("directed-by-degree", "tiled-coocsr", "all"),
                 K.make_variant(count_kernel="delta")]
+    if scale >= 20:  # the default variant (11 s), DELTA and the learned pick below
+        variants = [variants[0], variants[3]]
     if scale <= 16:
         variants += [K.make_variant("undirected", "coocsr", "all"),
                      K.make_variant("directed-by-index", "coocsr", "affected", e_iters=4)]
@@ -161,6 +163,15 @@ def test_rmat_matches_golden_fixture(scale):
         assert hashlib.sha256(np.ascontiguousarray(tau, "<u4").tobytes()).hexdigest() == \
             gold["tau_sha256"], v.describe()
         assert st["k_max"] == gold["k_max"]
+    # the committed cost model's per-phase pick
+    import learn
+    model, configs = learn.load_model()
+    p = K.Plan(g, K.default_variant())
+    p.set_candidates(configs)
+    p.set_pick(K.PICK_MODEL, model)
+    p.decompose()
+    assert hashlib.sha256(np.ascontiguousarray(p.truss_numbers(), "<u4").tobytes()).hexdigest() == \
+        gold["tau_sha256"], "learned pick"
 
 
 TINY = {"edgeless": (6, []), "no_vertices": (0, []), "one_edge": (10, [(3, 7)]),
