@@ -391,10 +391,12 @@ int kt_plan_set_candidates(kt_plan* p, const kt_count_config* cands, uint32_t n)
 /* model is read when mode is KT_PICK_MODEL (its parameters are copied). */
 int kt_plan_set_pick(kt_plan* p, uint32_t mode, const kt_mlp* model);
 /* Sampled SWEEP (training data from graphs too large to sweep whole): in
- * KT_PICK_SWEEP mode every candidate is timed only in phases 0, every,
- * 2*every, ...; the other phases run candidate `fill` alone and log no
- * per-candidate times.  every = 1 (the default) sweeps every phase. */
-int kt_plan_set_sweep_sampling(kt_plan* p, uint32_t every, uint32_t fill);
+ * KT_PICK_SWEEP mode every candidate is timed only in the phases whose index
+ * is offset modulo every; the other phases run candidate `fill` alone and log
+ * no per-candidate times.  every = 1 (the default) sweeps every phase; runs
+ * with offsets 0 .. every-1 together sweep every phase once (the peel, and so
+ * the phases, are the same in every run). */
+int kt_plan_set_sweep_sampling(kt_plan* p, uint32_t every, uint32_t offset, uint32_t fill);
 /* Phase log of the last decomposition, KT_PHASE_REC floats per phase:
  * [0] k, [1] iterations, [2] chosen candidate, [3] count ms of the chosen,
  * [4..19] features, [20..35] count ms per candidate (SWEEP; -1 otherwise). */

@@ -263,9 +263,10 @@ int kt_plan_tune_log(const kt_plan* p, kt_tune_trial* recs, uint64_t cap, uint64
     return KT_OK;
 }
 
-int kt_plan_set_sweep_sampling(kt_plan* p, uint32_t every, uint32_t fill) {
-    if (!p || every == 0) return kt::fail(KT_ERR_ARG, "plan is NULL or every is 0");
+int kt_plan_set_sweep_sampling(kt_plan* p, uint32_t every, uint32_t offset, uint32_t fill) {
+    if (!p || every == 0 || offset >= every) return kt::fail(KT_ERR_ARG, "plan is NULL, every is 0 or offset >= every");
     p->impl.sweep_every = every;
+    p->impl.sweep_offset = offset;
     p->impl.sweep_fill = fill;
     return KT_OK;
 }

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                 const uint32_t dv = dst[e], ei = eid[e], a = sa[e], c = sb[e], u = src[e];
                 const uint32_t vid = dv & KT_IDMASK;
                 if (dv & (KT_DEAD | KT_DYING)) atomicAdd(bad, 1u);
-                sup[e] = KT_SUP_GONE;
+                sup[e] = sp[j] ? KT_SUP_GONE : KT_SUP_NONE;  // NONE: in no triangle, nothing to update
                 dst[e] = vid | KT_DYING;
                 tau[ei] = k - 1;
                 snbr[a] = vid | KT_DYING;
@@ -219,7 +219,7 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCoun
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) {
             const uint32_t e = e0 + j * stride;
-            if (e < n && x[j] < m) m = x[j];
+            if (e < n && x[j] < m && x[j] < KT_SUP_NONE) m = x[j];
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -261,7 +261,8 @@ __global__ void k_phase_start(unsigned int* from, DevCounters* ctr, int zero_nde
 // Task list of the pending deletions, one kernel.  Per deletion i (d = dl[i]
 // = (u, v)): the shorter and the longer of the two adjacency lists and the
 // number of 64-candidate tasks over the shorter one (at least 1: the list
-// holds the edge itself).  Deletions form tiles of kTileD; a workgroup scans
+// holds the edge itself; none for a deletion that lies in no triangle, whose
+// support the mark left KT_SUP_NONE).  Deletions form tiles of kTileD; a workgroup scans
 // its tile (lofs[i] = first task of deletion i counted from the tile's
 // first) and stores the tile total; the last workgroup to finish (a ticket
 // taken after a device-scope fence) scans the tile totals in place, so the
@@ -285,7 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
                                                   const uint32_t* __restrict__ sptr,
                                                   uint4* __restrict__ meta, uint32_t* __restrict__ lofs,
                                                   uint32_t* tsum, DevCounters* ctr, unsigned int* zero_cnt,
-                                                  uint32_t* host_cnt, dev::ListMark lm) {
+                                                  uint32_t* host_cnt, dev::ListMark lm,
+                                                  const uint32_t* __restrict__ sup) {
     __shared__ uint32_t s_w[kBlock / kWave];
     __shared__ uint32_t s_c[kTileD];
     __shared__ uint32_t s_last;
@@ -315,12 +317,18 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
             const uint32_t i = base + r * kBlock;
             d[r] = dl[i < last ? i : last];
         }
+        bool none[kPerD];  // the deletion lies in no triangle
 #pragma unroll
         for (uint32_t r = 0; r < kPerD; ++r) {
             u[r] = src[d[r]];
             dv[r] = dst[d[r]];
+            none[r] = sup && !lm.k && sup[d[r]] == KT_SUP_NONE;
         }
         if (lm.k) {
+            // a list the update made: its entries fell to k - 3 or below, so
+            // at k = 3 they lie in no triangle
+#pragma unroll
+            for (uint32_t r = 0; r < kPerD; ++r) none[r] = sup && lm.k == 3;
             uint32_t e[kPerD], a[kPerD], b[kPerD];
 #pragma unroll
             for (uint32_t r = 0; r < kPerD; ++r) {
@@ -359,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
             uint32_t c = 0;
             if (i < nd) {
                 meta[i] = m;
-                c = (m.y - m.x + kWave - 1) / kWave;
+                c = none[r] ? 0u : (m.y - m.x + kWave - 1) / kWave;
             }
             s_c[r * kBlock + threadIdx.x] = c;
         }
@@ -631,8 +639,10 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
     uint32_t full = 0xFFFFFFFFu;     // deletion whose whole longer list is in buf
     uint32_t sampled = 0xFFFFFFFFu;  // deletion whose block samples are in smp
     unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
-    // deletion i: its descriptor, its first task and the next deletion's (every
-    // deletion has a task); the next task's candidates are loaded before this
+    // deletion i: its descriptor, its first task and the next deletion's (a
+    // deletion without tasks shares its first task with the next one, so the
+    // search above lands on the task's owner and the walk below steps over
+    // it); the next task's candidates are loaded before this
     // task's search when they belong to the same deletion, so their round trip
     // overlaps this task's slot->entry reads
     uint4 m = make_uint4(0, 0, 0, 0);
@@ -988,7 +998,7 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
-                         unsigned int* zero_cnt, uint32_t* host_cnt, const ListMark* lm) {
+                         unsigned int* zero_cnt, uint32_t* host_cnt, const ListMark* lm, const uint32_t* sup) {
     if (!nd_max && !zero_cnt && !host_cnt && !lm) return hipSuccess;
     ListMark m{};
     if (lm) {
@@ -1000,7 +1010,7 @@ hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd
     const uint32_t tiles = (nd_max + kTileD - 1) / kTileD;
     const uint32_t tb = tiles < 256 ? (tiles ? tiles : 1u) : 256u;
     hipLaunchKernelGGL(k_dprep, dim3(tb), dim3(kBlock), 0, s, dl, nd_dev, src, dst, sptr, ws.meta,
-                       ws.ntask, ws.tsum, ctr, zero_cnt, host_cnt, m);
+                       ws.ntask, ws.tsum, ctr, zero_cnt, host_cnt, m, sup);
     return hipGetLastError();
 }
 

@@ -55,6 +55,10 @@ int fail(int status, const std::string& msg);
 // Support of an entry a DELTA mark deleted: above every threshold, so later
 // marks find it without reading its flags.
 constexpr uint32_t KT_SUP_GONE = 0xFFFFFFFFu;
+// Support of an entry a DELTA mark deleted while it lay in no triangle (its
+// support was 0): above every threshold as KT_SUP_GONE, and its update has
+// nothing to find, so the task list gives it no tasks.
+constexpr uint32_t KT_SUP_NONE = 0xFFFFFFFEu;
 
 struct DevCounters {
     unsigned long long merge_elems;   // sum of L_u + L_v over recounted entries
@@ -206,11 +210,14 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
 // host_cnt (optional, device address of pinned host memory): receives *nd_dev;
 // lm (optional): list work fused into the same kernel, see ListMark.
 struct ListMark;
+// sup (optional): the supports, read for each listed deletion -- one whose
+// support is KT_SUP_NONE (or any of a list the same kernel flags at k = 3)
+// lies in no triangle and gets no tasks.
 hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                          const uint32_t* src, const uint32_t* dst, const uint32_t* sptr,
                          const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
                          unsigned int* zero_cnt = nullptr, uint32_t* host_cnt = nullptr,
-                         const ListMark* lm = nullptr);
+                         const ListMark* lm = nullptr, const uint32_t* sup = nullptr);
 // Work the task-list kernel can do besides the task list: flag the pending
 // list when an update made it (its entries' supports fell below k - 2: KT_DYING
 // on the entry and both adjacency slots, KT_SUP_GONE, truss number k - 1, live

@@ -478,12 +478,13 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bo
 // One phase at k: the peel loop of kernel_ddc.cu:116-214 (and its tiled /
 // undirected twins) on the current graph.
 // The phase's pick mode and fixed configuration: a sampled SWEEP times every
-// candidate only in every sweep_every-th phase and runs candidate sweep_fill
-// alone in the others.
+// candidate only in the phases of index sweep_offset modulo sweep_every and
+// runs candidate sweep_fill alone in the others.
 void Plan::begin_phase() {
     pick_now = pick;
     phase_fixed = fixed_cfg;
-    if (pick == KT_PICK_SWEEP && sweep_every > 1 && phase_log.size() % sweep_every != 0 && !cands.empty()) {
+    if (pick == KT_PICK_SWEEP && sweep_every > 1 && phase_log.size() % sweep_every != sweep_offset &&
+        !cands.empty()) {
         pick_now = KT_PICK_FIXED;
         phase_fixed = cands[std::min<size_t>(sweep_fill, cands.size() - 1)];
     }
@@ -890,7 +891,8 @@ int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
     if (pending && cfg.kernel == KT_COUNT_DELTA) {
         KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
                                       dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws,
-                                      ctr.as<DevCounters>(), stream));
+                                      ctr.as<DevCounters>(), stream, nullptr, nullptr, nullptr,
+                                      sup[cur].as<uint32_t>()));
         KT_HIP_TRY(hipEventRecord(ev_c0, stream));
         KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
@@ -951,7 +953,8 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c], stream));
             KT_HIP_TRY(dev::delta_prepare(dl_p, nd_dev, pending, src[cur].as<uint32_t>(),
                                           dst[cur].as<uint32_t>(), sptr_b[acur].as<uint32_t>(), dws,
-                                          ctr.as<DevCounters>(), stream));
+                                          ctr.as<DevCounters>(), stream, nullptr, nullptr, nullptr,
+                                          sup[cur].as<uint32_t>()));
             KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, snbr_b[acur].as<uint32_t>(),
                                         seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
                                         ctr.as<DevCounters>(), stream));
@@ -1098,7 +1101,8 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         m.fin_nd = counts[prev];
         KT_HIP_TRY(dev::delta_prepare(lists[in], counts[in], std::max<uint32_t>(hint, 1u),
                                       src[cur].as<uint32_t>(), dst[cur].as<uint32_t>(),
-                                      sptr_b[acur].as<uint32_t>(), dws, c, stream, zero_cnt, host_cnt, &m));
+                                      sptr_b[acur].as<uint32_t>(), dws, c, stream, zero_cnt, host_cnt, &m,
+                                      sup[cur].as<uint32_t>()));
         return KT_OK;
     };
     // iteration r; the task list's hint is the last count read back (the size

@@ -84,7 +84,7 @@ struct Plan {
     std::vector<CountCfg> cands;
     uint32_t pick = KT_PICK_FIXED;
     // sampled SWEEP (kt_plan_set_sweep_sampling) and the current phase's mode
-    uint32_t sweep_every = 1, sweep_fill = 0;
+    uint32_t sweep_every = 1, sweep_offset = 0, sweep_fill = 0;
     uint32_t pick_now = KT_PICK_FIXED;
     CountCfg phase_fixed;
     void begin_phase();

@@ -235,7 +235,7 @@ def lib():
         L.kt_batch_graph_info.argtypes = [_vp, ctypes.c_uint32, _u32p]
         L.kt_plan_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Variant)]
         L.kt_plan_tune_log.argtypes = [ctypes.c_void_p, ctypes.POINTER(TuneTrial), ctypes.c_uint64, _u64p]
-        L.kt_plan_set_sweep_sampling.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+        L.kt_plan_set_sweep_sampling.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.kt_ktruss_noalloc.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_uint32,
                                         _vp, _u64p]
         _lib = L
@@ -440,10 +440,10 @@ class Plan:
         _check(lib().kt_plan_tune_log(self._h, recs, n.value, ctypes.byref(n)), "kt_plan_tune_log")
         return [recs[i].as_dict() for i in range(n.value)]
 
-    def set_sweep_sampling(self, every, fill=0):
-        """SWEEP times every candidate only in every `every`-th phase; the
-        others run candidate `fill` alone."""
-        _check(lib().kt_plan_set_sweep_sampling(self._h, every, fill), "kt_plan_set_sweep_sampling")
+    def set_sweep_sampling(self, every, fill=0, offset=0):
+        """SWEEP times every candidate only in the phases of index `offset`
+        modulo `every`; the others run candidate `fill` alone."""
+        _check(lib().kt_plan_set_sweep_sampling(self._h, every, offset, fill), "kt_plan_set_sweep_sampling")
 
     def phase_log(self):
         """(phases, 36) float32: k, iterations, chosen, phase count ms, 16 features,

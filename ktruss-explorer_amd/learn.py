@@ -137,7 +137,7 @@ def sweep(graph, variant=None, every=1):
     p.set_candidates(candidate_configs())
     p.set_pick(K.PICK_SWEEP)
     if every > 1:
-        p.set_sweep_sampling(every, len(CANDIDATES) - 1)
+        p.set_sweep_sampling(every, len(CANDIDATES) - 1, 0)
     p.decompose()
     return p.truss_numbers(), p.phase_log()
 
@@ -299,6 +299,7 @@ def evaluate(graph, model, configs, sweep=False, singles=(), default_ms=None):
         return 1e3 * (time.perf_counter() - t0), st
 
     out = {"m": graph.m}
+    log(f"evaluating on m={graph.m}")
     if default_ms is None:
         p = K.Plan(graph, K.default_variant())
         out["default_ms"], st = timed(p)
@@ -319,10 +320,12 @@ def evaluate(graph, model, configs, sweep=False, singles=(), default_ms=None):
     p.set_pick(K.PICK_MODEL, model)
     timed(p)  # warm
     out["model_ms"], st = timed(p)
+    log(f"model: {out['model_ms']:.1f} ms")
     out["model_support_ms"] = st["ms_count"] + st["ms_delta"]
     out["model_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
     ph = p.phase_log()
-    out["model_picks"] = np.bincount(ph[:, 2].astype(int), minlength=len(configs)).tolist()
+    model_choice = ph[:, 2].astype(int)
+    out["model_picks"] = np.bincount(model_choice, minlength=len(configs)).tolist()
     singles_ms = {}
     for c in singles:
         cfg = configs[c]
@@ -332,6 +335,7 @@ def evaluate(graph, model, configs, sweep=False, singles=(), default_ms=None):
         timed(q)
         ms, _ = timed(q)
         singles_ms[cfg.describe()] = ms
+        log(f"single {cfg.describe()}: {ms:.1f} ms")
         assert np.array_equal(q.truss_numbers(), tau_default), cfg.describe()
         del q
     out["singles_ms"] = singles_ms
@@ -341,15 +345,32 @@ def evaluate(graph, model, configs, sweep=False, singles=(), default_ms=None):
         out["best_single_ms"] = best
         out["regret_vs_best_single"] = out["model_ms"] / best - 1
     if sweep:
+        # the exhaustive per-phase sweep, as `sweep` sampled runs (phases of
+        # index r modulo sweep in run r) merged; every run peels the same phases
         p.set_pick(K.PICK_SWEEP)
-        p.decompose()
-        out["sweep_parity"] = bool(np.array_equal(p.truss_numbers(), tau_default))
-        ph = p.phase_log()
-        ms = ph[:, 20:20 + len(configs)]
+        ms = None
+        parity = True
+        for r in range(sweep):
+            p.set_sweep_sampling(sweep, len(configs) - 1, r)
+            p.decompose()
+            parity = parity and bool(np.array_equal(p.truss_numbers(), tau_default))
+            ph = p.phase_log()
+            if ms is None:
+                ms = np.full((len(ph), len(configs)), -1.0)
+            rows = np.arange(len(ph)) % sweep == r
+            ms[rows] = ph[rows, 20:20 + len(configs)]
+            log(f"sweep run {r + 1}/{sweep}: {int(rows.sum())} phases timed")
+        out["sweep_parity"] = parity
+        ms = np.maximum(ms, 0.0)  # a phase without a support pass costs nothing
+        out["phases"] = int(len(ms))
+        out["phases_best_per_candidate"] = np.bincount(ms.argmin(1), minlength=len(configs)).tolist()
+        if len(model_choice) == len(ms):  # the same peel: the model's picks priced by the sweep
+            out["model_on_sweep_ms"] = float(ms[np.arange(len(ms)), model_choice].sum())
         out["exhaustive_support_ms"] = float(ms.min(1).sum())
         out["best_single_support_ms"] = float(ms.sum(0).min())
         out["best_single_candidate"] = configs[int(ms.sum(0).argmin())].describe()
-        out["regret_vs_exhaustive"] = out["model_support_ms"] / out["exhaustive_support_ms"] - 1
+        if "model_on_sweep_ms" in out:
+            out["regret_vs_exhaustive"] = out["model_on_sweep_ms"] / out["exhaustive_support_ms"] - 1
     return out
 
 
@@ -385,7 +406,8 @@ def main():
     e = sub.add_parser("eval")
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--scale", type=int, default=20)
-    e.add_argument("--sweep", action="store_true")
+    e.add_argument("--sweep", type=int, default=0,
+                   help="exhaustive per-phase sweep in this many sampled runs (0: none)")
     e.add_argument("--singles", type=int, nargs="*", default=[])
     e.add_argument("--default-ms", type=float, default=None,
                    help="use this recorded default-variant time instead of running it")

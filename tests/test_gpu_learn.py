@@ -92,11 +92,11 @@ def test_sweep_and_model_runs_are_bit_exact():
     assert np.array_equal(ph[:, 2].astype(int), host)
 
 
-@pytest.mark.parametrize("every", [2, 3])
-def test_sampled_sweep_is_bit_exact(every):
-    """Sampled SWEEP: every candidate is timed only in phases 0, every,
-    2 * every, ...; the others run the fill candidate (DELTA) alone.  Truss
-    numbers equal the default variant's either way."""
+@pytest.mark.parametrize("every,offset", [(2, 0), (3, 0), (3, 2)])
+def test_sampled_sweep_is_bit_exact(every, offset):
+    """Sampled SWEEP: every candidate is timed only in the phases of index
+    offset modulo every; the others run the fill candidate (DELTA) alone.
+    Truss numbers equal the default variant's either way."""
     g = _graph()
     p = K.Plan(g, K.default_variant())
     p.decompose()
@@ -104,12 +104,12 @@ def test_sampled_sweep_is_bit_exact(every):
     cands = [K.count_config("row", 512), K.count_config("search", 256, 1024), K.count_config("delta", 512)]
     p.set_candidates(cands)
     p.set_pick(K.PICK_SWEEP)
-    p.set_sweep_sampling(every, 2)
+    p.set_sweep_sampling(every, 2, offset)
     p.decompose()
     assert np.array_equal(p.truss_numbers(), ref)
     ph = p.phase_log()
     assert len(ph) > 2 * every
-    swept = np.arange(len(ph)) % every == 0
+    swept = np.arange(len(ph)) % every == offset
     assert np.all(ph[swept][:, 20:23] >= 0) and np.any(np.all(ph[swept][:, 20:23] > 0, axis=1))
     assert np.all(ph[~swept][:, 20:23] == -1) and np.all(ph[~swept][:, 2] == -1)
 
