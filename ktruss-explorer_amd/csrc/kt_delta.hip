@@ -584,7 +584,8 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                                                   const uint32_t* __restrict__ seid,
                                                   uint32_t* sup, DevCounters* ctr,
                                                   unsigned long long* __restrict__ bstat, uint32_t thr,
-                                                  uint32_t* __restrict__ next, unsigned int* next_cnt) {
+                                                  uint32_t* __restrict__ next, unsigned int* next_cnt,
+                                                  const uint32_t* __restrict__ gsmp) {
     __shared__ uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample)];
     __shared__ uint32_t s_cross[kCross ? (kDBlock / kWave) * kCrossCap : 1];
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -672,12 +673,26 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
         // load in flight
         // a longer list that fits the slice is staged, or the first ids of its
         // 64-entry blocks are, with the candidates' load in flight
-        const uint32_t nb = (m.w + kWave - 1) / kWave;
+        // samples of the longer list: with gsmp, its first id and the first
+        // ids of the 64-slot blocks that start inside it (gsmp[b] = id of
+        // slot 64 b, contiguous); else the first id of every 64-entry block
+        // of the list (one line per sample)
+        const uint32_t b0 = m.z >> 6;
+        const uint32_t nb = gsmp ? 1u + ((m.z + m.w - 1) >> 6) - b0 : (m.w + kWave - 1) / kWave;
         const bool fresh = m.w <= kStage && full != i;
         const bool fresh_smp = m.w > kStage && nb <= kSample && sampled != i;
         Staged<kStage> st;
-        if (fresh) st.load(snbr + m.z, m.w, 1u, lane);
-        else if (fresh_smp) st.load(snbr + m.z, nb, kWave, lane);
+        if (fresh) {
+            st.load(snbr + m.z, m.w, 1u, lane);
+        } else if (fresh_smp) {
+            if (gsmp) {
+                st.load(gsmp + b0, nb, 1u, lane);
+                const uint32_t f0 = snbr[m.z];  // block b0 may start before the list
+                if (lane == 0) st.r[0] = f0;
+            } else {
+                st.load(snbr + m.z, nb, kWave, lane);
+            }
+        }
         const uint32_t x = pre ? xpre : (j < m.y ? snbr[j] : KT_DEAD);
         pre = t < t1 && t < fnext;
         if (pre) xpre = j + kWave < m.y ? snbr[j + kWave] : KT_DEAD;
@@ -707,9 +722,16 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
             if (nb <= kSample) {
                 // blocks whose first id is below xmin / xmax + 1: the bounds lie
                 // in the last such block (or at the list start)
+                // sample q lies at slot m.z (q = 0) or at the start of block
+                // b0 + q (gsmp), else at slot m.z + 64 q
                 const uint32_t ca = count_less_lds(smp, nb, xmin), cb = count_less_lds(smp, nb, xmax + 1);
-                lo = m.z + (ca ? (ca - 1) * kWave : 0u);
-                hi = m.z + min(m.w, cb * kWave);
+                if (gsmp) {
+                    lo = ca > 1 ? (b0 + ca - 1) << 6 : m.z;
+                    hi = cb == 0 ? m.z : (cb < nb ? (b0 + cb) << 6 : m.z + m.w);
+                } else {
+                    lo = m.z + (ca ? (ca - 1) * kWave : 0u);
+                    hi = m.z + min(m.w, cb * kWave);
+                }
             } else {
                 lower_bounds_2(snbr, m.z, m.w, xmin, xmax + 1, lo, hi);
             }
@@ -937,6 +959,13 @@ __global__ void k_cmp_slots(const uint32_t* __restrict__ snbr2, const uint32_t* 
     }
 }
 
+// gsmp[b] = id of slot 64 b (flags masked) for the S slots of the lists.
+__global__ void k_block_samples(const uint32_t* __restrict__ snbr, uint32_t S, uint32_t* __restrict__ gsmp) {
+    const uint32_t nb = (S + kWave - 1) / kWave;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x)
+        gsmp[b] = snbr[(size_t)b * kWave] & KT_IDMASK;
+}
+
 __global__ void k_row_lengths(const uint32_t* __restrict__ ptr, uint32_t nv, uint32_t* __restrict__ out) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
         out[v] = ptr[v + 1] - ptr[v];
@@ -1017,7 +1046,7 @@ hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s, uint32_t thr,
-                       uint32_t* next, unsigned int* next_cnt) {
+                       uint32_t* next, unsigned int* next_cnt, const uint32_t* gsmp) {
     if (!nd_max) return hipSuccess;
     if (next && (!next_cnt || thr < 2)) return hipErrorInvalidValue;
     // four waves per deletion, 256 to 8192 waves
@@ -1027,10 +1056,17 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
     const uint32_t db = (w + wpb - 1) / wpb;
     if (next)
         hipLaunchKernelGGL(k_delta<true>, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask,
-                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, thr, next, next_cnt);
+                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, thr, next, next_cnt, gsmp);
     else
         hipLaunchKernelGGL(k_delta<false>, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask,
-                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, 0u, nullptr, nullptr);
+                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, 0u, nullptr, nullptr, gsmp);
+    return hipGetLastError();
+}
+
+hipError_t block_samples(const uint32_t* snbr, uint32_t S, uint32_t* gsmp, hipStream_t s) {
+    if (!S) return hipSuccess;
+    const uint32_t b = blocks_for((S + kWave - 1) / kWave, kBlock);
+    hipLaunchKernelGGL(k_block_samples, dim3(b < 2048 ? b : 2048), dim3(kBlock), 0, s, snbr, S, gsmp);
     return hipGetLastError();
 }
 

@@ -252,7 +252,12 @@ hipError_t delta_stats_fold(const DeltaScratch& ws, DevCounters* ctr, hipStream_
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
-                       uint32_t thr = 0, uint32_t* next = nullptr, unsigned int* next_cnt = nullptr);
+                       uint32_t thr = 0, uint32_t* next = nullptr, unsigned int* next_cnt = nullptr,
+                       const uint32_t* gsmp = nullptr);
+// gsmp[b] = id of adjacency slot 64 b (flags masked), b < ceil(S / 64): the
+// block samples delta_apply (gsmp) stages contiguously for long lists; to be
+// rebuilt whenever the lists are rebuilt.
+hipError_t block_samples(const uint32_t* snbr, uint32_t S, uint32_t* gsmp, hipStream_t s);
 hipError_t finalize_deleted(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                             uint32_t* dst, uint32_t* snbr, const uint32_t* sa, const uint32_t* sb,
                             hipStream_t s);

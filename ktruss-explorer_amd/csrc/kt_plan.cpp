@@ -822,6 +822,7 @@ int Plan::ensure_delta() {
     if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
         (rc = sa0.alloc(Eb)) || (rc = sb0.alloc(Eb)) || (rc = sa.alloc(Eb)) || (rc = sb.alloc(Eb)) ||
         (rc = dl.alloc(Eb)) || (rc = dl_alt.alloc(Eb)) || (rc = dl3.alloc(Eb)) || (rc = ldeg.alloc(std::max<size_t>(Pb - 4, 4))) ||
+        (rc = gsmp.alloc(((size_t)2 * E0 + 63) / 64 * 4 + 4)) ||
         (rc = dscratch.alloc(dev::delta_scratch_bytes(E0))) ||
         (rc = ctsum.alloc((size_t)ct * 4)) || (rc = cxs.alloc(2 * Eb + 4)))
         return rc;
@@ -851,6 +852,7 @@ int Plan::restore_delta() {
     }
     KT_HIP_TRY(dev::delta_stats_reset(dws, stream));
     KT_HIP_TRY(dev::row_lengths(ptr0.as<uint32_t>(), n, ldeg.as<uint32_t>(), stream));
+    KT_HIP_TRY(dev::block_samples(snbr_b[0].as<uint32_t>(), S_cur, gsmp.as<uint32_t>(), stream));
     return KT_OK;
 }
 
@@ -877,6 +879,7 @@ int Plan::compact_delta_state() {
                                       cxs.as<uint32_t>(), stream));
     acur = o;
     S_cur = keep;
+    KT_HIP_TRY(dev::block_samples(snbr_b[acur].as<uint32_t>(), S_cur, gsmp.as<uint32_t>(), stream));
     return KT_OK;
 }
 
@@ -896,7 +899,8 @@ int Plan::support_pass(const CountCfg& cfg, uint32_t k, float& ms) {
         KT_HIP_TRY(hipEventRecord(ev_c0, stream));
         KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
-                                    ctr.as<DevCounters>(), stream));
+                                    ctr.as<DevCounters>(), stream, 0u, nullptr, nullptr,
+                                    gsmp.as<uint32_t>()));
         KT_HIP_TRY(hipEventRecord(ev_c1, stream));
         last_pass_delta = true;
         KT_HIP_TRY(dev::finalize_deleted(dl_p, nd_dev, pending, dst[cur].as<uint32_t>(),
@@ -957,7 +961,8 @@ int Plan::sweep_pass(uint32_t k, std::vector<double>& cand_ms, float& best_ms) {
                                           sup[cur].as<uint32_t>()));
             KT_HIP_TRY(dev::delta_apply(dl_p, nd_dev, pending, snbr_b[acur].as<uint32_t>(),
                                         seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws,
-                                        ctr.as<DevCounters>(), stream));
+                                        ctr.as<DevCounters>(), stream, 0u, nullptr, nullptr,
+                                        gsmp.as<uint32_t>()));
             KT_HIP_TRY(hipEventRecord(ev_sw[2 * c + 1], stream));
         }
     }
@@ -1117,7 +1122,7 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
         KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream,
                                     light ? k - 2 : 0u, light ? lists[out] : nullptr,
-                                    light ? counts[out] : nullptr));
+                                    light ? counts[out] : nullptr, gsmp.as<uint32_t>()));
         if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
         if (!light)
             KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),

@@ -113,6 +113,7 @@ struct Plan {
     uint32_t S_cur = 0;
     DBuf dl, dl_alt, dl3, sup_snap, dscratch, ctsum, cxs;  // dl_alt, dl3: run-ahead deletion lists
     DBuf ldeg;  // live length of every oriented row (phase features of DELTA plans)
+    DBuf gsmp;  // block samples of the working adjacency lists (dev::block_samples)
     // the rows' live lengths are kept only when phases are picked (features)
     uint32_t* live_rows_ptr() const { return pick == KT_PICK_FIXED ? nullptr : ldeg.as<uint32_t>(); }
     dev::DeltaScratch dws{};
