@@ -559,6 +559,25 @@ __device__ __forceinline__ uint32_t count_less_lds(const uint32_t* s, uint32_t n
     return b + (s[b] < x ? 1u : 0u);
 }
 
+// Numbers of samples below x0 and below x1 in s[0, n) (n <= kSample, x0 and
+// x1 the same in every lane), counted lane-parallel: lane l compares
+// s[8l, 8l + 8) (two 16-byte reads) and the ballots are summed -- one LDS
+// round trip instead of two dependent binary searches.
+__device__ __forceinline__ void count_less2_lds(const uint32_t* s, uint32_t n, uint32_t x0, uint32_t x1,
+                                                uint32_t lane, uint32_t& c0, uint32_t& c1) {
+    const uint32_t b = 8u * lane;
+    const uint4 lo = *(const uint4*)(s + b);
+    const uint4 hi = *(const uint4*)(s + b + 4);
+    const uint32_t v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    c0 = c1 = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        const bool in = b + j < n;
+        c0 += (uint32_t)__popcll(ballot(in && v[j] < x0));
+        c1 += (uint32_t)__popcll(ballot(in && v[j] < x1));
+    }
+}
+
 // Searches xv in buf[0, n) (staged list entries) and returns the slot offset
 // and the entry (KT_DEAD when past the end).
 __device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t n, uint32_t xv,
@@ -586,7 +605,7 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                                                   unsigned long long* __restrict__ bstat, uint32_t thr,
                                                   uint32_t* __restrict__ next, unsigned int* next_cnt,
                                                   const uint32_t* __restrict__ gsmp) {
-    __shared__ uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample)];
+    __shared__ __attribute__((aligned(16))) uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample)];
     __shared__ uint32_t s_cross[kCross ? (kDBlock / kWave) * kCrossCap : 1];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample);
@@ -724,7 +743,8 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                 // in the last such block (or at the list start)
                 // sample q lies at slot m.z (q = 0) or at the start of block
                 // b0 + q (gsmp), else at slot m.z + 64 q
-                const uint32_t ca = count_less_lds(smp, nb, xmin), cb = count_less_lds(smp, nb, xmax + 1);
+                uint32_t ca, cb;
+                count_less2_lds(smp, nb, xmin, xmax + 1, lane, ca, cb);
                 if (gsmp) {
                     lo = ca > 1 ? (b0 + ca - 1) << 6 : m.z;
                     hi = cb == 0 ? m.z : (cb < nb ? (b0 + cb) << 6 : m.z + m.w);
