@@ -482,6 +482,8 @@ constexpr uint32_t kDeltaGrid = 2048;  // most k_delta workgroups per launch
 constexpr size_t kStatBytes = (size_t)kDeltaGrid * 4 * 8;
 static_assert(8192 / (kDBlock / kWave) <= kDeltaGrid, "k_delta grids of 8192 waves need a slot per workgroup");
 constexpr uint32_t kSample = 512;
+constexpr uint32_t kGather = kWave / 8;  // blocks gathered per round trip (8 lanes each)
+static_assert(kGather * kWave <= kStage, "gathered blocks fit the slice");
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -594,7 +596,7 @@ __device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t 
 }
 
 template <bool kCross>
-__global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ dl,
+__global__ __launch_bounds__(kDBlock, 8) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
                                                   const uint32_t* __restrict__ lofs,
@@ -605,11 +607,12 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                                                   unsigned long long* __restrict__ bstat, uint32_t thr,
                                                   uint32_t* __restrict__ next, unsigned int* next_cnt,
                                                   const uint32_t* __restrict__ gsmp) {
-    __shared__ __attribute__((aligned(16))) uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample)];
+    __shared__ __attribute__((aligned(16))) uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample + kWave)];
     __shared__ uint32_t s_cross[kCross ? (kDBlock / kWave) * kCrossCap : 1];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample);
+    uint32_t* buf = s_buf + (threadIdx.x / kWave) * (kStage + kSample + kWave);
     uint32_t* smp = buf + kStage;
+    uint32_t* gl = smp + kSample;  // distinct blocks of a gathered task
     uint32_t* cl = s_cross + (kCross ? (threadIdx.x / kWave) * kCrossCap : 0u);
     uint32_t nc = 0;  // entries in cl (wave-uniform)
     // the last task's decremented entries and their old supports (0: none;
@@ -658,7 +661,8 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
     }
     uint32_t full = 0xFFFFFFFFu;     // deletion whose whole longer list is in buf
     uint32_t sampled = 0xFFFFFFFFu;  // deletion whose block samples are in smp
-    unsigned long long acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
+    // a wave's statistics of one launch stay far below 2^32
+    uint32_t acc_elems = 0, acc_tri = 0, acc_win = 0, acc_live = 0;
     // deletion i: its descriptor, its first task and the next deletion's (a
     // deletion without tasks shares its first task with the next one, so the
     // search above lands on the task's owner and the walk below steps over
@@ -764,8 +768,58 @@ __global__ __launch_bounds__(kDBlock) void k_delta(const uint32_t* __restrict__ 
                 sw.load(snbr + lo, wl, 1u, lane);
                 sw.store(buf, wl, 0xFFFFFFFFu, lane);
                 p = lo + search_staged(buf, wl, xv, y);
+            } else if (gsmp && nb <= kSample) {
+                // a wide window: each candidate's 64-slot block (the last
+                // sample at or below it) is gathered instead, eight distinct
+                // blocks of the task per round trip, and searched in the slice
+                full = 0xFFFFFFFFu;  // buf now holds gathered blocks
+                // slot of sample q (q = nb: the list end)
+                auto spos = [&](uint32_t q) -> uint32_t {
+                    return q >= nb ? m.z + m.w : (q ? (b0 + q) << 6 : m.z);
+                };
+                const int32_t blk = cand ? (int32_t)count_less_lds(smp, nb, xv + 1) - 1 : -1;
+                // blocks ascend over the candidate lanes: a block's first lane
+                // differs from the running maximum of the lanes before it
+                int32_t run = blk;
+                for (uint32_t off = 1; off < kWave; off <<= 1) {
+                    const int32_t o = __shfl_up(run, off);
+                    if (lane >= off) run = o > run ? o : run;
+                }
+                int32_t before = __shfl_up(run, 1);
+                if (lane == 0) before = -1;
+                const bool first = blk >= 0 && blk != before;
+                const uint64_t firsts = ballot(first);
+                const uint32_t grp = (uint32_t)__popcll(firsts & ((2ull << lane) - 1ull)) - 1u;
+                const uint32_t D = (uint32_t)__popcll(firsts);
+                wave_lds_sync();
+                if (first) gl[grp] = (uint32_t)blk;  // the task's distinct blocks, ascending
+                wave_lds_sync();
+                p = 0;
+                y = KT_DEAD;
+                for (uint32_t g0 = 0; g0 < D; g0 += kGather) {
+                    // lanes 8q .. 8q + 7 load block g0 + q, eight slots each
+                    // from its 64-aligned start (the lists are padded to it)
+                    const uint32_t q = lane >> 3, bq = g0 + q < D ? gl[g0 + q] : gl[g0];
+                    const uint32_t a = ((b0 + bq) << 6) + 8u * (lane & 7u);
+                    const uint4 lo4 = *(const uint4*)(snbr + a);
+                    const uint4 hi4 = *(const uint4*)(snbr + a + 4);
+                    acc_win += min(D - g0, kGather) * kWave;
+                    wave_lds_sync();
+                    *(uint4*)(buf + q * kWave + 8u * (lane & 7u)) = lo4;
+                    *(uint4*)(buf + q * kWave + 8u * (lane & 7u) + 4) = hi4;
+                    wave_lds_sync();
+                    if (blk >= 0 && grp >= g0 && grp < g0 + kGather) {
+                        // the block's slots of the list: [s0, s1) of its 64
+                        const uint32_t base = ((b0 + (uint32_t)blk) << 6);
+                        const uint32_t s0 = spos((uint32_t)blk) - base, s1 = spos((uint32_t)blk + 1) - base;
+                        uint32_t yy;
+                        const uint32_t r = search_staged(buf + (grp - g0) * kWave + s0, s1 - s0, xv, yy);
+                        p = base + s0 + r;
+                        y = yy;
+                    }
+                }
             } else {
-                acc_win += (uint64_t)__popcll(cm) * (32u - __builtin_clz(wl));
+                acc_win += (uint32_t)__popcll(cm) * (32u - __builtin_clz(wl));
                 p = lower_bound_u(snbr, lo, wl, xv);
                 y = p < hi ? snbr[p] : KT_DEAD;
             }

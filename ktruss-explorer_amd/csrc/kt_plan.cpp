@@ -816,7 +816,8 @@ int Plan::ensure_delta() {
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4, Pb = ((size_t)n + 1) * 4;
     const uint32_t ct = dev::compact_tiles_for(2 * E0) + 2;
     for (int b = 0; b < 2; ++b)
-        if ((rc = sptr_b[b].alloc(Pb)) || (rc = snbr_b[b].alloc(2 * Eb)) ||
+        // the update reads whole 64-slot blocks: the lists are padded to the last one
+        if ((rc = sptr_b[b].alloc(Pb)) || (rc = snbr_b[b].alloc(2 * Eb + 256)) ||
             (rc = seid_b[b].alloc(2 * Eb)))
             return rc;
     if ((rc = sptr0.alloc(Pb)) || (rc = snbr0.alloc(2 * Eb)) || (rc = seid0.alloc(2 * Eb)) ||
