@@ -131,7 +131,53 @@ __device__ __forceinline__ void mark_flush(const uint32_t* s_out, uint32_t* s_cn
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ dst,
+// The flush of a listing mark: the staged task counts of the c buffered
+// deletions are scanned (thread t owns slots [16t, 16t + 16)), one 64-bit
+// atomic reserves the list slots and the task range, then the deletions,
+// their first tasks and their descriptors are written out.
+constexpr uint32_t kFlushPer = kMarkBuf / kBlock;
+__device__ __forceinline__ void mark_flush_tasks(const uint32_t* s_out, uint32_t* s_cnt, uint32_t* s_res,
+                                                 uint32_t* s_w, uint32_t* dl, const dev::MarkTasks& mt) {
+    const uint32_t c = *s_cnt;  // every thread reads it after a barrier and before anyone changes it
+    if (!c) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const uint32_t stage = blockIdx.x * kMarkBuf;
+    const uint32_t i0 = threadIdx.x * kFlushPer;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kFlushPer; ++r) sum += i0 + r < c ? mt.ws.ttmp[stage + i0 + r] : 0u;
+    uint32_t inc = sum;  // inclusive scan over the workgroup
+    for (int off = 1; off < (int)kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += y;
+    }
+    if (lane == kWave - 1) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum, total = 0;
+    for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+        if (w < wid) run += s_w[w];
+        total += s_w[w];
+    }
+    if (threadIdx.x == 0) {
+        const unsigned long long old = atomicAdd(mt.pair, ((unsigned long long)total << 32) | c);
+        s_res[0] = (uint32_t)old;
+        s_res[1] = (uint32_t)(old >> 32);
+    }
+    __syncthreads();
+    const uint32_t base = s_res[0];
+    uint32_t ft = s_res[1] + run;
+    for (uint32_t r = 0; r < kFlushPer && i0 + r < c; ++r) {  // the counts again (cached)
+        dl[base + i0 + r] = s_out[i0 + r];
+        mt.ws.ntask[base + i0 + r] = ft;
+        ft += mt.ws.ttmp[stage + i0 + r];
+    }
+    for (uint32_t i = threadIdx.x; i < c; i += kBlock) mt.ws.meta[base + i] = mt.ws.mtmp[stage + i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s_cnt = 0;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock, 8) void k_mark_dying(uint32_t* __restrict__ dst,
                                                        const uint32_t* __restrict__ eid,
                                                        uint32_t* __restrict__ sup,
                                                        uint32_t* __restrict__ tau,
@@ -143,9 +189,11 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                                                        const uint32_t* __restrict__ fin_nd,
                                                        const uint32_t* __restrict__ src,
                                                        uint32_t* __restrict__ ldeg, unsigned int* bad,
-                                                       uint32_t mark_blocks) {
+                                                       uint32_t mark_blocks, dev::MarkTasks mt) {
     __shared__ uint32_t s_out[kMarkBuf];
     __shared__ uint32_t s_cnt, s_base;
+    __shared__ uint32_t s_res[2], s_w[kBlock / kWave];
+    const bool listing = mt.pair != nullptr;
     if (blockIdx.x >= mark_blocks) {  // the previous deletions, whose update is done, become KT_DEAD
         finalize_list(fin_dl, fin_nd, blockIdx.x - mark_blocks, gridDim.x - mark_blocks, src, dst, snbr, sa,
                       sb);
@@ -175,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
             if (lane == 0) b = atomicAdd(&s_cnt, (uint32_t)__popcll(m));  // LDS
             b = __shfl(b, 0);
             if (del) {
-                s_out[b + (uint32_t)__popcll(m & below)] = e;
+                const uint32_t slot = b + (uint32_t)__popcll(m & below);
+                s_out[slot] = e;
                 const uint32_t dv = dst[e], ei = eid[e], a = sa[e], c = sb[e], u = src[e];
                 const uint32_t vid = dv & KT_IDMASK;
                 if (dv & (KT_DEAD | KT_DYING)) atomicAdd(bad, 1u);
@@ -185,6 +234,15 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
                 snbr[a] = vid | KT_DYING;
                 snbr[c] = u | KT_DYING;
                 if (ldeg) atomicSub(ldeg + u, 1u);
+                if (listing) {
+                    // the deletion's shorter and longer list and its tasks of
+                    // 64 candidates (none for a deletion in no triangle)
+                    const uint32_t ub = mt.sptr[u], ue = mt.sptr[u + 1], vb = mt.sptr[vid], ve = mt.sptr[vid + 1];
+                    const bool us = ue - ub <= ve - vb;
+                    const uint4 md = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
+                    mt.ws.mtmp[blockIdx.x * kMarkBuf + slot] = md;
+                    mt.ws.ttmp[blockIdx.x * kMarkBuf + slot] = sp[j] ? (md.y - md.x + kWave - 1) / kWave : 0u;
+                }
             }
         }
         // every thread reads the count between two barriers: a wave that
@@ -193,10 +251,14 @@ __global__ __launch_bounds__(kBlock) void k_mark_dying(uint32_t* __restrict__ ds
         __syncthreads();
         const uint32_t c = s_cnt;
         __syncthreads();
-        if (c > kMarkBuf - kMarkTile) mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
+        if (c > kMarkBuf - kMarkTile) {
+            if (listing) mark_flush_tasks(s_out, &s_cnt, s_res, s_w, dl, mt);
+            else mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
+        }
     }
     __syncthreads();
-    mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
+    if (listing) mark_flush_tasks(s_out, &s_cnt, s_res, s_w, dl, mt);
+    else mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
 }
 
 // Minimum live support (the JUMP target); one atomic per workgroup.
@@ -250,11 +312,15 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCoun
 }
 
 __global__ void k_phase_start(unsigned int* from, DevCounters* ctr, int zero_ndel, int reset_min) {
-    if (from) {
-        ctr->void_del |= *from;
-        *from = 0;
+    if (from) {  // a list's (count, task total) pair
+        ctr->void_del |= from[0];
+        from[0] = 0;
+        from[1] = 0;
     }
-    if (zero_ndel) ctr->ndel = 0;
+    if (zero_ndel) {
+        ctr->ndel = 0;
+        ctr->ntask = 0;
+    }
     if (reset_min) ctr->min_sup = 0xFFFFFFFFu;
 }
 
@@ -293,7 +359,10 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
     __shared__ uint32_t s_last;
     const uint32_t nd = *nd_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (zero_cnt) *zero_cnt = 0;
+        if (zero_cnt) {  // a list's (count, task total) pair
+            zero_cnt[0] = 0;
+            zero_cnt[1] = 0;
+        }
         if (host_cnt) __hip_atomic_store(host_cnt, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     if (lm.fin_dl) finalize_list(lm.fin_dl, lm.fin_nd, blockIdx.x, gridDim.x, src, lm.dst, lm.snbr, lm.sa, lm.sb);
@@ -445,9 +514,10 @@ __global__ __launch_bounds__(kBlock) void k_dprep(const uint32_t* __restrict__ d
     }
 }
 
-// First task of deletion i.
+// First task of deletion i: from the task-list kernel's per-tile offsets, or
+// (tsum null) as a listing mark stored it.
 __device__ __forceinline__ uint32_t first_task(const uint32_t* lofs, const uint32_t* tsum, uint32_t i) {
-    return tsum[i / kTileD] + lofs[i];
+    return tsum ? tsum[i / kTileD] + lofs[i] : lofs[i];
 }
 
 // The update proper.  Task t = (deletion i, chunk c) holds the candidates
@@ -596,17 +666,17 @@ __device__ __forceinline__ uint32_t search_staged(const uint32_t* buf, uint32_t 
 }
 
 template <bool kCross>
-__global__ __launch_bounds__(kDBlock, 8) void k_delta(const uint32_t* __restrict__ dl,
+__global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_t* __restrict__ dl,
                                                   const uint32_t* __restrict__ nd_dev,
                                                   const uint4* __restrict__ meta,
                                                   const uint32_t* __restrict__ lofs,
-                                                  const uint32_t* __restrict__ tsum,
+                                                  const uint32_t* tsum,
                                                   const uint32_t* __restrict__ snbr,
                                                   const uint32_t* __restrict__ seid,
                                                   uint32_t* sup, DevCounters* ctr,
                                                   unsigned long long* __restrict__ bstat, uint32_t thr,
                                                   uint32_t* __restrict__ next, unsigned int* next_cnt,
-                                                  const uint32_t* __restrict__ gsmp) {
+                                                  const uint32_t* __restrict__ gsmp, dev::DirectTasks dt) {
     __shared__ __attribute__((aligned(16))) uint32_t s_buf[(kDBlock / kWave) * (kStage + kSample + kWave)];
     __shared__ uint32_t s_cross[kCross ? (kDBlock / kWave) * kCrossCap : 1];
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -639,7 +709,12 @@ __global__ __launch_bounds__(kDBlock, 8) void k_delta(const uint32_t* __restrict
         }
     };
     const uint32_t nd = *nd_dev;
-    const uint32_t T = ctr->dtasks;
+    const uint32_t T = dt.ntask ? *dt.ntask : ctr->dtasks;
+    if (dt.ntask) tsum = nullptr;  // the list's first tasks are absolute
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the task-list kernel's other duties
+        if (dt.zero_pair) *dt.zero_pair = 0;
+        if (dt.host_cnt) __hip_atomic_store(dt.host_cnt, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     const uint32_t W = nwaves(), w = gwave();
     const uint32_t t0 = (uint32_t)((uint64_t)T * w / W);
     const uint32_t t1 = nd ? (uint32_t)((uint64_t)T * (w + 1) / W) : t0;  // no early return:
@@ -1053,8 +1128,11 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
                       const uint32_t* fin_nd, uint32_t fin_max, const uint32_t* src, uint32_t* ldeg,
-                      unsigned int* bad, hipStream_t s) {
+                      unsigned int* bad, hipStream_t s, const MarkTasks* mt) {
     if (!fin_max) fin_dl = nullptr;
+    if (mt && (!mt->pair || !mt->sptr || !mt->ws.meta || !mt->ws.ntask || !mt->ws.mtmp || !mt->ws.ttmp ||
+               (uint64_t)n > mt->ws.cap))
+        return hipErrorInvalidValue;
     if (!n) {
         if (fin_dl) return finalize_deleted(fin_dl, fin_nd, fin_max, dst, snbr, sa, sb, s);
         return hipSuccess;
@@ -1064,7 +1142,7 @@ hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_
     // finalizing workgroups: fin_max is a bound (the grid strides over the true count)
     const uint32_t fw = (uint32_t)std::min<uint64_t>(((uint64_t)fin_max + kBlock - 1) / kBlock, kFinGrid);
     hipLaunchKernelGGL(k_mark_dying, dim3(mb + (fin_dl ? fw : 0u)), dim3(kBlock), 0, s, dst, eid, sup, tau, dl,
-                       n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg, bad, mb);
+                       n, k, snbr, sa, sb, ndel, fin_dl, fin_nd, src, ldeg, bad, mb, mt ? *mt : MarkTasks{});
     return hipGetLastError();
 }
 
@@ -1083,9 +1161,17 @@ hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hip
     return hipGetLastError();
 }
 
+// a listing mark's staging: one buffer of kMarkBuf slots per mark workgroup
+// (a mark of at most cap entries launches at most this many)
+static size_t mark_stage_slots(uint32_t cap) {
+    const size_t tiles = ((size_t)(cap ? cap : 1) + kMarkTile - 1) / kMarkTile;
+    return (tiles < kMarkGrid ? tiles : kMarkGrid) * kMarkBuf;
+}
+
 size_t delta_scratch_bytes(uint32_t cap) {
     const size_t c = cap ? cap : 1;
-    return kStatBytes + 16 * c + 4 * c + 4 * (c / kTileD + 2) + 256;
+    const size_t tail = (20 * c + 4 * (c / kTileD + 2) + 15) & ~(size_t)15;
+    return kStatBytes + tail + 20 * mark_stage_slots(cap) + 256;
 }
 
 void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
@@ -1095,6 +1181,9 @@ void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s) {
     s.meta = (uint4*)p;
     s.ntask = (uint32_t*)(p + 16 * c);
     s.tsum = (uint32_t*)(p + 20 * c);
+    char* q = p + ((20 * c + 4 * (c / kTileD + 2) + 15) & ~(size_t)15);
+    s.mtmp = (uint4*)q;
+    s.ttmp = (uint32_t*)(q + 16 * mark_stage_slots(cap));
     s.cap = cap;
 }
 
@@ -1120,8 +1209,12 @@ hipError_t delta_prepare(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd
 hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_max,
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s, uint32_t thr,
-                       uint32_t* next, unsigned int* next_cnt, const uint32_t* gsmp) {
-    if (!nd_max) return hipSuccess;
+                       uint32_t* next, unsigned int* next_cnt, const uint32_t* gsmp,
+                       const DirectTasks* dt) {
+    if (dt && !dt->ntask) return hipErrorInvalidValue;
+    // with no deletions the launch still clears and publishes the counts
+    if (!nd_max && !(dt && (dt->zero_pair || dt->host_cnt))) return hipSuccess;
+    const DirectTasks d = dt ? *dt : DirectTasks{};
     if (next && (!next_cnt || thr < 2)) return hipErrorInvalidValue;
     // four waves per deletion, 256 to 8192 waves
     const uint32_t wpb = kDBlock / kWave;
@@ -1130,10 +1223,10 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
     const uint32_t db = (w + wpb - 1) / wpb;
     if (next)
         hipLaunchKernelGGL(k_delta<true>, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask,
-                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, thr, next, next_cnt, gsmp);
+                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, thr, next, next_cnt, gsmp, d);
     else
         hipLaunchKernelGGL(k_delta<false>, dim3(db), dim3(kDBlock), 0, s, dl, nd_dev, ws.meta, ws.ntask,
-                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, 0u, nullptr, nullptr, gsmp);
+                           ws.tsum, snbr, seid, sup, ctr, ws.bstat, 0u, nullptr, nullptr, gsmp, d);
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 #pragma once
 
 #include <atomic>
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -63,23 +64,35 @@ constexpr uint32_t KT_SUP_NONE = 0xFFFFFFFEu;
 struct DevCounters {
     unsigned long long merge_elems;   // sum of L_u + L_v over recounted entries
     unsigned long long recount_edges; // recounted entries
+    // the three deletion lists' counts, each followed by the DELTA task total
+    // of its deletions: a mark that lists the tasks reserves (deletions,
+    // tasks) with one 64-bit atomic on the pair, so the first tasks ascend
+    // with the list positions
     unsigned int ndel;                // entries deleted by the last mark pass
+    unsigned int ntask;
+    unsigned int ndel2;               // second deletion count (DELTA iterations run ahead)
+    unsigned int ntask2;
+    unsigned int ndel3;               // third deletion count (DELTA iterations run ahead)
+    unsigned int ntask3;
     unsigned int wl_size;             // worklist length
     unsigned int min_sup;             // minimum live support (JUMP)
     unsigned int live;                // entries kept by the last compaction
     unsigned int choice;              // candidate picked by the cost model
-    unsigned int dtasks;              // DELTA tasks of the pending deletions
+    unsigned int dtasks;              // DELTA tasks of the pending deletions (task-list kernel)
     unsigned int dticket;             // DELTA task list: workgroups done (reset by the last)
-    unsigned int ndel2;               // second deletion count (DELTA iterations run ahead)
     unsigned int void_del;            // deletions of void DELTA iterations (stays 0)
     unsigned int mark_bad;            // entries the DELTA mark found deleted already (stays 0)
-    unsigned int ndel3;               // third deletion count (DELTA iterations run ahead)
     unsigned int mticket;             // JUMP minimum: workgroups done (reset by the last)
     unsigned long long delta_elems;   // adjacency entries read by DELTA updates
     unsigned long long delta_tris;    // destroyed triangles removed by DELTA updates
     unsigned long long delta_win;     // longer-list window entries read by DELTA updates
     unsigned long long delta_live;    // candidates of DELTA updates whose edge was alive
 };
+
+static_assert(offsetof(DevCounters, ndel) % 8 == 0 && offsetof(DevCounters, ntask) == offsetof(DevCounters, ndel) + 4 &&
+                  offsetof(DevCounters, ndel2) % 8 == 0 && offsetof(DevCounters, ntask2) == offsetof(DevCounters, ndel2) + 4 &&
+                  offsetof(DevCounters, ndel3) % 8 == 0 && offsetof(DevCounters, ntask3) == offsetof(DevCounters, ndel3) + 4,
+              "a deletion count and its task total form one aligned 64-bit pair");
 
 // Arguments of the support-count kernels.
 // Statistics every wave of a count launch adds to go to one of kStripes
@@ -174,11 +187,12 @@ hipError_t min_live_support(const uint32_t* dst, const uint32_t* sup, uint32_t E
 // (optional): live length of every oriented row, decremented at src[e] for
 // every deleted entry e.  *bad counts entries below the threshold that were
 // flagged already (none while the supports keep the rule above).
+struct MarkTasks;
 hipError_t mark_dying(uint32_t* dst, const uint32_t* eid, uint32_t* sup, uint32_t* tau,
                       uint32_t* dl, uint32_t n, uint32_t k, uint32_t* snbr, const uint32_t* sa,
                       const uint32_t* sb, unsigned int* ndel, const uint32_t* fin_dl,
                       const uint32_t* fin_nd, uint32_t fin_max, const uint32_t* src, uint32_t* ldeg,
-                      unsigned int* bad, hipStream_t s);
+                      unsigned int* bad, hipStream_t s, const MarkTasks* mt = nullptr);
 // out[v] = ptr[v + 1] - ptr[v] for v < nv.
 hipError_t row_lengths(const uint32_t* ptr, uint32_t nv, uint32_t* out, hipStream_t s);
 // Minimum support of the live entries (those neither dead nor dying: the
@@ -198,8 +212,27 @@ struct DeltaScratch {
     unsigned long long* bstat;  // per k_delta workgroup: elems, win, live, tris
     uint4* meta;        // per deletion: short list [x, y), long list [z, z + w)
     uint32_t* ntask;    // per deletion: first task counted from its tile's first
+                        // (delta_prepare), or its first task (a listing mark)
     uint32_t* tsum;     // per 2048-deletion tile, plus one
+    uint4* mtmp;        // a listing mark's staged descriptors and task counts,
+    uint32_t* ttmp;     // one buffer of list slots per mark workgroup
     uint32_t cap;
+};
+// A mark pass that lists the DELTA tasks of its deletions itself (mark_dying
+// with MarkTasks): each deletion's descriptor and task count are made when it
+// is flagged (both endpoints' list starts) and staged per workgroup; a
+// workgroup's flush reserves its list slots and its task range with one
+// 64-bit atomic on the list's (count, task total) pair, so delta_apply with
+// DirectTasks needs no task-list kernel in between.
+struct MarkTasks {
+    unsigned long long* pair;  // the list's (deletion count, task total), zero on entry
+    const uint32_t* sptr;      // adjacency list starts
+    DeltaScratch ws;           // meta / ntask receive the list's descriptors and first tasks
+};
+struct DirectTasks {
+    const uint32_t* ntask;           // the list's task total (the pair's high half)
+    unsigned long long* zero_pair;   // optional: another list's pair, cleared by the update
+    uint32_t* host_cnt;              // optional: device address of pinned memory, receives the count
 };
 size_t delta_scratch_bytes(uint32_t cap);
 void delta_scratch_carve(void* base, uint32_t cap, DeltaScratch& s);
@@ -253,7 +286,7 @@ hipError_t delta_apply(const uint32_t* dl, const uint32_t* nd_dev, uint32_t nd_m
                        const uint32_t* snbr, const uint32_t* seid, uint32_t* sup,
                        const DeltaScratch& ws, DevCounters* ctr, hipStream_t s,
                        uint32_t thr = 0, uint32_t* next = nullptr, unsigned int* next_cnt = nullptr,
-                       const uint32_t* gsmp = nullptr);
+                       const uint32_t* gsmp = nullptr, const DirectTasks* dt = nullptr);
 // gsmp[b] = id of adjacency slot 64 b (flags masked), b < ceil(S / 64): the
 // block samples delta_apply (gsmp) stages contiguously for long lists; to be
 // rebuilt whenever the lists are rebuilt.

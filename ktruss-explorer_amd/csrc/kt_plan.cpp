@@ -1116,21 +1116,31 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
     auto queue = [&](int s) -> int {
         const uint32_t in = r % 3, out = (r + 1) % 3;
         const bool light = std::max(last_nd, prev_nd) <= kLightDeletions;
+        // a list a mark made carries its tasks: the update reads them
+        // directly and takes over the task-list kernel's other duties
+        // (clearing the next list's count, publishing this one's)
+        const bool direct = !light && !unflagged;
         __atomic_store_n(h_ra + s, kCountPending, __ATOMIC_RELEASE);  // the successor overwrites it
-        int rc = prepare(r, last_nd, counts[out], prev_slot >= 0 ? d_ra + prev_slot : nullptr);
-        if (rc) return rc;
+        uint32_t* host_cnt = prev_slot >= 0 ? d_ra + prev_slot : nullptr;
+        if (!direct) {
+            const int rc = prepare(r, last_nd, counts[out], host_cnt);
+            if (rc) return rc;
+        }
+        const dev::DirectTasks dt{counts[in] + 1, (unsigned long long*)counts[out], host_cnt};
         if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][0], stream));
         KT_HIP_TRY(dev::delta_apply(lists[in], counts[in], bound, snbr_b[acur].as<uint32_t>(),
                                     seid_b[acur].as<uint32_t>(), sup[cur].as<uint32_t>(), dws, c, stream,
                                     light ? k - 2 : 0u, light ? lists[out] : nullptr,
-                                    light ? counts[out] : nullptr, gsmp.as<uint32_t>()));
+                                    light ? counts[out] : nullptr, gsmp.as<uint32_t>(), direct ? &dt : nullptr));
         if (timing) KT_HIP_TRY(hipEventRecord(ev_ra[s][1], stream));
-        if (!light)
+        if (!light) {
+            const dev::MarkTasks mt{(unsigned long long*)counts[out], sptr_b[acur].as<uint32_t>(), dws};
             KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                        sup[cur].as<uint32_t>(), tau.as<uint32_t>(), lists[out], E, k,
                                        snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                        counts[out], lists[in], counts[in], bound, src[cur].as<uint32_t>(),
-                                       live_rows_ptr(), &c->mark_bad, stream));
+                                       live_rows_ptr(), &c->mark_bad, stream, &mt));
+        }
         unflagged = light;
         prev_slot = s;
         ++r;
@@ -1233,6 +1243,9 @@ int Plan::run_ahead_delta(uint32_t k, uint32_t& iter, double& phase_ms, bool& en
 // compacted away once they are a third of the stored ones.
 int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
     begin_phase();
+    // marks into dl / ctr->ndel list the DELTA tasks of their deletions (over
+    // the adjacency lists current at the mark)
+    dev::MarkTasks mt0{(unsigned long long*)&ctr.as<DevCounters>()->ndel, nullptr, dws};
     const bool jump = (schedule & KT_SCHED_JUMP) != 0;
     uint32_t iter = 0;
     bool broke = false, changed = false;
@@ -1281,11 +1294,12 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
         uint32_t mv = 0;
         if ((rc = poll_pinned(kSlotMin, mv))) return rc;
         if (mv != dev::kNoLiveSupport && mv + 3 > k) k = mv + 3;
+        mt0.sptr = sptr_b[acur].as<uint32_t>();
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(), sup[cur].as<uint32_t>(),
                                    tau.as<uint32_t>(), dl.as<uint32_t>(), E, k, snbr_b[acur].as<uint32_t>(),
                                    sa.as<uint32_t>(), sb.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, nullptr,
                                    nullptr, 0, src[cur].as<uint32_t>(), live_rows_ptr(),
-                                   &ctr.as<DevCounters>()->mark_bad, stream));
+                                   &ctr.as<DevCounters>()->mark_bad, stream, &mt0));
         if (unresolved) {
             uint32_t ch = 0;
             if ((rc = poll_pinned(kSlotChoice, ch))) return rc;
@@ -1324,7 +1338,9 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             counted = true;
             stats.count_passes++;
         }
-        if (!(iter == 0 && start_clear) && (rc = set_counter(offsetof(DevCounters, ndel), 0))) return rc;
+        if (!(iter == 0 && start_clear) &&
+            ((rc = set_counter(offsetof(DevCounters, ndel), 0)) || (rc = set_counter(offsetof(DevCounters, ntask), 0))))
+            return rc;
         if ((rc = compact_delta_state())) return rc;
         if (iter == 0 && jump) {
             KT_HIP_TRY(dev::min_live_delta(E, sup[cur].as<uint32_t>(), ctr.as<DevCounters>(), stream));
@@ -1337,23 +1353,25 @@ int Plan::run_phase_delta(uint32_t& k, bool reuse, uint32_t schedule) {
             // with the next iteration's
             if (s != 0xFFFFFFFFu && k > 3 && !counted && pick_now != KT_PICK_SWEEP && !unresolved &&
                 cfg.kernel == KT_COUNT_DELTA && !compaction_due()) {
+                mt0.sptr = sptr_b[acur].as<uint32_t>();
                 KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                            sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E,
                                            k, snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(),
                                            sb.as<uint32_t>(), &ctr.as<DevCounters>()->ndel, nullptr, nullptr,
                                            0, src[cur].as<uint32_t>(), live_rows_ptr(),
-                                   &ctr.as<DevCounters>()->mark_bad, stream));
+                                   &ctr.as<DevCounters>()->mark_bad, stream, &mt0));
                 bool ended = false;
                 if ((rc = run_ahead_delta(k, iter, phase_ms, ended, true))) return rc;
                 changed = !ended;
                 continue;
             }
         }
+        mt0.sptr = sptr_b[acur].as<uint32_t>();
         KT_HIP_TRY(dev::mark_dying(dst[cur].as<uint32_t>(), eid[cur].as<uint32_t>(),
                                    sup[cur].as<uint32_t>(), tau.as<uint32_t>(), dl.as<uint32_t>(), E, k,
                                    snbr_b[acur].as<uint32_t>(), sa.as<uint32_t>(), sb.as<uint32_t>(),
                                    &ctr.as<DevCounters>()->ndel, nullptr, nullptr, 0, src[cur].as<uint32_t>(),
-                                   live_rows_ptr(), &ctr.as<DevCounters>()->mark_bad, stream));
+                                   live_rows_ptr(), &ctr.as<DevCounters>()->mark_bad, stream, &mt0));
         if ((rc = sync_counters())) return rc;
         if (unresolved) resolve_choice();
         float count_ms = 0;
