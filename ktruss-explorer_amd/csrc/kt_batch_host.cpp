@@ -315,6 +315,7 @@ int Batch::prestep(hipEvent_t c0, hipEvent_t c1) {
     a.src = src.as<uint32_t>();
     a.dst = dst.as<uint32_t>();
     a.sup = sup.as<uint32_t>();
+    a.nitems = E;  // the layout's size (the kernel walks task lists): picks the count kernel
     a.chunk = tune.chunk;
     a.cap_lds = cap;
     a.ytiles = 1;
@@ -340,6 +341,7 @@ int Batch::launch_step(hipEvent_t c0, hipEvent_t c1) {
     a.src = src.as<uint32_t>();
     a.dst = dst.as<uint32_t>();
     a.sup = sup.as<uint32_t>();
+    a.nitems = E;  // the layout's size (the kernel walks task lists): picks the count kernel
     a.chunk = tune.chunk;
     a.cap_lds = cap;
     a.ytiles = 1;
