@@ -552,7 +552,7 @@ constexpr uint32_t kDeltaGrid = 2048;  // most k_delta workgroups per launch
 constexpr size_t kStatBytes = (size_t)kDeltaGrid * 4 * 8;
 static_assert(8192 / (kDBlock / kWave) <= kDeltaGrid, "k_delta grids of 8192 waves need a slot per workgroup");
 constexpr uint32_t kSample = 512;
-constexpr uint32_t kGather = kWave / 8;  // blocks gathered per round trip (8 lanes each)
+constexpr uint32_t kGather = 8;  // blocks gathered per round trip (two 16-byte loads per lane)
 static_assert(kGather * kWave <= kStage, "gathered blocks fit the slice");
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -633,18 +633,20 @@ __device__ __forceinline__ uint32_t count_less_lds(const uint32_t* s, uint32_t n
 
 // Numbers of samples below x0 and below x1 in s[0, n) (n <= kSample, x0 and
 // x1 the same in every lane), counted lane-parallel: lane l compares
-// s[8l, 8l + 8) (two 16-byte reads) and the ballots are summed -- one LDS
-// round trip instead of two dependent binary searches.
+// s[4l, 4l + 4) and s[256 + 4l, 256 + 4l + 4) (two 16-byte reads, lanes on
+// consecutive 16-byte chunks, so no bank conflicts) and the ballots are
+// summed -- one LDS round trip instead of two dependent binary searches.
+static_assert(kSample == 8 * kWave, "two 16-byte reads per lane cover the samples");
 __device__ __forceinline__ void count_less2_lds(const uint32_t* s, uint32_t n, uint32_t x0, uint32_t x1,
                                                 uint32_t lane, uint32_t& c0, uint32_t& c1) {
-    const uint32_t b = 8u * lane;
+    const uint32_t b = 4u * lane;
     const uint4 lo = *(const uint4*)(s + b);
-    const uint4 hi = *(const uint4*)(s + b + 4);
+    const uint4 hi = *(const uint4*)(s + 4 * kWave + b);
     const uint32_t v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
     c0 = c1 = 0;
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) {
-        const bool in = b + j < n;
+        const bool in = (j < 4 ? b + j : 4 * kWave + b + (j - 4)) < n;
         c0 += (uint32_t)__popcll(ballot(in && v[j] < x0));
         c1 += (uint32_t)__popcll(ballot(in && v[j] < x1));
     }
@@ -872,16 +874,20 @@ __global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_
                 p = 0;
                 y = KT_DEAD;
                 for (uint32_t g0 = 0; g0 < D; g0 += kGather) {
-                    // lanes 8q .. 8q + 7 load block g0 + q, eight slots each
-                    // from its 64-aligned start (the lists are padded to it)
-                    const uint32_t q = lane >> 3, bq = g0 + q < D ? gl[g0 + q] : gl[g0];
-                    const uint32_t a = ((b0 + bq) << 6) + 8u * (lane & 7u);
-                    const uint4 lo4 = *(const uint4*)(snbr + a);
-                    const uint4 hi4 = *(const uint4*)(snbr + a + 4);
+                    // blocks g0 .. g0 + 7 from their 64-aligned starts (the
+                    // lists are padded to the last): lane l loads slots
+                    // 4 (l & 15) .. + 3 of block g0 + (l >> 4) and of block
+                    // g0 + 4 + (l >> 4), so each 16-byte store of the wave
+                    // covers 1 KB of the slice without bank conflicts
+                    const uint32_t q = lane >> 4, o = 4u * (lane & 15u);
+                    const uint32_t b1 = g0 + q < D ? gl[g0 + q] : gl[g0];
+                    const uint32_t b2 = g0 + 4 + q < D ? gl[g0 + 4 + q] : gl[g0];
+                    const uint4 lo4 = *(const uint4*)(snbr + ((b0 + b1) << 6) + o);
+                    const uint4 hi4 = *(const uint4*)(snbr + ((b0 + b2) << 6) + o);
                     acc_win += min(D - g0, kGather) * kWave;
                     wave_lds_sync();
-                    *(uint4*)(buf + q * kWave + 8u * (lane & 7u)) = lo4;
-                    *(uint4*)(buf + q * kWave + 8u * (lane & 7u) + 4) = hi4;
+                    *(uint4*)(buf + 4u * lane) = lo4;
+                    *(uint4*)(buf + 4u * kWave + 4u * lane) = hi4;
                     wave_lds_sync();
                     if (blk >= 0 && grp >= g0 && grp < g0 + kGather) {
                         // the block's slots of the list: [s0, s1) of its 64
