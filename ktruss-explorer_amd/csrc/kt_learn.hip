@@ -41,7 +41,7 @@ __host__ __device__ inline Layout layout(int C) {
 // 5 entries in rows > 1024 (stored rows), 6..10 support histogram bins,
 // 11 live entries.
 constexpr uint32_t kFeatBlocks = KT_FEAT_BLOCKS;
-constexpr uint32_t kFeatThreads = 256;
+constexpr uint32_t kFeatThreads = 1024;  // 16 waves per CU: the scan is latency-bound
 
 template <int N>
 __device__ void block_sum_store(unsigned long long (&v)[N], bool is_max_3,
