@@ -781,31 +781,40 @@ __global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_
         const uint32_t nb = gsmp ? 1u + ((m.z + m.w - 1) >> 6) - b0 : (m.w + kWave - 1) / kWave;
         const bool fresh = m.w <= kStage && full != i;
         const bool fresh_smp = m.w > kStage && nb <= kSample && sampled != i;
+        // One staging load site for the three cases (loads from two sites
+        // would meet in register copies, each waiting for its load), and the
+        // gsmp case's first id loaded beside it, stored as sample 0 (merged
+        // into the staged registers here, its select would wait at once).
         Staged<kStage> st;
-        if (fresh) {
-            st.load(snbr + m.z, m.w, 1u, lane);
-        } else if (fresh_smp) {
-            if (gsmp) {
-                st.load(gsmp + b0, nb, 1u, lane);
-                const uint32_t f0 = snbr[m.z];  // block b0 may start before the list
-                if (lane == 0) st.r[0] = f0;
-            } else {
-                st.load(snbr + m.z, nb, kWave, lane);
-            }
+        uint32_t f0 = 0;
+        if (fresh || fresh_smp) {
+            const uint32_t* from = fresh ? snbr + m.z : (gsmp ? gsmp + b0 : snbr + m.z);
+            st.load(from, fresh ? m.w : nb, fresh || gsmp ? 1u : kWave, lane);
+            f0 = snbr[m.z];
         }
-        const uint32_t x = pre ? xpre : (j < m.y ? snbr[j] : KT_DEAD);
-        pre = t < t1 && t < fnext;
-        if (pre) xpre = j + kWave < m.y ? snbr[j + kWave] : KT_DEAD;
+        // This task's candidates (prefetched by the task before when it has
+        // the same deletion).  Loads are unconditional and clamped to the
+        // list, the lanes past it masked here: a load under a lane condition,
+        // or a select on its value, is waited for at once.
+        const uint32_t x = pre ? xpre : snbr[min(j, m.y - 1)];
         if (kCross) cross();
-        const bool cand = !(x & KT_DEAD);
+        const bool cand = j < m.y && !(x & KT_DEAD);
         const uint64_t cm = ballot(cand);
         acc_live += (uint32_t)__popcll(cm);
+        // the next task's candidates, issued once this task's are in use (a
+        // load issued before that use would be waited for with them)
+        pre = t < t1 && t < fnext;
+        xpre = snbr[min(j + kWave, m.y - 1)];
         if (fresh) {
             st.store(buf, m.w, 0xFFFFFFFFu, lane);
             full = i;
             acc_win += m.w;
         } else if (fresh_smp) {
             st.store(smp, nb, KT_IDMASK, lane);
+            if (gsmp) {  // block b0 may start before the list: sample 0 is its first id
+                if (lane == 0) smp[0] = f0 & KT_IDMASK;
+                wave_lds_sync();
+            }
             sampled = i;
             acc_win += nb;
         }
