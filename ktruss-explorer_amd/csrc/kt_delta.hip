@@ -747,6 +747,7 @@ __global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_
     // task's search when they belong to the same deletion, so their round trip
     // overlaps this task's slot->entry reads
     uint4 m = make_uint4(0, 0, 0, 0);
+    uint32_t dcur = 0;  // deletion i's entry (the ownership test's)
     uint32_t fi = 0, fnext = 0xFFFFFFFFu;
     uint32_t xpre = KT_DEAD;
     bool pre = false;  // xpre holds task t's candidates (wave-uniform)
@@ -754,6 +755,7 @@ __global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_
         fi = first_task(lofs, tsum, i);
         fnext = i + 1 < nd ? first_task(lofs, tsum, i + 1) : 0xFFFFFFFFu;
         m = meta[i];
+        dcur = dl[i];
     }
     for (uint32_t t = t0; t < t1;) {
         if (t >= fnext) {
@@ -763,6 +765,7 @@ __global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_
                 fnext = i + 1 < nd ? first_task(lofs, tsum, i + 1) : 0xFFFFFFFFu;
             }
             m = meta[i];
+            dcur = dl[i];
             pre = false;
         }
         const uint32_t tc = t++;  // this task; the body may continue below
@@ -919,12 +922,12 @@ __global__ __launch_bounds__(kDBlock, kCross ? 7 : 8) void k_delta(const uint32_
         if (hit) {
             const uint32_t e1 = seid[j], e2 = seid[p];
             const bool y1 = (x & KT_DYING) != 0, y2 = (y & KT_DYING) != 0;
-            // the deleted edge of lowest index owns the triangle
-            bool own = true;
-            if (y1 || y2) {
-                const uint32_t d = dl[i];
-                own = !((y1 && e1 < d) || (y2 && e2 < d));
-            }
+            // the deleted edge of lowest index owns the triangle; both
+            // entries are compared in every hit lane, so both loads are
+            // consumed before the decrements (a load left unconsumed where a
+            // decrement is skipped is waited for later, together with the
+            // decrements issued after it)
+            const bool own = !((y1 & (e1 < dcur)) | (y2 & (e2 < dcur)));
             if (own) {
                 if (kCross) {
                     pe1 = e1;
