@@ -79,6 +79,9 @@ __device__ __forceinline__ void finalize_one(uint32_t d, uint32_t* dst, uint32_t
 }
 
 constexpr uint32_t kMarkItems = 8;
+// a listed deletion that lies in no triangle (support 0): flag bit of its
+// LDS list word (entries stay below 2^31: DELTA plans have fewer)
+constexpr uint32_t kMarkNoTri = 0x80000000u;
 constexpr uint32_t kMarkTile = kBlock * kMarkItems;
 constexpr uint32_t kMarkBuf = 2 * kMarkTile;
 constexpr uint32_t kMarkGrid = 1024;
@@ -125,7 +128,7 @@ __device__ __forceinline__ void mark_flush(const uint32_t* s_out, uint32_t* s_cn
     if (threadIdx.x == 0) *s_base = atomicAdd(ndel, c);
     __syncthreads();
     const uint32_t base = *s_base;
-    for (uint32_t i = threadIdx.x; i < c; i += kBlock) dl[base + i] = s_out[i];
+    for (uint32_t i = threadIdx.x; i < c; i += kBlock) dl[base + i] = s_out[i] & ~kMarkNoTri;
     __syncthreads();
     if (threadIdx.x == 0) *s_cnt = 0;
     __syncthreads();
@@ -167,7 +170,7 @@ __device__ __forceinline__ void mark_flush_tasks(const uint32_t* s_out, uint32_t
     const uint32_t base = s_res[0];
     uint32_t ft = s_res[1] + run;
     for (uint32_t r = 0; r < kFlushPer && i0 + r < c; ++r) {  // the counts again (cached)
-        dl[base + i0 + r] = s_out[i0 + r];
+        dl[base + i0 + r] = s_out[i0 + r] & ~kMarkNoTri;
         mt.ws.ntask[base + i0 + r] = ft;
         ft += mt.ws.ttmp[stage + i0 + r];
     }
@@ -204,6 +207,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_mark_dying(uint32_t* __restrict__
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t ntile = (n + kMarkTile - 1) / kMarkTile;
+    uint32_t c_prev = 0;  // s_cnt before the tile (uniform)
     for (uint32_t tile = blockIdx.x; tile < ntile; tile += mark_blocks) {
         const uint32_t e0 = tile * kMarkTile + threadIdx.x;
         uint32_t sp[kMarkItems];
@@ -213,6 +217,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_mark_dying(uint32_t* __restrict__
             const uint32_t x = sup[e < n ? e : n - 1];  // unconditional: the loads stay in flight together
             sp[j] = e < n ? x : KT_SUP_GONE;
         }
+        // the tile's deletions into the LDS list first (entry, and whether it
+        // lies in no triangle), then flagged from the list one per thread: a
+        // deletion's reads (its words, then its endpoints' list starts) are
+        // two round trips per tile instead of two per 64-entry chunk with one
 #pragma unroll
         for (uint32_t j = 0; j < kMarkItems; ++j) {
             const uint32_t e = e0 + j * kBlock;
@@ -222,28 +230,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_mark_dying(uint32_t* __restrict__
             uint32_t b = 0;
             if (lane == 0) b = atomicAdd(&s_cnt, (uint32_t)__popcll(m));  // LDS
             b = __shfl(b, 0);
-            if (del) {
-                const uint32_t slot = b + (uint32_t)__popcll(m & below);
-                s_out[slot] = e;
-                const uint32_t dv = dst[e], ei = eid[e], a = sa[e], c = sb[e], u = src[e];
-                const uint32_t vid = dv & KT_IDMASK;
-                if (dv & (KT_DEAD | KT_DYING)) atomicAdd(bad, 1u);
-                sup[e] = sp[j] ? KT_SUP_GONE : KT_SUP_NONE;  // NONE: in no triangle, nothing to update
-                dst[e] = vid | KT_DYING;
-                tau[ei] = k - 1;
-                snbr[a] = vid | KT_DYING;
-                snbr[c] = u | KT_DYING;
-                if (ldeg) atomicSub(ldeg + u, 1u);
-                if (listing) {
-                    // the deletion's shorter and longer list and its tasks of
-                    // 64 candidates (none for a deletion in no triangle)
-                    const uint32_t ub = mt.sptr[u], ue = mt.sptr[u + 1], vb = mt.sptr[vid], ve = mt.sptr[vid + 1];
-                    const bool us = ue - ub <= ve - vb;
-                    const uint4 md = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
-                    mt.ws.mtmp[blockIdx.x * kMarkBuf + slot] = md;
-                    mt.ws.ttmp[blockIdx.x * kMarkBuf + slot] = sp[j] ? (md.y - md.x + kWave - 1) / kWave : 0u;
-                }
-            }
+            if (del) s_out[b + (uint32_t)__popcll(m & below)] = e | (sp[j] ? 0u : kMarkNoTri);
         }
         // every thread reads the count between two barriers: a wave that
         // went on to the next tile could otherwise add to it before a slower
@@ -251,9 +238,37 @@ __global__ __launch_bounds__(kBlock, 8) void k_mark_dying(uint32_t* __restrict__
         __syncthreads();
         const uint32_t c = s_cnt;
         __syncthreads();
+        for (uint32_t slot = c_prev + threadIdx.x; slot < c; slot += kBlock) {
+            const uint32_t w = s_out[slot];
+            const uint32_t e = w & ~kMarkNoTri;
+            const bool tri = !(w & kMarkNoTri);
+            const uint32_t dv = dst[e], ei = eid[e], a = sa[e], cc = sb[e], u = src[e];
+            const uint32_t vid = dv & KT_IDMASK;
+            if (dv & (KT_DEAD | KT_DYING)) atomicAdd(bad, 1u);
+            sup[e] = tri ? KT_SUP_GONE : KT_SUP_NONE;  // NONE: in no triangle, nothing to update
+            dst[e] = vid | KT_DYING;
+            tau[ei] = k - 1;
+            snbr[a] = vid | KT_DYING;
+            snbr[cc] = u | KT_DYING;
+            if (ldeg) atomicSub(ldeg + u, 1u);
+            if (listing) {
+                // the deletion's shorter and longer list and its tasks of
+                // 64 candidates (none for a deletion in no triangle)
+                const uint32_t ub = mt.sptr[u], ue = mt.sptr[u + 1], vb = mt.sptr[vid], ve = mt.sptr[vid + 1];
+                const bool us = ue - ub <= ve - vb;
+                const uint4 md = us ? make_uint4(ub, ue, vb, ve - vb) : make_uint4(vb, ve, ub, ue - ub);
+                mt.ws.mtmp[blockIdx.x * kMarkBuf + slot] = md;
+                mt.ws.ttmp[blockIdx.x * kMarkBuf + slot] = tri ? (md.y - md.x + kWave - 1) / kWave : 0u;
+            }
+        }
+        c_prev = c;
         if (c > kMarkBuf - kMarkTile) {
+            // the flush reads every thread's staged descriptors and task
+            // counts (global) and list words
+            __syncthreads();
             if (listing) mark_flush_tasks(s_out, &s_cnt, s_res, s_w, dl, mt);
             else mark_flush(s_out, &s_cnt, &s_base, dl, ndel);
+            c_prev = 0;
         }
     }
     __syncthreads();

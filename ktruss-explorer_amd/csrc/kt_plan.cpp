@@ -811,6 +811,8 @@ bool Plan::delta_capable() const {
 int Plan::ensure_delta() {
     if (delta_ready) return KT_OK;
     if (n >= KT_DYING) return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^30 vertices");
+    // adjacency slots (two per entry) are 32-bit indices
+    if ((uint64_t)E0 * 2 >= (1ull << 32)) return fail(KT_ERR_CAPACITY, "DELTA needs fewer than 2^31 entries");
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
     const size_t Eb = (size_t)std::max<uint32_t>(E0, 1) * 4, Pb = ((size_t)n + 1) * 4;
