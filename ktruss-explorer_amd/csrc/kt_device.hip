@@ -808,6 +808,7 @@ __global__ __launch_bounds__(1024) void k_count_flat(CountArgs a) {
 // BATCH: the kernel is persistent over a task list of many graphs stored as
 // one disjoint union; each wave walks tasks wave_id, wave_id + W, ... and a
 // task is `chunk` consecutive entries of one graph.
+constexpr uint32_t kCountAhead = 2;  // window chunks queued per wave (loads in flight)
 template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a, TaskList tl) {
     extern __shared__ uint32_t lds[];
@@ -880,32 +881,62 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a, TaskList tl) {
         }
         uint64_t todo = ballot(hi > lo);
         if (!todo) continue;
-        // current (item, chunk)
-        uint32_t l = (uint32_t)__ffsll((long long)todo) - 1;
+        // The batch's (item, 64-entry chunk of v's window) pairs in order,
+        // kCountAhead of them queued with their window loads in flight, so a
+        // chunk's load arrives while the chunk before it is searched.  The
+        // slots rotate in place: the next chunk's entries held in the
+        // register the load went to (copied to the current chunk's, the
+        // copy waited for every load before it, the prefetch just issued
+        // included, so no search overlapped a load).  Two slots: R-MAT 22's
+        // first count 104.7 -> 99.6 ms; three or four, 139 ms.  Slot q:
+        // item lane ql, first window slot qj, its entries qw.
+        uint32_t pl = (uint32_t)__ffsll((long long)todo) - 1;  // the item being cut into chunks
         todo &= todo - 1;
-        uint32_t cu = readlane(u, l), ce = readlane(e, l), cvb = readlane(vb, l);
-        uint32_t chi = readlane(hi, l), j0 = readlane(lo, l);
-        uint32_t w = j0 + lane < chi ? dst[cvb + j0 + lane] : KT_DEAD;
-        uint32_t c = 0;
-        while (true) {
-            // issue the next chunk's load before searching this one
-            const bool same = j0 + kWave < chi;
-            uint32_t nl = l, nj0 = j0 + kWave, nu = cu, ne = ce, nvb = cvb, nhi = chi;
-            bool more = same;
-            if (!same && todo) {
-                nl = (uint32_t)__ffsll((long long)todo) - 1;
-                nu = readlane(u, nl);
-                ne = readlane(e, nl);
-                nvb = readlane(vb, nl);
-                nhi = readlane(hi, nl);
-                nj0 = readlane(lo, nl);
-                more = true;
+        uint32_t pj = readlane(lo, pl), phi = readlane(hi, pl), pvb = readlane(vb, pl);
+        bool pdone = false;
+        auto produce = [&](uint32_t& ql, uint32_t& qj, uint32_t& qw, bool& qv) {
+            qv = !pdone;
+            ql = pl;
+            qj = pj;
+            // unconditional, clamped to the window (a load under a condition,
+            // or a select on its result, waits for it here); lanes past the
+            // window are masked when the chunk is searched
+            qw = dst[pvb + min(pj + lane, phi - 1)];
+            if (pdone) return;
+            pj += kWave;
+            if (pj >= phi) {
+                if (todo) {
+                    pl = (uint32_t)__ffsll((long long)todo) - 1;
+                    todo &= todo - 1;
+                    pj = readlane(lo, pl);
+                    phi = readlane(hi, pl);
+                    pvb = readlane(vb, pl);
+                } else {
+                    pdone = true;
+                }
             }
-            const uint32_t nw_ = more && nj0 + lane < nhi ? dst[nvb + nj0 + lane] : KT_DEAD;
-
+        };
+        static_assert(kCountAhead == 2, "two queue slots below");
+        uint32_t l0, l1, j0, j1, w0, w1;
+        bool v0, v1;
+        produce(l0, j0, w0, v0);
+        produce(l1, j1, w1, v1);
+        uint32_t c = 0;
+        // the oldest slot's chunk is searched after its slot is refilled; the
+        // slots rotate in place (a register copy of an entry still in flight
+        // would wait for every load before it)
+        auto consume = [&](uint32_t& ql, uint32_t& qj, uint32_t& qw, bool& qv) {
+            const uint32_t cl = ql, cj = qj;
+            const uint32_t cu = readlane(u, cl), ce = readlane(e, cl), cvb = readlane(vb, cl);
+            const uint32_t chi = readlane(hi, cl);
+            const uint32_t w = cj + lane < chi ? qw : KT_DEAD;
+            produce(ql, qj, qw, qv);  // issued before this chunk's search
             if (cu != su) {
                 flush();
                 stage(cu);
+                // the rare path's loads and atomics all done, so the compiler
+                // keeps counting the queued chunk loads precisely
+                __builtin_amdgcn_s_waitcnt(0);
             }
             const uint32_t q = ce - sub;
             bool hit = false;
@@ -918,25 +949,22 @@ __global__ __launch_bounds__(1024) void k_count_dir(CountArgs a, TaskList tl) {
                 const uint32_t p = lower_bound_uniform(dst, sub + q + 1, sulen - q - 1, w & KT_VMASK);
                 hit = !(w & KT_DEAD) && p < sub + sulen && dst[p] == w;
                 if (hit) atomicAdd(sup + p, 1u);
+                __builtin_amdgcn_s_waitcnt(0);
             }
-            if (hit) atomicAdd(sup + cvb + j0 + lane, 1u);
+            if (hit) atomicAdd(sup + cvb + cj + lane, 1u);
             c += (uint32_t)__popcll(ballot(hit));
-            if (!same) {  // last chunk of this item: credit (u, v)
+            if (cj + kWave >= chi) {  // last chunk of this item: credit (u, v)
                 if (c && lane == 0) {
                     if (s_in_lds) lds_credit16_n(s_cnt, q, c);
                     else atomicAdd(sup + ce, c);
                 }
                 c = 0;
             }
-            if (!more) break;
-            if (!same) todo &= todo - 1;
-            l = nl;
-            cu = nu;
-            ce = ne;
-            cvb = nvb;
-            chi = nhi;
-            j0 = nj0;
-            w = nw_;
+        };
+        while (v0) {
+            consume(l0, j0, w0, v0);
+            if (!v1) break;
+            consume(l1, j1, w1, v1);
         }
     }
     };
