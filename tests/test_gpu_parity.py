@@ -163,6 +163,10 @@ def test_rmat_matches_golden_fixture(scale):
         assert hashlib.sha256(np.ascontiguousarray(tau, "<u4").tobytes()).hexdigest() == \
             gold["tau_sha256"], v.describe()
         assert st["k_max"] == gold["k_max"]
+        if v.count_kernel == K.COUNT_DELTA:
+            # every triangle is destroyed exactly once over the decomposition
+            # (a lost or doubled task list entry would miss or repeat some)
+            assert st["delta_triangles"] == gold["triangles"], v.describe()
     # the committed cost model's per-phase pick
     import learn
     model, configs = learn.load_model()
