@@ -446,7 +446,10 @@ int Plan::phase_pick(uint32_t k, bool exact, CountCfg& cfg, uint32_t& chosen, bo
     // until the run ends
     const size_t ph = phase_log.size();
     if (ph >= feat_cap) {
-        const uint32_t cap = std::max<uint32_t>(256, 2 * feat_cap);
+        // phases without a pick leave the buffer as it is, so row ph may lie
+        // more than one doubling beyond it
+        uint32_t cap = std::max<uint32_t>(256, 2 * feat_cap);
+        while (cap <= ph) cap *= 2;
         DBuf grown;
         int rc = grown.alloc((size_t)cap * KT_MLP_IN * sizeof(float));
         if (rc) return rc;
@@ -703,9 +706,12 @@ int Plan::decompose() {
 int Plan::flush_phase_features() {
     const size_t np = phase_log.size();
     if (!np || !feat_cap) return KT_OK;
-    std::vector<float> f(np * KT_MLP_IN);
+    // phases without a pick (a sampled SWEEP's fill phases) do not grow the
+    // buffer: it may hold fewer rows than the log, and only those have features
+    const size_t nrow = std::min<size_t>(np, feat_cap);
+    std::vector<float> f(nrow * KT_MLP_IN);
     KT_HIP_TRY(hipMemcpy(f.data(), feat_log.p, f.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < np; ++i)
+    for (size_t i = 0; i < nrow; ++i)
         if (feat_on[i])
             for (int j = 0; j < KT_MLP_IN; ++j) phase_log[i][4 + j] = f[i * KT_MLP_IN + j];
     return KT_OK;

@@ -114,6 +114,43 @@ def test_sampled_sweep_is_bit_exact(every, offset):
     assert np.all(ph[~swept][:, 20:23] == -1) and np.all(ph[~swept][:, 2] == -1)
 
 
+def _cliques(sizes):
+    """Disjoint cliques: every edge of K_s has truss number s."""
+    src, dst, tau = [], [], []
+    base = 0
+    for s in sizes:
+        i, j = np.triu_indices(s, 1)
+        src.append(i + base)
+        dst.append(j + base)
+        tau.append(np.full(len(i), s, np.uint32))
+        base += s
+    return base, np.concatenate(src).astype(np.uint32), np.concatenate(dst).astype(np.uint32), np.concatenate(tau)
+
+
+@pytest.mark.parametrize("every,offset", [(1 << 30, 0), (258, 257)])
+def test_sampled_sweep_beyond_256_phases(every, offset):
+    """A sampled SWEEP over more phases than the phase-feature buffer's first
+    size (256 rows): phases without a pick do not grow it, so the run's end
+    copies only the rows it holds, and a swept phase beyond it (phase 257,
+    the only one swept) grows it past its row.  Disjoint cliques of 3 .. 262
+    vertices peel in one phase per clique size (JUMP), and every edge of K_s
+    has truss number s."""
+    n, s, d, tau = _cliques(range(3, 263))
+    g = K.Graph(n, s, d, pairs=True)
+    p = K.Plan(g, K.default_variant())
+    p.set_candidates([K.count_config("row", 512), K.count_config("delta", 512)])
+    p.set_pick(K.PICK_SWEEP)
+    p.set_sweep_sampling(every, 1, offset)
+    p.decompose()
+    assert np.array_equal(p.truss_numbers(), tau)
+    ph = p.phase_log()
+    assert len(ph) > 258
+    swept = np.arange(len(ph)) % every == offset
+    assert swept.sum() == 1
+    assert np.all(ph[swept][:, 20:22] > 0) and np.all(ph[swept][:, 4 + 15] >= 0)
+    assert np.all(ph[~swept][:, 2] == -1) and np.all(ph[~swept][:, 4:22] == -1)
+
+
 def test_candidate_validation():
     p = K.Plan(_graph(8), K.default_variant())
     with pytest.raises(K.KtError):
