@@ -47,6 +47,13 @@ def test_rmat22_learned_delta_and_default_agree():
     assert sl["k_max"] == sd["k_max"] == int(tau.max())
     assert sl["iterations"] == sd["iterations"]
     del pl
+    # DELTA whose first count (row-staged at this size) has 64-entry slices:
+    # the rows longer than 64 entries are searched in HBM, not in LDS
+    pc = K.Plan(g, K.make_variant(count_kernel="delta", block_size=256, cap_lds=64))
+    pc.set_timing(False)
+    pc.decompose()
+    assert np.array_equal(pc.truss_numbers(), tau)
+    del pc
     p = K.Plan(g, K.default_variant())
     for k in (3, 40, int(tau.max())):
         es, ed, st = p.ktruss(k)
