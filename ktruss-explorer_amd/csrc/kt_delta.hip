@@ -277,9 +277,15 @@ __global__ __launch_bounds__(kBlock, 8) void k_mark_dying(uint32_t* __restrict__
 }
 
 // Minimum live support (the JUMP target); one atomic per workgroup.
-__global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCounters* ctr,
-                           uint32_t* host_min) {
-    __shared__ uint32_t s_min[kBlock / kWave];
+constexpr uint32_t kMinThreads = 1024;
+// Its end -- one atomic and one ticket per workgroup on single addresses --
+// costs more than the scan: 128 workgroups of 1024 threads against 256 of
+// 256 took R-MAT 20 / 22 from 91.9 / 366.0 to 91.0 / 364.2 ms; 1024 and 2048
+// workgroups of 256, 94.4 / 371.5 and 99.2 / 378.8.
+constexpr uint32_t kMinGrid = 128;
+__global__ __launch_bounds__(kMinThreads) void k_min_list(uint32_t n, const uint32_t* __restrict__ sup,
+                                                          DevCounters* ctr, uint32_t* host_min) {
+    __shared__ uint32_t s_min[kMinThreads / kWave];
     __shared__ uint32_t s_last;
     uint32_t m = 0xFFFFFFFFu;
     // eight supports per thread in flight (indices past the end read entry
@@ -306,7 +312,7 @@ __global__ void k_min_list(uint32_t n, const uint32_t* __restrict__ sup, DevCoun
     if ((threadIdx.x & (kWave - 1)) == 0) s_min[threadIdx.x / kWave] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < kBlock / kWave; ++w) m = s_min[w] < m ? s_min[w] : m;
+        for (uint32_t w = 1; w < blockDim.x / kWave; ++w) m = s_min[w] < m ? s_min[w] : m;
         if (m != 0xFFFFFFFFu) atomicMin(&ctr->min_sup, m);
     }
     if (!host_min) return;  // uniform
@@ -1188,9 +1194,9 @@ hipError_t phase_start(unsigned int* from, DevCounters* ctr, bool zero_ndel, boo
 hipError_t min_live_delta(uint32_t n, const uint32_t* sup, DevCounters* ctr, hipStream_t s,
                           uint32_t* host_min) {
     if (!n) return host_min ? hipErrorInvalidValue : hipSuccess;
-    const uint32_t b = blocks_for(n, kBlock);
-    // at most 256 workgroups: one atomic each on the one counter
-    hipLaunchKernelGGL(k_min_list, dim3(b < 256 ? b : 256), dim3(kBlock), 0, s, n, sup, ctr, host_min);
+    const uint32_t b = blocks_for(n, kMinThreads);
+    hipLaunchKernelGGL(k_min_list, dim3(b < kMinGrid ? b : kMinGrid), dim3(kMinThreads), 0, s, n, sup, ctr,
+                       host_min);
     return hipGetLastError();
 }
 
